@@ -1,0 +1,326 @@
+"""ctypes bindings for the oracle libraries.
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg import this module.
+
+* ``Oracle``     -- the C restatement (oracle/lib/libphsp_oracle.so)
+* ``Reference``  -- the reference harness built from the reference headers
+                    (oracle/_ref/libhydra_ref*.so); present only where it was
+                    built.
+
+Layouts: SoA event columns are numpy arrays of shape (4N, n) (row 4*i+c is
+component c of particle i); AoS rows are (n, N, 4).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "lib", "libphsp_oracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libhydra_ref.so")
+REF_NATIVE_LIB = os.path.join(HERE, "_ref", "libhydra_ref_native.so")
+
+ENGINES = {"squares3": 0, "squares4": 1, "philox": 2, "philox_long": 3,
+           "threefry": 4, "threefry_long": 5}
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_U64 = ctypes.c_uint64
+_I = ctypes.c_int
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def functor_params(fid, terms=None):
+    """Pack functor parameters: terms = [(M, G, cr, ci, a, b), ...]."""
+    if fid != 1:
+        return np.zeros(1)
+    flat = [float(len(terms))]
+    for t in terms:
+        flat.extend(float(x) for x in t)
+    return np.array(flat, dtype=np.float64)
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_LIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"oracle library not built: {path}")
+        L = ctypes.CDLL(path)
+        self.L = L
+        L.po_splitmix64.argtypes = [_P]
+        L.po_splitmix64.restype = _U64
+        L.po_squares3.argtypes = [_U64, _U64]
+        L.po_squares3.restype = ctypes.c_uint32
+        L.po_squares4.argtypes = [_U64, _U64]
+        L.po_squares4.restype = ctypes.c_uint32
+        for nm in ("po_philox2x64", "po_philox4x64", "po_threefry2x64", "po_threefry4x64"):
+            getattr(L, nm).argtypes = [ctypes.c_uint, _P, _P, _P]
+        L.po_rng_words.argtypes = [_I, _U64, _U64, _U64, _P]
+        L.po_uniform.argtypes = [_I, _U64]
+        L.po_uniform.restype = _D
+        L.po_params.argtypes = [_I, _D, _P, _P, _P]
+        L.po_mother_beta.argtypes = [_P, _P]
+        L.po_decay_events.argtypes = [_I, _I, _P, _D, _D, _P, _U64, _P, _U64, _P, _P]
+        L.po_generate.argtypes = [_I, _I, _P, _D, _D, _P, _U64, _U64, _U64, _P, _P, _I]
+        L.po_generate_mothers.argtypes = [_I, _I, _P, _D, _D, _U64, _U64, _U64, _P, _P, _P, _I]
+        L.po_functor.argtypes = [_I, _P, _I, _P]
+        L.po_functor.restype = _D
+        L.po_event_weight.argtypes = [_I, _P, _D, _I, _P, _P, _U64, _P, _I]
+        L.po_unweight.argtypes = [_I, _P, _D, _I, _P, _D, _P, _U64, _U64, _P, _P, _P, _I]
+        L.po_unweight.restype = _U64
+        L.po_average_on.argtypes = [_I, _I, _P, _D, _D, _P, _U64, _U64, _U64, _I, _P, _I, _P]
+        L.po_average_on_mothers.argtypes = [_I, _I, _P, _D, _D, _U64, _U64, _P, _I, _P, _I, _P]
+        L.po_stats_merge.argtypes = [_P, _P, _P]
+
+    # -- engines -----------------------------------------------------------
+    def splitmix64(self, x):
+        v = _U64(x)
+        r = self.L.po_splitmix64(ctypes.byref(v))
+        return r, v.value
+
+    def rng_words(self, engine, seed, discard, n):
+        out = np.zeros(n, dtype=np.uint64)
+        rc = self.L.po_rng_words(ENGINES.get(engine, engine), seed, discard, n, _ptr(out))
+        if rc:
+            raise ValueError("bad engine")
+        return out
+
+    def bijection(self, name, rounds, ctr, key):
+        ctr = np.asarray(ctr, dtype=np.uint64)
+        key = np.asarray(key, dtype=np.uint64)
+        out = np.zeros_like(ctr)
+        getattr(self.L, "po_" + name)(rounds, _ptr(ctr), _ptr(key), _ptr(out))
+        return out
+
+    def uniform(self, engine, word):
+        return self.L.po_uniform(ENGINES.get(engine, engine), int(word))
+
+    # -- decay set-up ------------------------------------------------------
+    def params(self, M, masses):
+        m = _f64(masses)
+        ecm, mw = _D(), _D()
+        rc = self.L.po_params(len(m), M, _ptr(m), ctypes.byref(ecm), ctypes.byref(mw))
+        return rc, ecm.value, mw.value
+
+    def mother_beta(self, mother4):
+        m4 = _f64(mother4)
+        b = np.zeros(3)
+        self.L.po_mother_beta(_ptr(m4), _ptr(b))
+        return b
+
+    def decay_events(self, M, masses, seed, evts, engine=0, mother4=None):
+        m = _f64(masses)
+        N = len(m)
+        rc, ecm, mw = self.params(M, m)
+        assert rc == 0
+        beta = self.mother_beta(mother4 if mother4 is not None else [M, 0, 0, 0])
+        ev = np.ascontiguousarray(evts, dtype=np.uint64)
+        p = np.zeros((len(ev), N, 4))
+        wt = np.zeros(len(ev))
+        self.L.po_decay_events(N, ENGINES.get(engine, engine), _ptr(m), ecm, mw, _ptr(beta),
+                               seed, _ptr(ev), len(ev), _ptr(p), _ptr(wt))
+        return p, wt
+
+    def generate(self, M, masses, seed, n, first=0, engine=0, mother4=None, nthreads=0):
+        m = _f64(masses)
+        N = len(m)
+        rc, ecm, mw = self.params(M, m)
+        assert rc == 0
+        beta = self.mother_beta(mother4 if mother4 is not None else [M, 0, 0, 0])
+        cols = np.zeros((4 * N, n))
+        wt = np.zeros(n)
+        self.L.po_generate(N, ENGINES.get(engine, engine), _ptr(m), ecm, mw, _ptr(beta), seed,
+                           first, n, _ptr(cols), _ptr(wt), nthreads)
+        return cols, wt
+
+    def generate_mothers(self, M, masses, seed, mothers, first=0, engine=0, nthreads=0):
+        m = _f64(masses)
+        N = len(m)
+        rc, ecm, mw = self.params(M, m)
+        assert rc == 0
+        mc = _f64(mothers)  # (4, n)
+        n = mc.shape[1]
+        cols = np.zeros((4 * N, n))
+        wt = np.zeros(n)
+        self.L.po_generate_mothers(N, ENGINES.get(engine, engine), _ptr(m), ecm, mw, seed, first,
+                                   n, _ptr(mc), _ptr(cols), _ptr(wt), nthreads)
+        return cols, wt
+
+    def functor(self, fid, fpar, p):
+        p = _f64(p)
+        fp = _f64(fpar)
+        return self.L.po_functor(fid, _ptr(fp), p.shape[0], _ptr(p))
+
+    def event_weight(self, M, masses, cols, fid=-1, fpar=None, nthreads=0):
+        m = _f64(masses)
+        N = len(m)
+        rc, ecm, mw = self.params(M, m)
+        c = _f64(cols)
+        n = c.shape[1]
+        w = np.zeros(n)
+        fp = _f64(fpar if fpar is not None else np.zeros(1))
+        self.L.po_event_weight(N, _ptr(m), mw, fid, _ptr(fp), _ptr(c), n, _ptr(w), nthreads)
+        return w
+
+    def unweight(self, M, masses, cols, seed, fid=-1, fpar=None, max_value=-1.0, nthreads=0):
+        m = _f64(masses)
+        N = len(m)
+        rc, ecm, mw = self.params(M, m)
+        c = _f64(cols)
+        n = c.shape[1]
+        out = np.zeros_like(c)
+        idx = np.zeros(n, dtype=np.uint64)
+        flags = np.zeros(n, dtype=np.uint8)
+        fp = _f64(fpar if fpar is not None else np.zeros(1))
+        nacc = self.L.po_unweight(N, _ptr(m), mw, fid, _ptr(fp), max_value, _ptr(c), n, seed,
+                                  _ptr(out), _ptr(idx), _ptr(flags), nthreads)
+        return nacc, out, idx, flags
+
+    def average_on(self, M, masses, seed, n, fid=0, fpar=None, first=0, engine=0,
+                   mother4=None, intervals=1):
+        """Returns {W, mean, M2} with the OMP fold order for `intervals`."""
+        m = _f64(masses)
+        N = len(m)
+        rc, ecm, mw = self.params(M, m)
+        assert rc == 0
+        beta = self.mother_beta(mother4 if mother4 is not None else [M, 0, 0, 0])
+        fp = _f64(fpar if fpar is not None else np.zeros(1))
+        out = np.zeros(3)
+        self.L.po_average_on(N, ENGINES.get(engine, engine), _ptr(m), ecm, mw, _ptr(beta), seed,
+                             first, n, fid, _ptr(fp), intervals, _ptr(out))
+        return out
+
+    def average_on_mothers(self, M, masses, seed, mothers, fid=0, fpar=None, engine=0,
+                           intervals=1):
+        m = _f64(masses)
+        N = len(m)
+        rc, ecm, mw = self.params(M, m)
+        mc = _f64(mothers)
+        fp = _f64(fpar if fpar is not None else np.zeros(1))
+        out = np.zeros(3)
+        self.L.po_average_on_mothers(N, ENGINES.get(engine, engine), _ptr(m), ecm, mw, seed,
+                                     mc.shape[1], _ptr(mc), fid, _ptr(fp), intervals, _ptr(out))
+        return out
+
+    def stats_merge(self, x, y):
+        x = _f64(x)
+        y = _f64(y)
+        o = np.zeros(3)
+        self.L.po_stats_merge(_ptr(x), _ptr(y), _ptr(o))
+        return o
+
+
+class Reference:
+    """The reference's own templates, compiled from its headers (OMP backend)."""
+
+    def __init__(self, path=REF_LIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"reference harness not built: {path}")
+        L = ctypes.CDLL(path)
+        self.L = L
+        L.hr_set_threads.argtypes = [_I]
+        L.hr_rng_words.argtypes = [_I, _U64, _U64, _U64, _P]
+        L.hr_phsp_params.argtypes = [_I, _D, _P, _P, _P, _P, _P]
+        L.hr_decay_events.argtypes = [_I, _I, _D, _P, _P, _U64, _P, _U64, _P, _P]
+        L.hr_generate.argtypes = [_I, _D, _P, _P, _I, ctypes.c_int32, _U64, _P, _P, _P]
+        L.hr_generate_mothers.argtypes = [_I, _D, _P, ctypes.c_int32, _U64, _P, _P, _P]
+        L.hr_unweight.argtypes = [_I, _D, _P, _P, ctypes.c_int32, _U64, _I, _I, _P, _D, _U64,
+                                  _P, _P, _P, _P]
+        L.hr_average_on.argtypes = [_I, _D, _P, _P, _I, ctypes.c_int32, _U64, _I, _P, _P, _P,
+                                    _P]
+
+    def set_threads(self, n):
+        return self.L.hr_set_threads(n)
+
+    def rng_words(self, engine, seed, discard, n):
+        out = np.zeros(n, dtype=np.uint64)
+        self.L.hr_rng_words(ENGINES.get(engine, engine), seed, discard, n, _ptr(out))
+        return out
+
+    def params(self, M, masses):
+        m = _f64(masses)
+        ecm, mw, sf, sg = _D(), _D(), _U64(), ctypes.c_int32()
+        rc = self.L.hr_phsp_params(len(m), M, _ptr(m), ctypes.byref(ecm), ctypes.byref(mw),
+                                   ctypes.byref(sf), ctypes.byref(sg))
+        return rc, ecm.value, mw.value, sf.value, sg.value
+
+    def decay_events(self, M, masses, seed, evts, engine=0, mother4=None):
+        m = _f64(masses)
+        N = len(m)
+        m4 = _f64(mother4 if mother4 is not None else [M, 0, 0, 0])
+        ev = np.ascontiguousarray(evts, dtype=np.uint64)
+        p = np.zeros((len(ev), N, 4))
+        wt = np.zeros(len(ev))
+        self.L.hr_decay_events(N, ENGINES.get(engine, engine), M, _ptr(m4), _ptr(m), seed,
+                               _ptr(ev), len(ev), _ptr(p), _ptr(wt))
+        return p, wt
+
+    def generate(self, M, masses, seed, n, mother4=None, want_output=True):
+        """Returns (rows (n,N,4), weights, seconds); seed=None uses the default seed."""
+        m = _f64(masses)
+        N = len(m)
+        m4 = _f64(mother4 if mother4 is not None else [M, 0, 0, 0])
+        p = np.zeros((n, N, 4)) if want_output else None
+        w = np.zeros(n) if want_output else None
+        sec = _D()
+        use = 0 if seed is None else 1
+        self.L.hr_generate(N, M, _ptr(m4), _ptr(m), use, 0 if seed is None else seed, n,
+                           _ptr(p), _ptr(w), ctypes.byref(sec))
+        return p, w, sec.value
+
+    def generate_mothers(self, M, masses, seed, mothers):
+        m = _f64(masses)
+        N = len(m)
+        mr = np.ascontiguousarray(np.asarray(mothers, dtype=np.float64).T)  # (n,4)
+        n = mr.shape[0]
+        p = np.zeros((n, N, 4))
+        w = np.zeros(n)
+        self.L.hr_generate_mothers(N, M, _ptr(m), seed, n, _ptr(mr), _ptr(p), _ptr(w))
+        return p, w
+
+    def unweight(self, M, masses, seed, n, uw_seed, fid=None, fpar=None, max_weight=-1.0,
+                 mother4=None, want_output=True):
+        m = _f64(masses)
+        N = len(m)
+        m4 = _f64(mother4 if mother4 is not None else [M, 0, 0, 0])
+        p = np.zeros((n, N, 4)) if want_output else None
+        idx = np.zeros(n, dtype=np.uint64) if want_output else None
+        nacc = _U64()
+        sec = _D()
+        fp = _f64(fpar if fpar is not None else np.zeros(1))
+        rc = self.L.hr_unweight(N, M, _ptr(m4), _ptr(m), seed, n, 0 if fid is None else 1,
+                                0 if fid is None else fid, _ptr(fp), max_weight, uw_seed,
+                                _ptr(p), _ptr(idx), ctypes.byref(nacc), ctypes.byref(sec))
+        if rc:
+            raise RuntimeError(f"hr_unweight failed: {rc}")
+        return nacc.value, p, idx, sec.value
+
+    def average_on(self, M, masses, seed, n, fid=0, fpar=None, mother4=None):
+        m = _f64(masses)
+        N = len(m)
+        m4 = _f64(mother4 if mother4 is not None else [M, 0, 0, 0])
+        fp = _f64(fpar if fpar is not None else np.zeros(1))
+        mean, sig, sec = _D(), _D(), _D()
+        use = 0 if seed is None else 1
+        self.L.hr_average_on(N, M, _ptr(m4), _ptr(m), use, 0 if seed is None else seed, n, fid,
+                             _ptr(fp), ctypes.byref(mean), ctypes.byref(sig), ctypes.byref(sec))
+        return mean.value, sig.value, sec.value
+
+
+def rows_to_cols(p):
+    """(n, N, 4) AoS rows -> (4N, n) SoA columns."""
+    n, N, _ = p.shape
+    return np.ascontiguousarray(p.reshape(n, 4 * N).T)
+
+
+def cols_to_rows(c, N):
+    return np.ascontiguousarray(np.asarray(c).T.reshape(-1, N, 4))

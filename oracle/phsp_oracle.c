@@ -1,0 +1,702 @@
+/*
+ * CPU restatement of the Hydra 4.0.1 phase-space hot path (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP kernels in hydra_amd/.
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load
+ * the library built from this file; the product never links or calls it.
+ *
+ * Every function follows the reference's IEEE operation order (build with
+ * -ffp-contract=off) so that, compiled by the same gcc against the same
+ * glibc libm as the reference harness, it reproduces the reference bit for bit.
+ * Reference files are cited relative to the Hydra source root.
+ */
+#include "phsp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Types.h:75 -- PI rounds to M_PI in double */
+static const double PO_PI = 3.1415926535897932384626422832795028841971;
+
+/* ------------------------------------------------------------------------ */
+/* SplitMix64 (Steele, Lea, Flood 2014), as used for key derivation at
+ * hydra/detail/random/splitmix.h:54-61: pre-increment by the golden gamma,
+ * two xor-shift-multiply mixes, final xor-shift.                            */
+uint64_t po_splitmix64(uint64_t* x) {
+  uint64_t z = (*x += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+static inline uint64_t rot32(uint64_t x) { return (x >> 32) | (x << 32); }
+
+/* Widynski's counter-based middle-square generator, 3 rounds
+ * (hydra/detail/random/squares3.h:105-119; arXiv:2004.06278).               */
+uint32_t po_squares3(uint64_t key, uint64_t ctr) {
+  uint64_t x, y, z;
+  y = x = key * ctr;
+  z = y + key;
+  x = x * x + y;
+  x = rot32(x);
+  x = x * x + z;
+  x = rot32(x);
+  return (uint32_t)((x * x + y) >> 32);
+}
+
+/* 4-round variant (hydra/detail/random/squares4.h:106-122).                 */
+uint32_t po_squares4(uint64_t key, uint64_t ctr) {
+  uint64_t x, y, z;
+  y = x = key * ctr;
+  z = y + key;
+  x = x * x + y;
+  x = rot32(x);
+  x = x * x + z;
+  x = rot32(x);
+  x = x * x + y;
+  x = rot32(x);
+  return (uint32_t)((x * x + z) >> 32);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Random123 bijections (Salmon, Moraes, Dror, Shaw, SC'11), restated from the
+ * published algorithm; constants as in hydra/detail/external/hydra_R123/
+ * philox.h:223-258 and threefry.h:66-90,177-178.                           */
+static inline void mulhilo64(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
+  unsigned __int128 p = (unsigned __int128)a * (unsigned __int128)b;
+  *hi = (uint64_t)(p >> 64);
+  *lo = (uint64_t)p;
+}
+
+void po_philox2x64(unsigned R, const uint64_t ctr[2], const uint64_t key[1], uint64_t out[2]) {
+  uint64_t c0 = ctr[0], c1 = ctr[1], k = key[0];
+  for (unsigned r = 0; r < R; ++r) {
+    if (r > 0) k += 0x9E3779B97F4A7C15ULL;
+    uint64_t hi, lo;
+    mulhilo64(0xD2B74407B1CE6E93ULL, c0, &hi, &lo);
+    c0 = hi ^ k ^ c1;
+    c1 = lo;
+  }
+  out[0] = c0;
+  out[1] = c1;
+}
+
+void po_philox4x64(unsigned R, const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) {
+  uint64_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint64_t k0 = key[0], k1 = key[1];
+  for (unsigned r = 0; r < R; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B97F4A7C15ULL;
+      k1 += 0xBB67AE8584CAA73BULL;
+    }
+    uint64_t hi0, lo0, hi1, lo1;
+    mulhilo64(0xD2E7470EE14C6C93ULL, c0, &hi0, &lo0);
+    mulhilo64(0xCA5A826395121157ULL, c2, &hi1, &lo1);
+    uint64_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline uint64_t rotl64(uint64_t x, unsigned n) {
+  return (x << (n & 63)) | (x >> ((64 - n) & 63));
+}
+
+#define PO_SKEIN_PARITY 0x1BD11BDAA9FC1A22ULL
+
+void po_threefry2x64(unsigned R, const uint64_t ctr[2], const uint64_t key[2], uint64_t out[2]) {
+  static const unsigned rot[8] = {16, 42, 12, 31, 16, 32, 24, 21};
+  uint64_t ks[3];
+  ks[0] = key[0];
+  ks[1] = key[1];
+  ks[2] = PO_SKEIN_PARITY ^ key[0] ^ key[1];
+  uint64_t x0 = ctr[0] + ks[0], x1 = ctr[1] + ks[1];
+  for (unsigned r = 0; r < R; ++r) {
+    x0 += x1;
+    x1 = rotl64(x1, rot[r % 8]);
+    x1 ^= x0;
+    if ((r % 4) == 3) {
+      unsigned s = (r + 1) / 4;
+      x0 += ks[s % 3];
+      x1 += ks[(s + 1) % 3];
+      x1 += s;
+    }
+  }
+  out[0] = x0;
+  out[1] = x1;
+}
+
+void po_threefry4x64(unsigned R, const uint64_t ctr[4], const uint64_t key[4], uint64_t out[4]) {
+  static const unsigned rot[8][2] = {{14, 16}, {52, 57}, {23, 40}, {5, 37},
+                                     {25, 33}, {46, 12}, {58, 22}, {32, 32}};
+  uint64_t ks[5];
+  ks[4] = PO_SKEIN_PARITY;
+  for (int i = 0; i < 4; ++i) {
+    ks[i] = key[i];
+    ks[4] ^= key[i];
+  }
+  uint64_t x[4];
+  for (int i = 0; i < 4; ++i) x[i] = ctr[i] + ks[i];
+  for (unsigned r = 0; r < R; ++r) {
+    const unsigned* q = rot[r % 8];
+    if ((r % 2) == 0) {
+      x[0] += x[1]; x[1] = rotl64(x[1], q[0]); x[1] ^= x[0];
+      x[2] += x[3]; x[3] = rotl64(x[3], q[1]); x[3] ^= x[2];
+    } else {
+      x[0] += x[3]; x[3] = rotl64(x[3], q[0]); x[3] ^= x[0];
+      x[2] += x[1]; x[1] = rotl64(x[1], q[1]); x[1] ^= x[2];
+    }
+    if ((r % 4) == 3) {
+      unsigned s = (r + 1) / 4;
+      for (int i = 0; i < 4; ++i) x[i] += ks[(s + i) % 5];
+      x[3] += s;
+    }
+  }
+  for (int i = 0; i < 4; ++i) out[i] = x[i];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Engine objects with the reference's stream semantics.
+ *   squares3/4: key = splitmix64(seed) (squares3.h:81-84), counter from 0,
+ *               operator() uses then post-increments the counter,
+ *               discard(n) adds n (squares3.h:105-125).
+ *   EngineR123: key words are successive splitmix64(seed) draws
+ *               (EngineR123.h:97-112); counter block starts at 0 and is
+ *               pre-incremented (with carry, array.h:151-183) before each
+ *               block evaluation; words are served from the cached block;
+ *               discard(n) adds n blocks and drops the cache (EngineR123.h:148-173).
+ */
+typedef struct {
+  int engine;
+  unsigned arity;
+  uint64_t key[4];
+  uint64_t ctr[4];
+  uint64_t cache[4];
+  unsigned trig;
+} po_engine;
+
+static unsigned engine_arity(int e) {
+  switch (e) {
+    case 2: case 4: return 2;
+    case 3: case 5: return 4;
+    default: return 1;
+  }
+}
+
+static void engine_init(po_engine* g, int engine, uint64_t seed) {
+  memset(g, 0, sizeof(*g));
+  g->engine = engine;
+  g->arity = engine_arity(engine);
+  uint64_t s = seed;
+  if (engine == 0 || engine == 1) {
+    g->key[0] = po_splitmix64(&s);
+  } else {
+    unsigned nkey = (engine == 2) ? 1 : (engine == 3) ? 2 : (engine == 4) ? 2 : 4;
+    for (unsigned i = 0; i < nkey; ++i) g->key[i] = po_splitmix64(&s);
+    g->trig = g->arity;
+  }
+}
+
+static void ctr_incr(uint64_t* v, unsigned width, uint64_t n) {
+  v[0] += n;
+  if (width == 1 || n <= v[0]) return;
+  for (unsigned i = 1; i < width; ++i) {
+    ++v[i];
+    if (v[i] != 0) return;
+  }
+}
+
+static void engine_discard(po_engine* g, uint64_t n) {
+  if (g->engine == 0 || g->engine == 1) {
+    g->ctr[0] += n;
+  } else {
+    ctr_incr(g->ctr, g->arity, n);
+    g->trig = g->arity;
+  }
+}
+
+static void engine_block(po_engine* g) {
+  switch (g->engine) {
+    case 2: po_philox2x64(7, g->ctr, g->key, g->cache); break;
+    case 3: po_philox4x64(7, g->ctr, g->key, g->cache); break;
+    case 4: po_threefry2x64(12, g->ctr, g->key, g->cache); break;
+    case 5: po_threefry4x64(12, g->ctr, g->key, g->cache); break;
+    default: break;
+  }
+}
+
+static uint64_t engine_next(po_engine* g) {
+  if (g->engine == 0) return po_squares3(g->key[0], g->ctr[0]++);
+  if (g->engine == 1) return po_squares4(g->key[0], g->ctr[0]++);
+  if (g->trig == g->arity) {
+    ctr_incr(g->ctr, g->arity, 1);
+    engine_block(g);
+    g->trig = 0;
+  }
+  return g->cache[g->trig++];
+}
+
+/* uniform_real_distribution<double>(0,1) over the engine's word
+ * (hydra_thrust/random/detail/uniform_real_distribution.inl:62-80):
+ * double(word - min) / (1 + double(max - min)), then *(b-a)+a with a=0,b=1.
+ * For 32-bit words this is word * 2^-32 exactly; for 64-bit words the
+ * conversion rounds and the result can equal 1.0.                          */
+double po_uniform(int engine, uint64_t word) {
+  double r = (double)word;
+  if (engine == 0 || engine == 1)
+    r /= (1.0 + (double)UINT32_MAX);
+  else
+    r /= (1.0 + (double)UINT64_MAX);
+  return r * (1.0 - 0.0) + 0.0;
+}
+
+static inline double engine_uniform(po_engine* g) {
+  return po_uniform(g->engine, engine_next(g));
+}
+
+int po_rng_words(int engine, uint64_t seed, uint64_t discard, uint64_t n, uint64_t* out) {
+  if (engine < 0 || engine > 5) return -1;
+  po_engine g;
+  engine_init(&g, engine, seed);
+  engine_discard(&g, discard);
+  for (uint64_t i = 0; i < n; ++i) out[i] = engine_next(&g);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Kinematics helpers                                                         */
+
+/* DecayMother.h:118-126 -- left-to-right product, then sqrt / (2a) */
+static inline double pdk(double a, double b, double c) {
+  return sqrt((a - b - c) * (a + b + c) * (a - b + c) * (a + b - c)) / (2 * a);
+}
+
+/* Vector4R::applyBoostTo(bx,by,bz), forward direction (Vector4R.inl:312-369) */
+static void boost(double* v, double bx, double by, double bz) {
+  double bxx = bx * bx;
+  double byy = by * by;
+  double bzz = bz * bz;
+  double b2 = bxx + byy + bzz;
+  if (b2 > 0.0 && b2 < 1.0) {
+    double gamma = 1.0 / sqrt(1.0 - b2);
+    double gb2 = (gamma - 1.0) / b2;
+    double gb2xy = gb2 * bx * by;
+    double gb2xz = gb2 * bx * bz;
+    double gb2yz = gb2 * by * bz;
+    double gbx = gamma * bx;
+    double gby = gamma * by;
+    double gbz = gamma * bz;
+    double e2 = v[0], px2 = v[1], py2 = v[2], pz2 = v[3];
+    v[0] = gamma * e2 + gbx * px2 + gby * py2 + gbz * pz2;
+    v[1] = gbx * e2 + gb2 * bxx * px2 + px2 + gb2xy * py2 + gb2xz * pz2;
+    v[2] = gby * e2 + gb2 * byy * py2 + py2 + gb2xy * px2 + gb2yz * pz2;
+    v[3] = gbz * e2 + gb2 * bzz * pz2 + pz2 + gb2yz * py2 + gb2xz * px2;
+  }
+}
+
+/* Vector4R::mass2 / mass (Vector4R.inl:56-60,196-208) */
+static inline double mass2(const double* v) {
+  return v[0] * v[0] - v[1] * v[1] - v[2] * v[2] - v[3] * v[3];
+}
+static inline double mass(const double* v) {
+  double m2 = v[0] * v[0] - v[1] * v[1] - v[2] * v[2] - v[3] * v[3];
+  return (m2 > 0.0) ? sqrt(m2) : 0.0;
+}
+
+/* PhaseSpace ctor (PhaseSpace.inl:42-90): validation, fECM, fMaxWeight.
+ * Returns 0, -1 for a non-positive mass (std::invalid_argument), -2 for
+ * fECM <= 0 (std::runtime_error), -3 for N out of range.                   */
+int po_params(int N, double M, const double* masses, double* ecm, double* max_weight) {
+  if (N < 2 || N > PO_MAXN) return -3;
+  if (M <= 0.0) return -1;
+  for (int i = 0; i < N; ++i)
+    if (masses[i] <= 0.0) return -1;
+  double e = M;
+  for (int n = 0; n < N; ++n) e -= masses[n];
+  if (e <= 0.0) return -2;
+  double emmax = e + masses[0];
+  double emmin = 0.0;
+  double wtmax = 1.0;
+  for (int n = 1; n < N; ++n) {
+    emmin += masses[n - 1];
+    emmax += masses[n];
+    double x = (emmax - emmin - masses[n]) * (emmax + emmin + masses[n]) *
+               (emmax - emmin + masses[n]) * (emmax + emmin - masses[n]);
+    x = sqrt(x) / (2 * emmax);
+    wtmax *= x;
+  }
+  *ecm = e;
+  *max_weight = 1.0 / wtmax;
+  return 0;
+}
+
+/* DecayMother ctor boost vector (DecayMother.h:88-98), including the
+ * reference's (E, px, py)/E component choice.                               */
+void po_mother_beta(const double* m4, double* b) {
+  double d3 = sqrt(m4[1] * m4[1] + m4[2] * m4[2] + m4[3] * m4[3]);
+  double beta = d3 / m4[0];
+  if (beta != 0.0) {
+    double w = beta / d3;
+    b[0] = m4[0] * w;
+    b[1] = m4[1] * w;
+    b[2] = m4[2] * w;
+  } else {
+    b[0] = b[1] = b[2] = 0.0;
+  }
+}
+
+/* GENBOD body: DecayMother::process (DecayMother.h:163-276) and
+ * DecayMothers::process (DecayMothers.h:131-242).  final_mode 0 boosts by
+ * beta3 (single mother); 1 boosts by the per-event mother four-vector
+ * (applyBoostTo(Vector4R), Vector4R.inl:237-247).                           */
+static double decay_one(int N, int engine, const double* m, double ecm, double maxw,
+                        int final_mode, const double* fin, uint64_t seed, uint64_t evt,
+                        double (*p)[4]) {
+  po_engine g;
+  engine_init(&g, engine, seed);
+  engine_discard(&g, evt + 3 * (uint64_t)N);
+
+  double rno[PO_MAXN];
+  rno[0] = 0.0;
+  rno[N - 1] = 1.0;
+  if (N > 2) {
+    for (int n = 1; n < N - 1; ++n) rno[n] = engine_uniform(&g);
+    /* bubble sort of rno[1..N-2] (DecayMother.h:128-151) */
+    int cnt = N - 2;
+    for (int c = 0; c < cnt; ++c) {
+      int nswap = 0;
+      for (int d = 0; d < cnt - c - 1; ++d) {
+        if (rno[1 + d] > rno[1 + d + 1]) {
+          double t = rno[1 + d];
+          rno[1 + d] = rno[1 + d + 1];
+          rno[1 + d + 1] = t;
+          nswap++;
+        }
+      }
+      if (nswap == 0) break;
+    }
+  }
+
+  double invMas[PO_MAXN], sum = 0.0;
+  for (int n = 0; n < N; ++n) {
+    sum += m[n];
+    invMas[n] = rno[n] * ecm + sum;
+  }
+
+  double wt = maxw;
+  double pd[PO_MAXN];
+  for (int n = 0; n < N - 1; ++n) {
+    pd[n] = pdk(invMas[n + 1], invMas[n], m[n + 1]);
+    wt *= pd[n];
+  }
+
+  p[0][0] = sqrt(pd[0] * pd[0] + m[0] * m[0]);
+  p[0][1] = 0.0;
+  p[0][2] = pd[0];
+  p[0][3] = 0.0;
+
+  for (int i = 1; i < N; ++i) {
+    p[i][0] = sqrt(pd[i - 1] * pd[i - 1] + m[i] * m[i]);
+    p[i][1] = 0.0;
+    p[i][2] = -pd[i - 1];
+    p[i][3] = 0.0;
+
+    double cZ = 2 * engine_uniform(&g) - 1;
+    double sZ = sqrt(1 - cZ * cZ);
+    double angY = 2 * PO_PI * engine_uniform(&g);
+    double cY = cos(angY);
+    double sY = sin(angY);
+    for (int j = 0; j <= i; ++j) {
+      double x = p[j][1];
+      double y = p[j][2];
+      p[j][1] = cZ * x - sZ * y;
+      p[j][2] = sZ * x + cZ * y;
+      x = p[j][1];
+      double z = p[j][3];
+      p[j][1] = cY * x - sY * z;
+      p[j][3] = sY * x + cY * z;
+    }
+    if (i == N - 1) break;
+    double beta = pd[i] / sqrt(pd[i] * pd[i] + invMas[i] * invMas[i]);
+    for (int j = 0; j <= i; ++j) boost(p[j], 0, beta, 0);
+  }
+
+  if (final_mode == 0) {
+    for (int n = 0; n < N; ++n) boost(p[n], fin[0], fin[1], fin[2]);
+  } else {
+    double e = fin[0];
+    double bx = fin[1] / e, by = fin[2] / e, bz = fin[3] / e;
+    for (int n = 0; n < N; ++n) boost(p[n], bx, by, bz);
+  }
+  return wt;
+}
+
+void po_decay_events(int N, int engine, const double* masses, double ecm, double max_weight,
+                     const double* beta3, uint64_t seed, const uint64_t* evts, uint64_t nevt,
+                     double* out_p, double* out_wt) {
+  for (uint64_t k = 0; k < nevt; ++k) {
+    double p[PO_MAXN][4];
+    double wt = decay_one(N, engine, masses, ecm, max_weight, 0, beta3, seed, evts[k], p);
+    for (int i = 0; i < N; ++i)
+      for (int c = 0; c < 4; ++c) out_p[(k * N + i) * 4 + c] = p[i][c];
+    if (out_wt) out_wt[k] = wt;
+  }
+}
+
+static void set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+}
+
+/* PhaseSpace::Generate(mother, Decays) over events first..first+n-1
+ * (PhaseSpace.inl:377-394, launch_decayers.inl:133-139); SoA output.       */
+void po_generate(int N, int engine, const double* masses, double ecm, double max_weight,
+                 const double* beta3, uint64_t seed, uint64_t first, uint64_t n, double* cols,
+                 double* wt, int nthreads) {
+  set_threads(nthreads);
+  long long nn = (long long)n;
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < nn; ++k) {
+    double p[PO_MAXN][4];
+    double w = decay_one(N, engine, masses, ecm, max_weight, 0, beta3, seed,
+                         first + (uint64_t)k, p);
+    for (int i = 0; i < N; ++i)
+      for (int c = 0; c < 4; ++c) cols[(uint64_t)(4 * i + c) * n + (uint64_t)k] = p[i][c];
+    if (wt) wt[k] = w;
+  }
+}
+
+/* PhaseSpace::Generate(mothers, daughters) (PhaseSpace.inl:396-415,
+ * launch_decayers.inl:153-166): event k uses counter index first+k and
+ * mother column row k.                                                      */
+void po_generate_mothers(int N, int engine, const double* masses, double ecm,
+                         double max_weight, uint64_t seed, uint64_t first, uint64_t n,
+                         const double* mother_cols, double* cols, double* wt, int nthreads) {
+  set_threads(nthreads);
+  long long nn = (long long)n;
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < nn; ++k) {
+    double p[PO_MAXN][4];
+    double m4[4];
+    for (int c = 0; c < 4; ++c) m4[c] = mother_cols[(uint64_t)c * n + (uint64_t)k];
+    double w = decay_one(N, engine, masses, ecm, max_weight, 1, m4, seed,
+                         first + (uint64_t)k, p);
+    for (int i = 0; i < N; ++i)
+      for (int c = 0; c < 4; ++c) cols[(uint64_t)(4 * i + c) * n + (uint64_t)k] = p[i][c];
+    if (wt) wt[k] = w;
+  }
+}
+
+/* Functors (arithmetic order fixed; see header).                            */
+double po_functor(int fid, const double* fpar, int N, const double* p) {
+  (void)N;
+  if (fid == 0) return 1.0;
+  if (fid == 2) {
+    double s[4];
+    for (int c = 0; c < 4; ++c) s[c] = p[c] + p[4 + c];
+    return mass2(s);
+  }
+  int K = (int)fpar[0];
+  double Ar = 0.0, Ai = 0.0;
+  for (int k = 0; k < K; ++k) {
+    const double* q = fpar + 1 + 6 * k;
+    int a = (int)q[4], b = (int)q[5];
+    double v[4];
+    for (int c = 0; c < 4; ++c) v[c] = p[4 * a + c] + p[4 * b + c];
+    double s = mass2(v);
+    double dr = q[0] * q[0] - s;
+    double di = -(q[0] * q[1]);
+    double den = dr * dr + di * di;
+    double br = dr / den;
+    double bi = -di / den;
+    double tr = q[2] * br - q[3] * bi;
+    double ti = q[2] * bi + q[3] * br;
+    Ar += tr;
+    Ai += ti;
+  }
+  return Ar * Ar + Ai * Ai;
+}
+
+/* PhaseSpaceWeight::Evaluate (Decays.inl:215-231); with fid >= 0 the
+ * PhaseSpaceReweight product w * f (Decays.inl:362-379).                    */
+static double event_weight_row(int N, const double* m, double maxw, const double* p) {
+  double R[4] = {p[0], p[1], p[2], p[3]};
+  double w = maxw;
+  for (int i = 1; i < N; ++i) {
+    double s[4];
+    for (int c = 0; c < 4; ++c) s[c] = R[c] + p[4 * i + c];
+    w *= pdk(mass(s), m[i], mass(R));
+    for (int c = 0; c < 4; ++c) R[c] += p[4 * i + c];
+  }
+  return w;
+}
+
+static void gather_row(int N, const double* cols, uint64_t n, uint64_t k, double* p) {
+  for (int i = 0; i < N; ++i)
+    for (int c = 0; c < 4; ++c) p[4 * i + c] = cols[(uint64_t)(4 * i + c) * n + k];
+}
+
+void po_event_weight(int N, const double* masses, double max_weight, int fid,
+                     const double* fpar, const double* cols, uint64_t n, double* w_out,
+                     int nthreads) {
+  set_threads(nthreads);
+  long long nn = (long long)n;
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < nn; ++k) {
+    double p[PO_MAXN * 4];
+    gather_row(N, cols, n, (uint64_t)k, p);
+    double w = event_weight_row(N, masses, max_weight, p);
+    if (fid >= 0) w = w * po_functor(fid, fpar, N, p);
+    w_out[k] = w;
+  }
+}
+
+/* Decays::Unweight (Decays.inl:415-455) / Unweight(functor, max, seed)
+ * (Decays.inl:457-511): flag event k iff weight/max_value > u_k with
+ * u_k from squares3(seed).discard(k) (FlagDaugthers, Decays.inl:75-89),
+ * then a stable partition (accepted first, both halves in original order).
+ * max_value: fMaxWeight for the plain form; for the functor form the given
+ * value if > 0, else the maximum of w*f over the container.               */
+uint64_t po_unweight(int N, const double* masses, double max_weight, int fid,
+                     const double* fpar, double max_value, const double* cols_in, uint64_t n,
+                     uint64_t seed, double* cols_out, uint64_t* idx_out, uint8_t* flags_out,
+                     int nthreads) {
+  double* w = (double*)malloc(sizeof(double) * (n ? n : 1));
+  uint8_t* f = flags_out ? flags_out : (uint8_t*)malloc(n ? n : 1);
+  po_event_weight(N, masses, max_weight, fid, fpar, cols_in, n, w, nthreads);
+  double mv = max_weight;
+  if (fid >= 0) {
+    if (max_value > 0.0) {
+      mv = max_value;
+    } else {
+      mv = w[0];
+      for (uint64_t k = 1; k < n; ++k)
+        if (mv < w[k]) mv = w[k];
+    }
+  }
+  uint64_t key = seed;
+  key = po_splitmix64(&key);
+  uint64_t nacc = 0;
+  for (uint64_t k = 0; k < n; ++k) {
+    double u = po_uniform(0, po_squares3(key, k));
+    f[k] = (w[k] / mv > u) ? 1 : 0;
+    nacc += f[k];
+  }
+  if (cols_out || idx_out) {
+    uint64_t a = 0, r = nacc;
+    for (uint64_t k = 0; k < n; ++k) {
+      uint64_t dst = f[k] ? a++ : r++;
+      if (idx_out) idx_out[dst] = k;
+      if (cols_out)
+        for (int c = 0; c < 4 * N; ++c)
+          cols_out[(uint64_t)c * n + dst] = cols_in[(uint64_t)c * n + k];
+    }
+  }
+  free(w);
+  if (!flags_out) free(f);
+  return nacc;
+}
+
+/* AddStatsPHSP (StatsPHSP.h:62-86); stats are {W, mean, M2}.                */
+void po_stats_merge(const double* x, const double* y, double* out) {
+  double w = x[0] + y[0];
+  double delta = y[1] - x[1];
+  double delta2 = delta * delta;
+  double mean = (x[1] * x[0] + y[1] * y[0]) / w;
+  double m2 = x[2] + y[2];
+  m2 += delta2 * x[0] * y[0] / w;
+  out[0] = w;
+  out[1] = mean;
+  out[2] = m2;
+}
+
+/* transform_reduce on the OMP system (system/omp/detail/reduce.inl:40-63,
+ * reduce_intervals.inl:66-80, internal/decompose.h:53-90): P intervals of a
+ * uniform decomposition, each a left fold started at its first element,
+ * then a left fold of {init, partials...}.  With intervals = P this repeats
+ * the reference's order exactly when it ran with omp_get_num_procs() = P.   */
+static void average_core(int N, int engine, const double* masses, double ecm, double maxw,
+                         int final_mode, const double* fin, const double* mother_cols,
+                         uint64_t seed, uint64_t first, uint64_t n, int fid,
+                         const double* fpar, int intervals, double* out3) {
+  long long P = intervals > 0 ? intervals : 1;
+  long long nn = (long long)n;
+  long long nint = nn, small = 1, large = 0, thr = 0;
+  if (nint > P) {
+    small = nn / P;
+    large = small + 1;
+    thr = nn % P;
+    nint = P;
+  }
+  double* part = (double*)calloc((size_t)(nint + 1) * 3, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long long i = 0; i < nint; ++i) {
+    long long b, e;
+    if (i < thr) {
+      b = large * i;
+      e = b + large;
+    } else {
+      b = large * thr + small * (i - thr);
+      e = (b + small < nn) ? b + small : nn;
+    }
+    double acc[3] = {0, 0, 0};
+    for (long long k = b; k < e; ++k) {
+      double p[PO_MAXN + 1][4];
+      double wt, fv;
+      uint64_t evt = first + (uint64_t)k;
+      if (final_mode == 0) {
+        wt = decay_one(N, engine, masses, ecm, maxw, 0, fin, seed, evt, p);
+        fv = po_functor(fid, fpar, N, &p[0][0]);
+      } else {
+        double m4[4];
+        for (int c = 0; c < 4; ++c) m4[c] = mother_cols[(uint64_t)c * n + (uint64_t)k];
+        wt = decay_one(N, engine, masses, ecm, maxw, 1, m4, seed, evt, &p[1]);
+        for (int c = 0; c < 4; ++c) p[0][c] = m4[c];
+        fv = po_functor(fid, fpar, N + 1, &p[0][0]);
+      }
+      double s[3] = {wt, fv, 0.0};
+      if (k == b) {
+        acc[0] = s[0]; acc[1] = s[1]; acc[2] = s[2];
+      } else {
+        po_stats_merge(acc, s, acc);
+      }
+    }
+    part[(i + 1) * 3 + 0] = acc[0];
+    part[(i + 1) * 3 + 1] = acc[1];
+    part[(i + 1) * 3 + 2] = acc[2];
+  }
+  double r[3] = {part[0], part[1], part[2]};
+  for (long long i = 1; i <= nint; ++i) po_stats_merge(r, &part[i * 3], r);
+  out3[0] = r[0];
+  out3[1] = r[1];
+  out3[2] = r[2];
+  free(part);
+}
+
+/* PhaseSpace::AverageOn(policy, mother, functor, n) (PhaseSpace.inl:255-273,
+ * AverageMother.h:272-292): per event {f, wt, 0}; returns {W, mean, M2};
+ * the reference reports (mean, sqrt(M2)/W).                                 */
+void po_average_on(int N, int engine, const double* masses, double ecm, double max_weight,
+                   const double* beta3, uint64_t seed, uint64_t first, uint64_t n, int fid,
+                   const double* fpar, int intervals, double* out3) {
+  average_core(N, engine, masses, ecm, max_weight, 0, beta3, NULL, seed, first, n, fid, fpar,
+               intervals, out3);
+}
+
+/* PhaseSpace::AverageOn(begin, end, functor) over mothers
+ * (PhaseSpace.inl:275-287, AverageMothers.h:245-283): the functor sees the
+ * mother as particle 0 followed by the N daughters.                         */
+void po_average_on_mothers(int N, int engine, const double* masses, double ecm,
+                           double max_weight, uint64_t seed, uint64_t n,
+                           const double* mother_cols, int fid, const double* fpar,
+                           int intervals, double* out3) {
+  average_core(N, engine, masses, ecm, max_weight, 1, NULL, mother_cols, seed, 0, n, fid, fpar,
+               intervals, out3);
+}

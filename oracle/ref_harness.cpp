@@ -1,0 +1,459 @@
+// Reference harness: drives the reference Hydra 4.0.1 headers (mounted at
+// build time under /root/reference, OMP device backend) through a flat C
+// interface so that golden vectors can be written and the reference CPU path
+// can be timed.  This file contains no reference code; it only instantiates
+// the reference's own templates:
+//
+//   hydra::PhaseSpace<N,GRND>            hydra/PhaseSpace.h:86, detail/PhaseSpace.inl:42-455
+//   hydra::Decays<tuple<P...>,device>    hydra/Decays.h:67-651, detail/Decays.inl:415-511
+//   hydra::detail::DecayMother::process  hydra/detail/functors/DecayMother.h:163-276
+//   engines squares3/4, philox(_long), threefry(_long)
+//                                        hydra/detail/random/{squares3,squares4,EngineR123}.h
+//
+// It is test infrastructure: only tests/, bench.py's CPU-baseline leg and the
+// golden-vector script load the library built from it (oracle/_ref/).
+#include <hydra/Types.h>
+#include <hydra/Vector4R.h>
+#include <hydra/PhaseSpace.h>
+#include <hydra/Lambda.h>
+#include <hydra/Tuple.h>
+#include <hydra/host/System.h>
+#include <hydra/device/System.h>
+#include <hydra/Decays.h>
+#include <hydra/detail/Hash.h>
+
+#include <omp.h>
+#include <array>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+declarg(P0, hydra::Vector4R)
+declarg(P1, hydra::Vector4R)
+declarg(P2, hydra::Vector4R)
+declarg(P3, hydra::Vector4R)
+declarg(P4, hydra::Vector4R)
+declarg(P5, hydra::Vector4R)
+declarg(P6, hydra::Vector4R)
+declarg(P7, hydra::Vector4R)
+declarg(P8, hydra::Vector4R)
+declarg(P9, hydra::Vector4R)
+declarg(P10, hydra::Vector4R)
+declarg(P11, hydra::Vector4R)
+
+using namespace hydra::arguments;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Particle type lists P0..P(N-1)
+template <size_t I> struct ptype;
+template <> struct ptype<0> { using type = P0; };
+template <> struct ptype<1> { using type = P1; };
+template <> struct ptype<2> { using type = P2; };
+template <> struct ptype<3> { using type = P3; };
+template <> struct ptype<4> { using type = P4; };
+template <> struct ptype<5> { using type = P5; };
+template <> struct ptype<6> { using type = P6; };
+template <> struct ptype<7> { using type = P7; };
+template <> struct ptype<8> { using type = P8; };
+template <> struct ptype<9> { using type = P9; };
+template <> struct ptype<10> { using type = P10; };
+template <> struct ptype<11> { using type = P11; };
+
+template <typename Seq> struct tuple_of;
+template <size_t... I> struct tuple_of<std::index_sequence<I...>> {
+  using type = hydra::tuple<typename ptype<I>::type...>;
+};
+template <size_t N>
+using decays_t = hydra::Decays<typename tuple_of<std::make_index_sequence<N>>::type,
+                               hydra::device::sys_t>;
+
+template <size_t N, typename Row, size_t... I>
+inline void row_to_array(Row const& row, double* out, std::index_sequence<I...>) {
+  hydra::Vector4R v[N] = {hydra::Vector4R(
+      typename ptype<I>::type(hydra::get<I>(row)).Value())...};
+  for (size_t p = 0; p < N; ++p)
+    for (int c = 0; c < 4; ++c) out[4 * p + c] = v[p].get(c);
+}
+
+// ---------------------------------------------------------------------------
+// Functors used by AverageOn / Unweight(functor).  Their arithmetic is spelled
+// out so the GPU kernels and the C restatement can follow the same sequence.
+//   id 0: f = 1
+//   id 1: f = |sum_k c_k / (M_k^2 - s_k - i M_k G_k)|^2,  s_k = (p_a + p_b)^2
+//         par = {K, (M, G, cr, ci, a, b) x K}
+//   id 2: f = (p0 + p1)^2
+struct Functor {
+  int id;
+  int K;
+  double M[4], G[4], cr[4], ci[4];
+  int a[4], b[4];
+
+  template <typename V>
+  double eval(V const* p) const {
+    if (id == 0) return 1.0;
+    if (id == 2) {
+      hydra::Vector4R s = p[0] + p[1];
+      return s.mass2();
+    }
+    double Ar = 0.0, Ai = 0.0;
+    for (int k = 0; k < K; ++k) {
+      hydra::Vector4R q = p[a[k]] + p[b[k]];
+      double s = q.mass2();
+      double dr = M[k] * M[k] - s;
+      double di = -(M[k] * G[k]);
+      double den = dr * dr + di * di;
+      double br = dr / den;
+      double bi = -di / den;
+      double tr = cr[k] * br - ci[k] * bi;
+      double ti = cr[k] * bi + ci[k] * br;
+      Ar += tr;
+      Ai += ti;
+    }
+    return Ar * Ar + Ai * Ai;
+  }
+};
+
+Functor make_functor(int id, const double* par) {
+  Functor f{};
+  f.id = id;
+  if (id == 1) {
+    f.K = (int)par[0];
+    for (int k = 0; k < f.K; ++k) {
+      const double* q = par + 1 + 6 * k;
+      f.M[k] = q[0];
+      f.G[k] = q[1];
+      f.cr[k] = q[2];
+      f.ci[k] = q[3];
+      f.a[k] = (int)q[4];
+      f.b[k] = (int)q[5];
+    }
+  }
+  return f;
+}
+
+// Callable handed to PhaseSpace::AverageOn: receives a tuple of N Vector4R.
+template <size_t N>
+struct TupleFunctor {
+  Functor f;
+  template <typename Tuple, size_t... I>
+  double call(Tuple const& t, std::index_sequence<I...>) const {
+    hydra::Vector4R v[N] = {hydra::Vector4R(hydra::get<I>(t))...};
+    return f.eval(v);
+  }
+  template <typename Tuple>
+  __hydra_host__ __hydra_device__ double operator()(Tuple const& t) const {
+    return call(t, std::make_index_sequence<N>{});
+  }
+};
+
+// Hydra functor (typed arguments) for Decays::Unweight(functor, ...).
+template <size_t N, typename Sig> struct HydraF;
+template <size_t N, typename... Ps>
+struct HydraF<N, void(Ps...)>
+    : public hydra::BaseFunctor<HydraF<N, void(Ps...)>, double(Ps...), 0> {
+  typedef hydra::BaseFunctor<HydraF<N, void(Ps...)>, double(Ps...), 0> super_type;
+  Functor f;
+  HydraF() = default;
+  explicit HydraF(Functor const& ff) : super_type(), f(ff) {}
+  __hydra_host__ __hydra_device__ HydraF(HydraF const& o) : super_type(o), f(o.f) {}
+  __hydra_host__ __hydra_device__ HydraF& operator=(HydraF const& o) {
+    if (this == &o) return *this;
+    super_type::operator=(o);
+    f = o.f;
+    return *this;
+  }
+  __hydra_host__ __hydra_device__ double Evaluate(Ps... p) const {
+    hydra::Vector4R v[N] = {hydra::Vector4R(p.Value())...};
+    return f.eval(v);
+  }
+};
+template <typename Seq> struct sig_of;
+template <size_t... I> struct sig_of<std::index_sequence<I...>> {
+  using type = void(typename ptype<I>::type...);
+};
+template <size_t N>
+using hydra_functor_t = HydraF<N, typename sig_of<std::make_index_sequence<N>>::type>;
+
+// ---------------------------------------------------------------------------
+template <typename GRND>
+void engine_words(uint64_t seed, uint64_t discard, uint64_t n, uint64_t* out) {
+  GRND eng(seed);
+  eng.discard(discard);
+  for (uint64_t i = 0; i < n; ++i) out[i] = (uint64_t)eng();
+}
+
+template <size_t N, typename GRND>
+void decay_events(double M, const double* mother4, const double* masses, uint64_t seed,
+                  const uint64_t* evts, uint64_t nevt, double* out_p, double* out_wt) {
+  double m[N];
+  for (size_t i = 0; i < N; ++i) m[i] = masses[i];
+  hydra::PhaseSpace<N, GRND> phsp(M, m);
+  hydra::Vector4R mother(mother4[0], mother4[1], mother4[2], mother4[3]);
+  hydra::detail::DecayMother<N, GRND> dm(mother, m, phsp.GetMaxWeight(), phsp.GetECM(),
+                                         (size_t)seed);
+  for (uint64_t k = 0; k < nevt; ++k) {
+    hydra::Vector4R p[N];
+    double wt = dm.process((size_t)evts[k], p);
+    for (size_t i = 0; i < N; ++i)
+      for (int c = 0; c < 4; ++c) out_p[(k * N + i) * 4 + c] = p[i].get(c);
+    out_wt[k] = wt;
+  }
+}
+
+template <size_t N>
+void copy_out(decays_t<N>& ev, double* out_p, double* out_w) {
+  const size_t n = ev.size();
+  if (out_p)
+    for (size_t i = 0; i < n; ++i)
+      row_to_array<N>(ev[i], out_p + i * N * 4, std::make_index_sequence<N>{});
+  if (out_w) {
+    auto w = ev | ev.GetEventWeightFunctor();
+    for (size_t i = 0; i < n; ++i) out_w[i] = w[i];
+  }
+}
+
+double now_s() {
+  return std::chrono::duration<double>(
+             std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <size_t N>
+void generate(double M, const double* mother4, const double* masses, int use_seed,
+              int32_t seed, uint64_t n, double* out_p, double* out_w, double* seconds) {
+  double m[N];
+  for (size_t i = 0; i < N; ++i) m[i] = masses[i];
+  hydra::PhaseSpace<N> phsp(M, m);
+  if (use_seed) phsp.SetSeed(seed);
+  hydra::Vector4R mother(mother4[0], mother4[1], mother4[2], mother4[3]);
+  decays_t<N> ev(M, m, n);
+  double t0 = now_s();
+  phsp.Generate(mother, ev);
+  double t1 = now_s();
+  if (seconds) *seconds = t1 - t0;
+  copy_out<N>(ev, out_p, out_w);
+}
+
+template <size_t N>
+void generate_mothers(double M, const double* masses, int32_t seed, uint64_t n,
+                      const double* mothers /* n x 4 */, double* out_p, double* out_w) {
+  double m[N];
+  for (size_t i = 0; i < N; ++i) m[i] = masses[i];
+  hydra::PhaseSpace<N> phsp(M, m);
+  phsp.SetSeed(seed);
+  hydra::device::vector<hydra::Vector4R> mv(n);
+  for (uint64_t i = 0; i < n; ++i)
+    mv[i] = hydra::Vector4R(mothers[4 * i], mothers[4 * i + 1], mothers[4 * i + 2],
+                            mothers[4 * i + 3]);
+  decays_t<N> ev(M, m, n);
+  phsp.Generate(mv, ev);
+  copy_out<N>(ev, out_p, out_w);
+}
+
+template <size_t N>
+uint64_t permuted_indices(decays_t<N>& ev, std::vector<double> const& orig,
+                          double* out_p, uint64_t* out_idx) {
+  const size_t n = ev.size();
+  std::map<std::string, uint64_t> where;
+  for (size_t i = 0; i < n; ++i)
+    where.emplace(std::string((const char*)&orig[i * N * 4], sizeof(double) * N * 4), i);
+  std::vector<double> row(N * 4);
+  for (size_t i = 0; i < n; ++i) {
+    row_to_array<N>(ev[i], row.data(), std::make_index_sequence<N>{});
+    if (out_p) std::memcpy(out_p + i * N * 4, row.data(), sizeof(double) * N * 4);
+    auto it = where.find(std::string((const char*)row.data(), sizeof(double) * N * 4));
+    if (it == where.end()) return 1;
+    out_idx[i] = it->second;
+  }
+  return 0;
+}
+
+template <size_t N>
+int unweight(double M, const double* mother4, const double* masses, int32_t seed, uint64_t n,
+             int with_functor, int fid, const double* fpar, double max_weight,
+             uint64_t uw_seed, double* out_p, uint64_t* out_idx, uint64_t* n_acc,
+             double* seconds) {
+  double m[N];
+  for (size_t i = 0; i < N; ++i) m[i] = masses[i];
+  hydra::PhaseSpace<N> phsp(M, m);
+  phsp.SetSeed(seed);
+  hydra::Vector4R mother(mother4[0], mother4[1], mother4[2], mother4[3]);
+  decays_t<N> ev(M, m, n);
+  phsp.Generate(mother, ev);
+  std::vector<double> orig;
+  if (out_idx) {
+    orig.resize(n * N * 4);
+    copy_out<N>(ev, orig.data(), nullptr);
+  }
+  double t0 = now_s();
+  size_t acc;
+  if (with_functor) {
+    hydra_functor_t<N> hf(make_functor(fid, fpar));
+    acc = ev.Unweight(hf, max_weight, (size_t)uw_seed).size();
+  } else {
+    acc = ev.Unweight((size_t)uw_seed).size();
+  }
+  double t1 = now_s();
+  if (seconds) *seconds = t1 - t0;
+  *n_acc = acc;
+  if (out_idx) return (int)permuted_indices<N>(ev, orig, out_p, out_idx);
+  return 0;
+}
+
+template <size_t N>
+void average_on(double M, const double* mother4, const double* masses, int use_seed,
+                int32_t seed, uint64_t n, int fid, const double* fpar, double* mean,
+                double* sigma, double* seconds) {
+  double m[N];
+  for (size_t i = 0; i < N; ++i) m[i] = masses[i];
+  hydra::PhaseSpace<N> phsp(M, m);
+  if (use_seed) phsp.SetSeed(seed);
+  hydra::Vector4R mother(mother4[0], mother4[1], mother4[2], mother4[3]);
+  TupleFunctor<N> tf{make_functor(fid, fpar)};
+  double t0 = now_s();
+  auto r = phsp.AverageOn(hydra::device::sys, mother, tf, n);
+  double t1 = now_s();
+  if (seconds) *seconds = t1 - t0;
+  *mean = r.first;
+  *sigma = r.second;
+}
+
+// Runtime N dispatch (2..12).
+template <template <size_t> class Op, typename... Args>
+int dispatch(int N, Args&&... args) {
+  switch (N) {
+    case 2: Op<2>::run(args...); return 0;
+    case 3: Op<3>::run(args...); return 0;
+    case 4: Op<4>::run(args...); return 0;
+    case 5: Op<5>::run(args...); return 0;
+    case 6: Op<6>::run(args...); return 0;
+    case 7: Op<7>::run(args...); return 0;
+    case 8: Op<8>::run(args...); return 0;
+    case 9: Op<9>::run(args...); return 0;
+    case 10: Op<10>::run(args...); return 0;
+    case 11: Op<11>::run(args...); return 0;
+    case 12: Op<12>::run(args...); return 0;
+    default: return -1;
+  }
+}
+
+template <size_t N> struct GenOp {
+  template <typename... A> static void run(A... a) { generate<N>(a...); }
+};
+template <size_t N> struct GenMothersOp {
+  template <typename... A> static void run(A... a) { generate_mothers<N>(a...); }
+};
+template <size_t N> struct AvgOp {
+  template <typename... A> static void run(A... a) { average_on<N>(a...); }
+};
+template <size_t N> struct UwOp {
+  template <typename... A> static void run(int* rc, A... a) { *rc = unweight<N>(a...); }
+};
+template <size_t N> struct ParamsOp {
+  static void run(double M, const double* masses, double* ecm, double* maxw,
+                  uint64_t* seed_full, int32_t* seed_get) {
+    double m[N];
+    for (size_t i = 0; i < N; ++i) m[i] = masses[i];
+    hydra::PhaseSpace<N> phsp(M, m);
+    *ecm = phsp.GetECM();
+    *maxw = phsp.GetMaxWeight();
+    *seed_full = (uint64_t)(1 * (1 + hydra::detail::hash_range(&m[0], &m[0] + N)));
+    *seed_get = phsp.GetSeed();
+  }
+};
+template <size_t N> struct DecayEvtOp {
+  static void run(int engine, double M, const double* mother4, const double* masses,
+                  uint64_t seed, const uint64_t* evts, uint64_t nevt, double* out_p,
+                  double* out_wt) {
+    switch (engine) {
+      case 0: decay_events<N, hydra::random::squares3>(M, mother4, masses, seed, evts, nevt, out_p, out_wt); break;
+      case 1: decay_events<N, hydra::random::squares4>(M, mother4, masses, seed, evts, nevt, out_p, out_wt); break;
+      case 2: decay_events<N, hydra::random::philox>(M, mother4, masses, seed, evts, nevt, out_p, out_wt); break;
+      case 3: decay_events<N, hydra::random::philox_long>(M, mother4, masses, seed, evts, nevt, out_p, out_wt); break;
+      case 4: decay_events<N, hydra::random::threefry>(M, mother4, masses, seed, evts, nevt, out_p, out_wt); break;
+      case 5: decay_events<N, hydra::random::threefry_long>(M, mother4, masses, seed, evts, nevt, out_p, out_wt); break;
+      default: break;
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int hr_set_threads(int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  return omp_get_max_threads();
+}
+
+// Engine words: GRND eng(seed); eng.discard(discard); out[i] = eng().
+// engine: 0 squares3, 1 squares4, 2 philox, 3 philox_long, 4 threefry, 5 threefry_long
+int hr_rng_words(int engine, uint64_t seed, uint64_t discard, uint64_t n, uint64_t* out) {
+  switch (engine) {
+    case 0: engine_words<hydra::random::squares3>(seed, discard, n, out); return 0;
+    case 1: engine_words<hydra::random::squares4>(seed, discard, n, out); return 0;
+    case 2: engine_words<hydra::random::philox>(seed, discard, n, out); return 0;
+    case 3: engine_words<hydra::random::philox_long>(seed, discard, n, out); return 0;
+    case 4: engine_words<hydra::random::threefry>(seed, discard, n, out); return 0;
+    case 5: engine_words<hydra::random::threefry_long>(seed, discard, n, out); return 0;
+    default: return -1;
+  }
+}
+
+// PhaseSpace<N> constructor products: fECM, fMaxWeight, the full 64-bit default
+// seed (1 * (1 + hash_range(masses))) and what GetSeed() returns for it.
+int hr_phsp_params(int N, double M, const double* masses, double* ecm, double* maxw,
+                   uint64_t* seed_full, int32_t* seed_get) {
+  try {
+    return dispatch<ParamsOp>(N, M, masses, ecm, maxw, seed_full, seed_get);
+  } catch (std::exception const&) {
+    return -2;
+  }
+}
+
+// DecayMother<N,GRND>::process on explicit event indices: momenta (AoS,
+// nevt x N x 4) and the generation weight wt.
+int hr_decay_events(int N, int engine, double M, const double* mother4, const double* masses,
+                    uint64_t seed, const uint64_t* evts, uint64_t nevt, double* out_p,
+                    double* out_wt) {
+  return dispatch<DecayEvtOp>(N, engine, M, mother4, masses, seed, evts, nevt, out_p, out_wt);
+}
+
+// PhaseSpace<N>::Generate(mother, Decays) then GetEventWeightFunctor weights.
+int hr_generate(int N, double M, const double* mother4, const double* masses, int use_seed,
+                int32_t seed, uint64_t n, double* out_p, double* out_w, double* seconds) {
+  return dispatch<GenOp>(N, M, mother4, masses, use_seed, seed, n, out_p, out_w, seconds);
+}
+
+int hr_generate_mothers(int N, double M, const double* masses, int32_t seed, uint64_t n,
+                        const double* mothers, double* out_p, double* out_w) {
+  return dispatch<GenMothersOp>(N, M, masses, seed, n, mothers, out_p, out_w);
+}
+
+// Generate then Decays::Unweight(seed) (with_functor=0) or
+// Decays::Unweight(functor, max_weight, seed) (with_functor=1).  Returns the
+// permuted container and, for each output row, its original event index.
+int hr_unweight(int N, double M, const double* mother4, const double* masses, int32_t seed,
+                uint64_t n, int with_functor, int fid, const double* fpar, double max_weight,
+                uint64_t uw_seed, double* out_p, uint64_t* out_idx, uint64_t* n_acc,
+                double* seconds) {
+  int rc = 0;
+  int d = dispatch<UwOp>(N, &rc, M, mother4, masses, seed, n, with_functor, fid, fpar,
+                         max_weight, uw_seed, out_p, out_idx, n_acc, seconds);
+  return d ? d : rc;
+}
+
+// PhaseSpace<N>::AverageOn(device::sys, mother, functor, n) -> (mean, sigma).
+int hr_average_on(int N, double M, const double* mother4, const double* masses, int use_seed,
+                  int32_t seed, uint64_t n, int fid, const double* fpar, double* mean,
+                  double* sigma, double* seconds) {
+  return dispatch<AvgOp>(N, M, mother4, masses, use_seed, seed, n, fid, fpar, mean, sigma,
+                         seconds);
+}
+
+}  // extern "C"
