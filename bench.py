@@ -1,0 +1,335 @@
+#!/usr/bin/env python3
+"""Phase-space benchmark on one MI355X.
+
+Headline workload (one "step"): the 3-body decay M = 5.279 GeV ->
+(3.0969, 0.493677, 0.13957) GeV, FP64, 2^27 events generated into SoA
+four-momentum columns (with the PhaseSpaceWeight column the Decays
+container keeps), then Decays::Unweight() with the reference's default seed
+(accept-reject, stable partition written out of place).  value = events per
+second of the whole step; inputs never leave HBM.
+
+Also measured (reported under "extras", not the headline): the 10-body
+decay at 2^26 events in FP64 and mixed precision (plus the near-threshold
+variant), and AverageOn of the three-Breit-Wigner Dalitz functor over 2^30
+events in 2^24-event chunks, FP64 and mixed.
+
+Kernel durations come from HIP events recorded on the stream the kernels are
+launched on (torch's current stream, handed to the C ABI).  The CPU baseline
+is the reference's own CPU path (oracle/_ref, OMP backend) on a bounded
+sample of the same workload, or the oracle's C restatement when the
+reference build is absent.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+M3 = 5.279
+MASSES3 = [3.0969, 0.493677, 0.13957]
+SEED = 159753
+UW_SEED = 0x180EC6D33CFD0ABA
+BW3 = [(0.892, 0.05, 0.4967141530, -0.1382643012, 1, 2),
+       (1.43, 0.11, 0.6476885381, 1.5230298564, 1, 2),
+       (0.775, 0.149, -0.2341533747, -0.2341368981, 0, 2)]
+
+
+def genbod_flops(N):
+    """Algorithmic FP64 operations of one GENBOD event (add/sub/mul/div/sqrt
+    each one; sin and cos not counted), from the reference's op sequence."""
+    s1 = sum(11 + 12 * (i + 1) for i in range(1, N))
+    s2 = sum(5 + 53 * (i + 1) for i in range(1, N - 1))
+    return 3 * (3 * N - 4) + 3 * N + 15 * (N - 1) + 4 + s1 + s2 + 5 * N
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 27, help="events per step (headline)")
+    ap.add_argument("--extras", type=int, default=1, help="also time the other workloads")
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--only", default="", help="run one workload: c2, c3, c4 (profiling)")
+    return ap.parse_args()
+
+
+class Timer:
+    """HIP events on torch's current stream (where the C ABI launches)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.a = torch.cuda.Event(enable_timing=True)
+        self.b = torch.cuda.Event(enable_timing=True)
+        self.total = 0.0
+        self.count = 0
+
+    def start(self):
+        self.a.record()
+
+    def stop(self):
+        self.b.record()
+        self.b.synchronize()
+        self.total += self.a.elapsed_time(self.b)
+        self.count += 1
+
+    @property
+    def avg_ms(self):
+        return self.total / max(self.count, 1)
+
+
+def run_c2(torch, capi, n, steps, warmup):
+    lib = capi.lib()
+    dec = capi.make_decay(M3, MASSES3)
+    N = 3
+    dev = torch.device("cuda:0")
+    cols = torch.empty((4 * N, n), dtype=torch.float64, device=dev)
+    evw = torch.empty(n, dtype=torch.float64, device=dev)
+    out = torch.empty((4 * N, n), dtype=torch.float64, device=dev)
+    ws_bytes = lib.hpsp_unweight_workspace_size(n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    nacc = torch.zeros(1, dtype=torch.int64, device=dev)
+    sh = capi.stream_handle()
+    tg, tu = Timer(torch), Timer(torch)
+
+    def step(timed):
+        if timed:
+            tg.start()
+        capi.check(lib.hpsp_generate(ctypes.byref(dec), SEED, 0, n, capi.FP64, capi.ptr(cols), n,
+                                     0, None, capi.ptr(evw), sh), "generate")
+        if timed:
+            tg.stop()
+            tu.start()
+        capi.check(lib.hpsp_unweight(ctypes.byref(dec), None, -1.0, UW_SEED, capi.ptr(cols), n,
+                                     n, capi.ptr(evw), capi.ptr(out), n, None, None,
+                                     capi.ptr(ws), ws_bytes, capi.ptr(nacc), sh), "unweight")
+        if timed:
+            tu.stop()
+
+    for _ in range(warmup):
+        step(False)
+    torch.cuda.synchronize()
+    # kernel-level timing (separate pass, events synchronise)
+    for _ in range(max(3, min(steps, 10))):
+        step(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1000.0 / steps
+    res = {"ms_per_step": ms, "events_per_s": n / (ms / 1000.0), "n_acc": int(nacc.item()),
+           "generate_ms": tg.avg_ms, "unweight_ms": tu.avg_ms}
+    gen_bytes = n * (32 * N + 8)
+    uw_bytes = n * 64 * N
+    res["generate_GBs"] = gen_bytes / (tg.avg_ms * 1e6)
+    res["unweight_GBs"] = uw_bytes / (tu.avg_ms * 1e6)
+    res["generate_bytes_per_launch"] = gen_bytes
+    res["unweight_bytes_per_launch"] = uw_bytes
+    del cols, evw, out, ws
+    torch.cuda.empty_cache()
+    return res
+
+
+def time_generate(torch, capi, M, masses, n, prec, out_f32, reps=5):
+    lib = capi.lib()
+    dec = capi.make_decay(M, masses)
+    N = len(masses)
+    cols = torch.empty((4 * N, n), dtype=torch.float32 if out_f32 else torch.float64,
+                       device="cuda:0")
+    sh = capi.stream_handle()
+    t = Timer(torch)
+    for i in range(reps + 1):
+        if i:
+            t.start()
+        capi.check(lib.hpsp_generate(ctypes.byref(dec), SEED, 0, n, prec, capi.ptr(cols), n,
+                                     int(out_f32), None, None, sh), "generate")
+        if i:
+            t.stop()
+    return t.avg_ms, cols
+
+
+def compare_cols(a, b, N, sample=1 << 20):
+    """max / RMS of |p_a - p_b| / E over the first `sample` events."""
+    import torch
+    x = a[:, :sample].double().reshape(N, 4, -1)
+    y = b[:, :sample].double().reshape(N, 4, -1)
+    d = (x - y).abs() / y[:, 0:1, :].abs()
+    return float(d.max()), float(torch.sqrt((d * d).mean()))
+
+
+def run_c3(torch, capi):
+    res = {}
+    n = 1 << 26
+    N = 10
+    fl = genbod_flops(N)
+    for tag, masses in (("c3", [0.13957] * 10), ("c3_threshold", [0.999] * 10)):
+        ms64, c64 = time_generate(torch, capi, 10.0, masses, n, capi.FP64, False)
+        msmx, cmx = time_generate(torch, capi, 10.0, masses, n, capi.MIXED, True)
+        err_max, err_rms = compare_cols(cmx, c64, N)
+        res[tag] = {
+            "events": n, "fp64_ms": ms64, "mixed_ms": msmx,
+            "fp64_events_per_s": n / (ms64 / 1e3), "mixed_events_per_s": n / (msmx / 1e3),
+            "mixed_speedup": ms64 / msmx,
+            "fp64_TFLOPs": n * fl / (ms64 * 1e9), "fp64_vs_fp64_peak": n * fl / (ms64 * 1e9) / FP64_PEAK_TFLOPS,
+            "fp64_GBs": n * 32 * N / (ms64 * 1e6), "mixed_GBs": n * 16 * N / (msmx * 1e6),
+            "mixed_vs_fp64_momentum_err_max": err_max, "mixed_vs_fp64_momentum_err_rms": err_rms,
+            "flops_per_event": fl,
+        }
+        del c64, cmx
+        torch.cuda.empty_cache()
+    return res
+
+
+def run_c4(torch, capi, chunk=1 << 24, nchunks=64):
+    lib = capi.lib()
+    dec = capi.make_decay(M3, MASSES3)
+    f = capi.make_functor(capi.F_BW_SUM, BW3)
+    ws_bytes = lib.hpsp_average_on_workspace_size(chunk)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda:0")
+    outs = torch.empty((nchunks, 3), dtype=torch.float64, device="cuda:0")
+    sh = capi.stream_handle()
+    res = {"events": chunk * nchunks, "chunk": chunk}
+    means = {}
+    for prec, tag in ((capi.FP64, "fp64"), (capi.MIXED, "mixed")):
+        for rep in range(2):  # first pass warms up
+            t = Timer(torch)
+            t.start()
+            for c in range(nchunks):
+                o = ctypes.c_void_p(outs.data_ptr() + 24 * c)
+                capi.check(lib.hpsp_average_on(ctypes.byref(dec), ctypes.byref(f), SEED,
+                                               c * chunk, chunk, prec, capi.ptr(ws), ws_bytes, o,
+                                               sh), "average_on")
+            t.stop()
+        st = outs.cpu().numpy()
+        acc = capi.Stats(*st[0])
+        for c in range(1, nchunks):
+            r = capi.Stats()
+            lib.hpsp_stats_merge(acc, capi.Stats(*st[c]), r)
+            acc = r
+        means[tag] = (acc.w, acc.mean, acc.m2)
+        res[f"{tag}_ms"] = t.avg_ms
+        res[f"{tag}_events_per_s"] = res["events"] / (t.avg_ms / 1e3)
+        res[f"{tag}_mean"] = acc.mean
+        res[f"{tag}_sigma"] = math.sqrt(acc.m2) / acc.w
+    res["mixed_speedup"] = res["fp64_ms"] / res["mixed_ms"]
+    res["mixed_rel_err_mean"] = abs(means["mixed"][1] - means["fp64"][1]) / abs(means["fp64"][1])
+    res["mixed_rel_err_sigma"] = abs(res["mixed_sigma"] - res["fp64_sigma"]) / res["fp64_sigma"]
+    fl = genbod_flops(3)
+    res["fp64_TFLOPs_genbod_only"] = res["events"] * fl / (res["fp64_ms"] * 1e9)
+    return res
+
+
+def cpu_baseline(n=1 << 25):
+    """The reference CPU path (OMP) on a bounded sample of the headline step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    cores = len(os.sched_getaffinity(0))
+    if os.path.exists(orc.REF_LIB):
+        R = orc.Reference(orc.REF_LIB)
+        R.set_threads(cores)
+        t0 = time.perf_counter()
+        _, _, gsec = R.generate(M3, MASSES3, SEED, n, want_output=False)
+        nacc, _, _, usec = R.unweight(M3, MASSES3, SEED, n, UW_SEED, want_output=False)
+        t1 = time.perf_counter()
+        step = gsec + usec
+        return {"value": n / step, "unit": "events/s", "cores": cores, "kind": "reference",
+                "sample": f"{n} events of the same 3-body FP64 decay: PhaseSpace<3>::Generate "
+                          f"into Decays + Decays::Unweight() on the reference's OMP backend "
+                          f"(built from its headers, -O2 -ffp-contract=off), {cores} threads; "
+                          f"generate {gsec:.3f} s, unweight {usec:.3f} s",
+                "wall_s": t1 - t0}
+    O = orc.Oracle()
+    import numpy as np
+    t0 = time.perf_counter()
+    cols, _ = O.generate(M3, MASSES3, SEED, n // 8, nthreads=cores)
+    O.unweight(M3, MASSES3, cols, UW_SEED, nthreads=cores)
+    t1 = time.perf_counter()
+    return {"value": (n // 8) / (t1 - t0), "unit": "events/s", "cores": cores, "kind": "port",
+            "sample": f"{n // 8} events, oracle C restatement (OpenMP generate, serial "
+                      f"partition), {cores} threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this "
+              "build (the path shards by event index, see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from hydra_amd import capi
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    extras = {}
+    if args.only == "c3":
+        extras["c3"] = run_c3(torch, capi)
+        print(json.dumps(extras), flush=True)
+        return
+    if args.only == "c4":
+        extras["c4"] = run_c4(torch, capi)
+        print(json.dumps(extras), flush=True)
+        return
+
+    c2 = run_c2(torch, capi, args.n, args.steps, args.warmup)
+    print(f"# c2 done: {c2['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    if args.extras and not args.only:
+        extras["c3"] = run_c3(torch, capi)
+        print("# c3 done", file=sys.stderr, flush=True)
+        extras["c4"] = run_c4(torch, capi)
+        print("# c4 done", file=sys.stderr, flush=True)
+    extras["c2"] = c2
+
+    if c2["generate_ms"] >= c2["unweight_ms"]:
+        dom = ("generate (k_generate_f64<3,squares3>)", c2["generate_GBs"], c2["generate_bytes_per_launch"])
+    else:
+        dom = ("unweight (k_uw_flags + k_uw_scan + k_uw_scatter)", c2["unweight_GBs"],
+               c2["unweight_bytes_per_launch"])
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get(dom[0].split()[0])
+    roof = {"bound": "hbm", "achieved": dom[1], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dom[1] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom[0],
+            "algorithmic_bytes_per_launch": dom[2]}
+    cpu = None
+    if args.cpu_baseline:
+        cpu = cpu_baseline()
+        print("# cpu baseline done", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "phase-space events/s (FP64 generate + unweight, 3-body)",
+        "value": c2["events_per_s"],
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": c2["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (counter-based RNG, seed 159753)",
+        "config": {"workload": "3-body decay 5.279 -> (3.0969, 0.493677, 0.13957) GeV, FP64, "
+                               "Generate into SoA + Unweight()", "events_per_step": args.n,
+                   "n_daughters": 3, "engine": "squares3", "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

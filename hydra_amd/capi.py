@@ -1,0 +1,161 @@
+"""ctypes binding of libhpsp.so (include/hpsp.h) for tests and benchmarks.
+
+Device buffers are passed as raw pointers; the helpers accept torch tensors
+(which supply device memory and the current HIP stream) and plain integers.
+Loading fails loudly when the library has not been built: there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhpsp.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "hpsp.h")
+
+MAX_N = 12
+MAX_TERMS = 4
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_MASS = 2
+ERR_ENERGY = 3
+ERR_WORKSPACE = 4
+ERR_HIP = 5
+
+ENGINES = {"squares3": 0, "squares4": 1, "philox": 2, "philox_long": 3,
+           "threefry": 4, "threefry_long": 5}
+FP64, MIXED = 0, 1
+F_ONE, F_BW_SUM, F_MASS2_01 = 0, 1, 2
+
+
+class Decay(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("engine", ctypes.c_int32),
+                ("mother_mass", ctypes.c_double),
+                ("masses", ctypes.c_double * MAX_N),
+                ("ecm", ctypes.c_double), ("max_weight", ctypes.c_double),
+                ("beta", ctypes.c_double * 3)]
+
+
+class Functor(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("nterms", ctypes.c_int32),
+                ("m", ctypes.c_double * MAX_TERMS), ("g", ctypes.c_double * MAX_TERMS),
+                ("cr", ctypes.c_double * MAX_TERMS), ("ci", ctypes.c_double * MAX_TERMS),
+                ("a", ctypes.c_int32 * MAX_TERMS), ("b", ctypes.c_int32 * MAX_TERMS)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_double), ("mean", ctypes.c_double), ("m2", ctypes.c_double)]
+
+
+class HpspError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__(f"{what}: hpsp status {code}")
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_I = ctypes.c_int
+_D = ctypes.c_double
+_SZ = ctypes.c_size_t
+
+_SIGS = {
+    "hpsp_decay_init": (_I, [ctypes.POINTER(Decay), _I, _D, _P, _I]),
+    "hpsp_decay_set_mother": (_I, [ctypes.POINTER(Decay), _P]),
+    "hpsp_stats_merge": (None, [ctypes.POINTER(Stats), ctypes.POINTER(Stats),
+                                ctypes.POINTER(Stats)]),
+    "hpsp_error_string": (ctypes.c_char_p, [_I]),
+    "hpsp_rng_words": (_I, [_I, _U64, _U64, _U64, _P, _P]),
+    "hpsp_generate": (_I, [ctypes.POINTER(Decay), _U64, _U64, _U64, _I, _P, _U64, _I, _P, _P,
+                           _P]),
+    "hpsp_generate_mothers": (_I, [ctypes.POINTER(Decay), _U64, _U64, _U64, _P, _U64, _I, _P,
+                                   _U64, _I, _P, _P, _P]),
+    "hpsp_event_weight": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _P, _U64, _U64,
+                               _P, _P]),
+    "hpsp_max_weight": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _P, _U64, _U64, _P,
+                             _SZ, _P, _P]),
+    "hpsp_unweight_workspace_size": (_SZ, [_U64]),
+    "hpsp_unweight": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _D, _U64, _P, _U64,
+                           _U64, _P, _P, _U64, _P, _P, _P, _SZ, _P, _P]),
+    "hpsp_average_on_workspace_size": (_SZ, [_U64]),
+    "hpsp_average_on": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _U64, _U64, _U64,
+                             _I, _P, _SZ, _P, _P]),
+    "hpsp_average_on_mothers": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _U64, _P,
+                                     _U64, _U64, _I, _P, _SZ, _P, _P]),
+    "hpsp_malloc": (_I, [ctypes.POINTER(_P), _SZ]),
+    "hpsp_free": (_I, [_P]),
+    "hpsp_memcpy_to_device": (_I, [_P, _P, _SZ]),
+    "hpsp_memcpy_to_host": (_I, [_P, _P, _SZ]),
+    "hpsp_memcpy_device": (_I, [_P, _P, _SZ, _P]),
+    "hpsp_stream_synchronize": (_I, [_P]),
+}
+
+
+def lib():
+    """Load libhpsp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `make -C hydra_amd` "
+                "(or __graft_entry__.build()); there is no fallback path")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != OK:
+        msg = lib().hpsp_error_string(rc).decode()
+        raise HpspError(rc, f"{what} ({msg})")
+
+
+def ptr(t):
+    """Device/host address of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def make_decay(mother_mass, masses, engine="squares3", mother4=None):
+    d = Decay()
+    m = (ctypes.c_double * len(masses))(*[float(x) for x in masses])
+    e = ENGINES.get(engine, engine)
+    check(lib().hpsp_decay_init(ctypes.byref(d), len(masses), float(mother_mass), m, e),
+          "hpsp_decay_init")
+    if mother4 is not None:
+        m4 = (ctypes.c_double * 4)(*[float(x) for x in mother4])
+        check(lib().hpsp_decay_set_mother(ctypes.byref(d), m4), "hpsp_decay_set_mother")
+    return d
+
+
+def make_functor(kind=F_ONE, terms=()):
+    f = Functor()
+    f.kind = kind
+    f.nterms = len(terms)
+    for k, (M, G, cr, ci, a, b) in enumerate(terms):
+        f.m[k], f.g[k], f.cr[k], f.ci[k] = M, G, cr, ci
+        f.a[k], f.b[k] = int(a), int(b)
+    return f
+
+
+def exported_symbols():
+    """Names of the functions declared in include/hpsp.h."""
+    import re
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|size_t|const char\*)\s+(hpsp_\w+)\s*\(",
+                                 text, re.M)))

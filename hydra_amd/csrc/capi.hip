@@ -1,0 +1,417 @@
+// C ABI of libhpsp.so (declared in include/hpsp.h): argument checking,
+// host-side folding of per-decay constants, workspace carving and kernel
+// dispatch.  Host arithmetic here follows the reference's order
+// (built with -ffp-contract=off).
+#include <math.h>
+#include <string.h>
+
+#include "hpsp.h"
+#include "hpsp_launch.h"
+
+using namespace hpsp;
+
+namespace {
+
+uint64_t splitmix64_host(uint64_t* x) {
+  uint64_t z = (*x += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// Engine keys as the engines' constructors derive them from a seed:
+// squares3/4: splitmix64(seed) (squares3.h:81-84); Random123: successive
+// splitmix64 draws, one per key word (EngineR123.h:97-112).
+void engine_keys(int engine, uint64_t seed, uint64_t key[4]) {
+  uint64_t s = seed;
+  key[0] = key[1] = key[2] = key[3] = 0;
+  int nkey = 1;
+  if (engine == HPSP_PHILOX_LONG || engine == HPSP_THREEFRY) nkey = 2;
+  if (engine == HPSP_THREEFRY_LONG) nkey = 4;
+  for (int i = 0; i < nkey; ++i) key[i] = splitmix64_host(&s);
+}
+
+bool valid_engine(int e) { return e >= HPSP_SQUARES3 && e <= HPSP_THREEFRY_LONG; }
+
+int check_decay(const hpsp_decay* d) {
+  if (!d) return HPSP_ERR_INVALID_ARGUMENT;
+  if (d->n < 2 || d->n > HPSP_MAX_N) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!valid_engine(d->engine)) return HPSP_ERR_INVALID_ARGUMENT;
+  return HPSP_OK;
+}
+
+DecayK make_decayk(const hpsp_decay* d, uint64_t seed) {
+  DecayK k;
+  memset(&k, 0, sizeof(k));
+  for (int i = 0; i < HPSP_MAX_N; ++i) k.m[i] = (i < d->n) ? d->masses[i] : 0.0;
+  k.ecm = d->ecm;
+  k.maxw = d->max_weight;
+  engine_keys(d->engine, seed, k.key);
+  // Vector4R::applyBoostTo(bx,by,bz) constants (Vector4R.inl:318-337).
+  const double bx = d->beta[0], by = d->beta[1], bz = d->beta[2];
+  const double bxx = bx * bx, byy = by * by, bzz = bz * bz;
+  const double b2 = bxx + byy + bzz;
+  k.do_boost = (b2 > 0.0 && b2 < 1.0) ? 1 : 0;
+  if (k.do_boost) {
+    k.bx = bx;
+    k.by = by;
+    k.bz = bz;
+    k.bxx = bxx;
+    k.byy = byy;
+    k.bzz = bzz;
+    k.gamma = 1.0 / sqrt(1.0 - b2);
+    k.gb2 = (k.gamma - 1.0) / b2;
+    k.gb2xy = k.gb2 * bx * by;
+    k.gb2xz = k.gb2 * bx * bz;
+    k.gb2yz = k.gb2 * by * bz;
+    k.gbx = k.gamma * bx;
+    k.gby = k.gamma * by;
+    k.gbz = k.gamma * bz;
+  }
+  return k;
+}
+
+int make_functork(const hpsp_functor* f, int nvec, FunctorK* out) {
+  memset(out, 0, sizeof(*out));
+  if (!f) {
+    out->kind = HPSP_F_ONE;
+    return HPSP_OK;
+  }
+  if (f->kind < HPSP_F_ONE || f->kind > HPSP_F_MASS2_01) return HPSP_ERR_INVALID_ARGUMENT;
+  out->kind = f->kind;
+  if (f->kind == HPSP_F_BW_SUM) {
+    if (f->nterms < 0 || f->nterms > HPSP_MAX_TERMS) return HPSP_ERR_INVALID_ARGUMENT;
+    out->nterms = f->nterms;
+    for (int k = 0; k < f->nterms; ++k) {
+      if (f->a[k] < 0 || f->a[k] >= nvec || f->b[k] < 0 || f->b[k] >= nvec)
+        return HPSP_ERR_INVALID_ARGUMENT;
+      out->m[k] = f->m[k];
+      out->g[k] = f->g[k];
+      out->cr[k] = f->cr[k];
+      out->ci[k] = f->ci[k];
+      out->a[k] = f->a[k];
+      out->b[k] = f->b[k];
+    }
+  }
+  if (f->kind == HPSP_F_MASS2_01 && nvec < 2) return HPSP_ERR_INVALID_ARGUMENT;
+  return HPSP_OK;
+}
+
+int hip_status(int e) { return e == (int)hipSuccess ? HPSP_OK : HPSP_ERR_HIP; }
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+struct UwLayout {
+  size_t flags, tile_count, tile_off, maxv, block_max, total;
+};
+
+UwLayout uw_layout(uint64_t n) {
+  const uint64_t ngroups = (n + kWave - 1) / kWave + kUwTile / kWave;
+  const uint64_t ntiles = (n + kUwTile - 1) / kUwTile + 1;
+  const uint64_t nblocks = grid_for(n);
+  UwLayout L;
+  L.flags = 0;
+  L.tile_count = L.flags + align_up(ngroups * sizeof(uint64_t));
+  L.tile_off = L.tile_count + align_up(ntiles * sizeof(uint32_t));
+  L.maxv = L.tile_off + align_up(ntiles * sizeof(uint64_t));
+  L.block_max = L.maxv + align_up(sizeof(double));
+  L.total = L.block_max + align_up(nblocks * sizeof(double));
+  return L;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace
+
+extern "C" {
+
+int hpsp_decay_init(hpsp_decay* d, int n, double mother_mass, const double* masses,
+                    int engine) {
+  if (!d || !masses || n < 2 || n > HPSP_MAX_N || !valid_engine(engine))
+    return HPSP_ERR_INVALID_ARGUMENT;
+  memset(d, 0, sizeof(*d));
+  if (mother_mass <= 0.0) return HPSP_ERR_MASS;
+  for (int i = 0; i < n; ++i)
+    if (masses[i] <= 0.0) return HPSP_ERR_MASS;
+  d->n = n;
+  d->engine = engine;
+  d->mother_mass = mother_mass;
+  for (int i = 0; i < n; ++i) d->masses[i] = masses[i];
+  double ecm = mother_mass;
+  for (int i = 0; i < n; ++i) ecm -= masses[i];
+  if (ecm <= 0.0) return HPSP_ERR_ENERGY;
+  double emmax = ecm + masses[0];
+  double emmin = 0.0;
+  double wtmax = 1.0;
+  for (int i = 1; i < n; ++i) {
+    emmin += masses[i - 1];
+    emmax += masses[i];
+    double x = (emmax - emmin - masses[i]) * (emmax + emmin + masses[i]) *
+               (emmax - emmin + masses[i]) * (emmax + emmin - masses[i]);
+    x = sqrt(x) / (2 * emmax);
+    wtmax *= x;
+  }
+  d->ecm = ecm;
+  d->max_weight = 1.0 / wtmax;
+  return HPSP_OK;
+}
+
+int hpsp_decay_set_mother(hpsp_decay* d, const double mother4[4]) {
+  if (!d || !mother4) return HPSP_ERR_INVALID_ARGUMENT;
+  const double e = mother4[0];
+  const double d3 = sqrt(mother4[1] * mother4[1] + mother4[2] * mother4[2] + mother4[3] * mother4[3]);
+  const double beta = d3 / e;
+  if (beta != 0.0) {
+    const double w = beta / d3;
+    d->beta[0] = mother4[0] * w;
+    d->beta[1] = mother4[1] * w;
+    d->beta[2] = mother4[2] * w;
+  } else {
+    d->beta[0] = d->beta[1] = d->beta[2] = 0.0;
+  }
+  return HPSP_OK;
+}
+
+void hpsp_stats_merge(const hpsp_stats* x, const hpsp_stats* y, hpsp_stats* out) {
+  const double w = x->w + y->w;
+  const double delta = y->mean - x->mean;
+  const double delta2 = delta * delta;
+  hpsp_stats r;
+  r.w = w;
+  r.mean = (x->mean * x->w + y->mean * y->w) / w;
+  r.m2 = x->m2 + y->m2;
+  r.m2 += delta2 * x->w * y->w / w;
+  *out = r;
+}
+
+const char* hpsp_error_string(int status) {
+  switch (status) {
+    case HPSP_OK: return "ok";
+    case HPSP_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case HPSP_ERR_MASS: return "mass of mother or daughter particle is negative or zero";
+    case HPSP_ERR_ENERGY: return "not enough energy to generate this decay (fECM <= 0)";
+    case HPSP_ERR_WORKSPACE: return "workspace missing or too small";
+    case HPSP_ERR_HIP: return "HIP runtime error";
+    case HPSP_ERR_NO_DEVICE: return "no HIP device";
+    default: return "unknown status";
+  }
+}
+
+int hpsp_rng_words(int engine, uint64_t seed, uint64_t discard, uint64_t n, uint64_t* out,
+                   void* stream) {
+  if (!valid_engine(engine)) return HPSP_ERR_INVALID_ARGUMENT;
+  if (n == 0) return HPSP_OK;
+  if (!out) return HPSP_ERR_INVALID_ARGUMENT;
+  uint64_t key[4];
+  engine_keys(engine, seed, key);
+  return hip_status(launch_rng_words(engine, key, discard, n, out, as_stream(stream)));
+}
+
+static int generate_common(const hpsp_decay* d, uint64_t seed, uint64_t first, uint64_t n,
+                           const double* mothers, uint64_t mld, int prec, void* cols,
+                           uint64_t ld, int out_f32, double* wt, double* evw, void* stream) {
+  int rc = check_decay(d);
+  if (rc) return rc;
+  if (prec != HPSP_FP64 && prec != HPSP_MIXED) return HPSP_ERR_INVALID_ARGUMENT;
+  if (n == 0) return HPSP_OK;
+  if (!cols || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
+  if (mothers && mld < n) return HPSP_ERR_INVALID_ARGUMENT;
+  GenArgs a;
+  memset(&a, 0, sizeof(a));
+  a.d = make_decayk(d, seed);
+  a.first = first;
+  a.n = n;
+  a.cols = cols;
+  a.ld = ld;
+  a.out_f32 = out_f32 ? 1 : 0;
+  a.wt = wt;
+  a.evw = evw;
+  a.mothers = mothers;
+  a.mld = mld;
+  const int e = (prec == HPSP_FP64) ? launch_generate_f64(d->n, d->engine, a, as_stream(stream))
+                                    : launch_generate_mixed(d->n, d->engine, a, as_stream(stream));
+  return hip_status(e);
+}
+
+int hpsp_generate(const hpsp_decay* d, uint64_t seed, uint64_t first, uint64_t n, int prec,
+                  void* cols, uint64_t ld, int out_f32, double* wt, double* evw, void* stream) {
+  return generate_common(d, seed, first, n, nullptr, 0, prec, cols, ld, out_f32, wt, evw, stream);
+}
+
+int hpsp_generate_mothers(const hpsp_decay* d, uint64_t seed, uint64_t first, uint64_t n,
+                          const double* mothers, uint64_t mld, int prec, void* cols,
+                          uint64_t ld, int out_f32, double* wt, double* evw, void* stream) {
+  if (n > 0 && !mothers) return HPSP_ERR_INVALID_ARGUMENT;
+  return generate_common(d, seed, first, n, mothers, mld, prec, cols, ld, out_f32, wt, evw,
+                         stream);
+}
+
+int hpsp_event_weight(const hpsp_decay* d, const hpsp_functor* f, const double* cols,
+                      uint64_t ld, uint64_t n, double* w_out, void* stream) {
+  int rc = check_decay(d);
+  if (rc) return rc;
+  if (n == 0) return HPSP_OK;
+  if (!cols || !w_out || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
+  WeightArgs a;
+  memset(&a, 0, sizeof(a));
+  a.d = make_decayk(d, 0);
+  rc = make_functork(f, d->n, &a.f);
+  if (rc) return rc;
+  a.use_f = f ? 1 : 0;
+  a.cols = cols;
+  a.ld = ld;
+  a.n = n;
+  a.w_out = w_out;
+  return hip_status(launch_event_weight(d->n, a, as_stream(stream)));
+}
+
+size_t hpsp_unweight_workspace_size(uint64_t n) { return uw_layout(n).total; }
+
+int hpsp_max_weight(const hpsp_decay* d, const hpsp_functor* f, const double* cols,
+                    uint64_t ld, uint64_t n, void* ws, size_t ws_bytes, double* max_dev,
+                    void* stream) {
+  int rc = check_decay(d);
+  if (rc) return rc;
+  if (n == 0 || !cols || !max_dev || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
+  const UwLayout L = uw_layout(n);
+  if (!ws || ws_bytes < L.total) return HPSP_ERR_WORKSPACE;
+  WeightArgs a;
+  memset(&a, 0, sizeof(a));
+  a.d = make_decayk(d, 0);
+  rc = make_functork(f, d->n, &a.f);
+  if (rc) return rc;
+  a.use_f = f ? 1 : 0;
+  a.cols = cols;
+  a.ld = ld;
+  a.n = n;
+  char* base = static_cast<char*>(ws);
+  return hip_status(launch_max_weight(d->n, a, reinterpret_cast<double*>(base + L.block_max),
+                                      grid_for(n), max_dev, as_stream(stream)));
+}
+
+int hpsp_unweight(const hpsp_decay* d, const hpsp_functor* f, double max_value, uint64_t seed,
+                  const double* cols_in, uint64_t ld_in, uint64_t n, const double* w_in,
+                  double* cols_out, uint64_t ld_out, uint64_t* idx_out, double* w_out,
+                  void* ws, size_t ws_bytes, uint64_t* n_acc_dev, void* stream) {
+  int rc = check_decay(d);
+  if (rc) return rc;
+  if (!n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (n > 0) {
+    if (!cols_in || ld_in < n) return HPSP_ERR_INVALID_ARGUMENT;
+    if (cols_out && ld_out < n) return HPSP_ERR_INVALID_ARGUMENT;
+    if (w_out && !w_in) return HPSP_ERR_INVALID_ARGUMENT;
+  }
+  const UwLayout L = uw_layout(n);
+  if (!ws || ws_bytes < L.total) return HPSP_ERR_WORKSPACE;
+  char* base = static_cast<char*>(ws);
+  hipStream_t s = as_stream(stream);
+
+  UnweightArgs a;
+  memset(&a, 0, sizeof(a));
+  a.d = make_decayk(d, 0);
+  rc = make_functork(f, d->n, &a.f);
+  if (rc) return rc;
+  a.use_f = f ? 1 : 0;
+  a.maxv = d->max_weight;
+  a.maxv_dev = nullptr;
+  if (f) {
+    if (max_value > 0.0) {
+      a.maxv = max_value;
+    } else if (n > 0) {
+      double* mv = reinterpret_cast<double*>(base + L.maxv);
+      rc = hpsp_max_weight(d, f, cols_in, ld_in, n, ws, ws_bytes, mv, stream);
+      if (rc) return rc;
+      a.maxv_dev = mv;
+    }
+  }
+  uint64_t s0 = seed;
+  a.uw_key = splitmix64_host(&s0);
+  a.cols_in = cols_in;
+  a.ld_in = ld_in;
+  a.n = n;
+  a.w_in = w_in;
+  a.cols_out = cols_out;
+  a.ld_out = ld_out;
+  a.idx_out = idx_out;
+  a.w_out = w_out;
+  a.flags = reinterpret_cast<uint64_t*>(base + L.flags);
+  a.tile_count = reinterpret_cast<uint32_t*>(base + L.tile_count);
+  a.tile_off = reinterpret_cast<uint64_t*>(base + L.tile_off);
+  a.n_acc = n_acc_dev;
+  return hip_status(launch_unweight(d->n, a, s));
+}
+
+size_t hpsp_average_on_workspace_size(uint64_t n) {
+  const uint64_t m = (n + kAvgChunk - 1) / kAvgChunk;
+  return align_up((m ? m : 1) * sizeof(Stats));
+}
+
+static int average_common(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed,
+                          uint64_t first, uint64_t n, const double* mothers, uint64_t mld,
+                          int prec, void* ws, size_t ws_bytes, hpsp_stats* out_dev,
+                          void* stream) {
+  int rc = check_decay(d);
+  if (rc) return rc;
+  if (prec != HPSP_FP64 && prec != HPSP_MIXED) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!out_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (mothers && mld < n) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!ws || ws_bytes < hpsp_average_on_workspace_size(n)) return HPSP_ERR_WORKSPACE;
+  AvgArgs a;
+  memset(&a, 0, sizeof(a));
+  a.d = make_decayk(d, seed);
+  rc = make_functork(f, d->n + (mothers ? 1 : 0), &a.f);
+  if (rc) return rc;
+  a.first = first;
+  a.n = n;
+  a.mothers = mothers;
+  a.mld = mld;
+  a.partials = static_cast<Stats*>(ws);
+  hipStream_t s = as_stream(stream);
+  const uint64_t m = (n + kAvgChunk - 1) / kAvgChunk;
+  if (n > 0) {
+    const int e = (prec == HPSP_FP64) ? launch_average_f64(d->n, d->engine, a, s)
+                                      : launch_average_mixed(d->n, d->engine, a, s);
+    if (e) return HPSP_ERR_HIP;
+  }
+  return hip_status(launch_stats_finalize(a.partials, m, reinterpret_cast<Stats*>(out_dev), s));
+}
+
+int hpsp_average_on(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed, uint64_t first,
+                    uint64_t n, int prec, void* ws, size_t ws_bytes, hpsp_stats* out_dev,
+                    void* stream) {
+  return average_common(d, f, seed, first, n, nullptr, 0, prec, ws, ws_bytes, out_dev, stream);
+}
+
+int hpsp_average_on_mothers(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed,
+                            const double* mothers, uint64_t mld, uint64_t n, int prec,
+                            void* ws, size_t ws_bytes, hpsp_stats* out_dev, void* stream) {
+  if (n > 0 && !mothers) return HPSP_ERR_INVALID_ARGUMENT;
+  return average_common(d, f, seed, 0, n, mothers, mld, prec, ws, ws_bytes, out_dev, stream);
+}
+
+int hpsp_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return HPSP_ERR_INVALID_ARGUMENT;
+  return hip_status((int)hipMalloc(ptr, bytes ? bytes : 1));
+}
+
+int hpsp_free(void* ptr) { return hip_status((int)hipFree(ptr)); }
+
+int hpsp_memcpy_to_device(void* dst, const void* src, size_t bytes) {
+  return hip_status((int)hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+
+int hpsp_memcpy_to_host(void* dst, const void* src, size_t bytes) {
+  return hip_status((int)hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+
+int hpsp_memcpy_device(void* dst, const void* src, size_t bytes, void* stream) {
+  return hip_status(
+      (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+}
+
+int hpsp_stream_synchronize(void* stream) {
+  return hip_status((int)hipStreamSynchronize(as_stream(stream)));
+}
+
+}  // extern "C"

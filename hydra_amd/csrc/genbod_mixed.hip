@@ -1,0 +1,339 @@
+// Mixed-precision phase-space kernels for gfx950.  The weight chain stays
+// FP64 and follows the reference operation for operation (the uniforms of
+// the invariant masses, invMas, pdk -- where a - b - c cancels near
+// threshold -- and the weight product), so generation weights are the FP64
+// ones bit for bit.  The kinematics after pdk (energies, rotations, boosts,
+// functor) run in FP32 with FMA contraction, hardware sin/cos in revolutions
+// and reciprocal square roots.  Accumulation of AverageOn stays FP64.
+#include <array>
+#include <type_traits>
+#include <utility>
+
+#include "hpsp_launch.h"
+
+namespace hpsp {
+
+// The 32 bits of a word that feed the FP32 angles (Random123 words: the top
+// 32 bits, so u differs from the FP64 path by less than 2^-32).
+template <int ENGINE>
+__device__ __forceinline__ uint32_t word32(uint64_t w) {
+  if constexpr (is_squares<ENGINE>())
+    return (uint32_t)w;
+  else
+    return (uint32_t)(w >> 32);
+}
+
+// cos and sin of the polar angle from u = w / 2^32 without cancellation:
+// cZ = 2u - 1 = (2w - 2^32) / 2^32 and sZ = sqrt(1 - cZ^2) = 2 sqrt(w (2^32 - w)) / 2^32,
+// both formed from exact integers and rounded once to FP32.
+__device__ __forceinline__ void polar_f32(uint32_t w, float& cZ, float& sZ) {
+  const int64_t t = 2 * (int64_t)w - (int64_t)(1ULL << 32);
+  cZ = (float)t * 0x1p-32f;
+  const float a = (float)w;
+  const float b = (float)((1ULL << 32) - (uint64_t)w);
+  sZ = __builtin_sqrtf(a * b) * 0x1p-31f;
+}
+
+// FP64 weight chain of DecayMother::process (DecayMother.h:170-212):
+// sorted uniforms, invMas, pdk and the weight product, operation for
+// operation; the momenta pd are handed on in FP32.
+template <int N, int ENGINE, typename WordFn>
+__device__ __forceinline__ double weight_chain(const DecayK& d, const WordFn& word,
+                                               double (&invMas)[N], float (&pd)[N - 1]) {
+#pragma clang fp contract(off)
+  double rno[N];
+  rno[0] = 0.0;
+  rno[N - 1] = 1.0;
+  if constexpr (N > 2) {
+    using WT = std::conditional_t<is_squares<ENGINE>(), uint32_t, uint64_t>;
+    WT v[N - 2];
+#pragma unroll
+    for (int k = 0; k < N - 2; ++k) v[k] = (WT)word(k);
+    sort_net(v);
+#pragma unroll
+    for (int k = 0; k < N - 2; ++k) rno[k + 1] = to_unit<ENGINE>(v[k]);
+  }
+  double sum = 0.0;
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    sum += d.m[n];
+    invMas[n] = rno[n] * d.ecm + sum;
+  }
+  double wt = d.maxw;
+#pragma unroll
+  for (int n = 0; n < N - 1; ++n) {
+    const double q = pdk(invMas[n + 1], invMas[n], d.m[n + 1]);
+    wt *= q;
+    pd[n] = (float)q;
+  }
+  return wt;
+}
+
+template <int N, int ENGINE, typename WordFn>
+__device__ __forceinline__ double genbod_mixed(const DecayK& d, const WordFn& word,
+                                               float (&p)[N][4]) {
+  float pd[N - 1];
+  double invMas[N];
+  const double wt = weight_chain<N, ENGINE>(d, word, invMas, pd);
+
+  const float m0 = (float)d.m[0];
+  p[0][0] = __builtin_sqrtf(pd[0] * pd[0] + m0 * m0);
+  p[0][1] = 0.0f;
+  p[0][2] = pd[0];
+  p[0][3] = 0.0f;
+
+#pragma unroll
+  for (int i = 1; i < N; ++i) {
+    const float mi = (float)d.m[i];
+    p[i][0] = __builtin_sqrtf(pd[i - 1] * pd[i - 1] + mi * mi);
+    p[i][1] = 0.0f;
+    p[i][2] = -pd[i - 1];
+    p[i][3] = 0.0f;
+
+    float cZ, sZ;
+    polar_f32(word32<ENGINE>(word(N - 2 + 2 * (i - 1))), cZ, sZ);
+    const float rev = (float)word32<ENGINE>(word(N - 2 + 2 * (i - 1) + 1)) * 0x1p-32f;
+    // v_sin_f32 / v_cos_f32 take the angle in revolutions: sin(2*pi*rev)
+    const float sY = __builtin_amdgcn_sinf(rev);
+    const float cY = __builtin_amdgcn_cosf(rev);
+
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      const bool fresh = (j == i) || (i == 1);
+      if (fresh) {
+        const float y = p[j][2];
+        const float t = -(sZ * y);
+        p[j][2] = cZ * y;
+        p[j][1] = cY * t;
+        p[j][3] = sY * t;
+      } else {
+        float x = p[j][1];
+        const float y = p[j][2];
+        p[j][1] = cZ * x - sZ * y;
+        p[j][2] = sZ * x + cZ * y;
+        x = p[j][1];
+        const float z = p[j][3];
+        p[j][1] = cY * x - sY * z;
+        p[j][3] = sY * x + cY * z;
+      }
+    }
+    if (i == N - 1) break;
+
+    // beta = pd/sqrt(pd^2 + M^2); gamma = 1/sqrt(1-beta^2);
+    // (gamma-1)/beta^2 written as gamma^2/(gamma+1) (no cancellation).
+    const float M = (float)invMas[i];
+    const float pi2 = pd[i] * pd[i];
+    const float beta = pd[i] * __builtin_amdgcn_rsqf(pi2 + M * M);
+    const float b2 = beta * beta;
+    const float gamma = __builtin_amdgcn_rsqf(1.0f - b2);
+    const float gb2yy = gamma * gamma / (gamma + 1.0f) * b2;
+    const float gby = gamma * beta;
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      const float e2 = p[j][0], py2 = p[j][2];
+      p[j][0] = gamma * e2 + gby * py2;
+      p[j][2] = gby * e2 + gb2yy * py2 + py2;
+    }
+  }
+  return wt;
+}
+
+__device__ __forceinline__ void boost_general_f32(float (&v)[4], float bx, float by, float bz) {
+  const float b2 = bx * bx + by * by + bz * bz;
+  if (b2 > 0.0f && b2 < 1.0f) {
+    const float gamma = __builtin_amdgcn_rsqf(1.0f - b2);
+    const float gb2 = gamma * gamma / (gamma + 1.0f);
+    const float e2 = v[0], px2 = v[1], py2 = v[2], pz2 = v[3];
+    const float bp = bx * px2 + by * py2 + bz * pz2;
+    v[0] = gamma * (e2 + bp);
+    const float k = gb2 * bp + gamma * e2;
+    v[1] = px2 + k * bx;
+    v[2] = py2 + k * by;
+    v[3] = pz2 + k * bz;
+  }
+}
+
+template <int N, int ENGINE>
+struct EventWordsM {
+  static constexpr int W = 3 * N - 4;
+  std::conditional_t<is_squares<ENGINE>(), WaveWords, char> ww;
+  uint64_t w[is_squares<ENGINE>() ? 1 : W];
+  __device__ __forceinline__ void init(const DecayK& d, uint64_t wave_first_evt, int lane) {
+    if constexpr (is_squares<ENGINE>()) {
+      const uint64_t c0 = wave_first_evt + 3 * (uint64_t)N;
+      ww.lane = lane;
+      ww.lo = squares_word<ENGINE>(d.key[0], c0 + (uint64_t)lane);
+      ww.hi = squares_word<ENGINE>(d.key[0], c0 + kWave + (uint64_t)lane);
+    } else {
+      r123_words<ENGINE, W>(d.key, wave_first_evt + (uint64_t)lane + 3 * (uint64_t)N, w);
+    }
+  }
+  __device__ __forceinline__ uint64_t operator()(int j) const {
+    if constexpr (is_squares<ENGINE>())
+      return ww.get(j);
+    else
+      return w[j];
+  }
+};
+
+template <int N>
+__device__ __forceinline__ void final_boost_f32(float (&p)[N][4], const DecayK& d) {
+  if (d.do_boost) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) boost_general_f32(p[i], (float)d.bx, (float)d.by, (float)d.bz);
+  }
+}
+
+template <int N, int ENGINE, bool MOTHERS>
+__global__ __launch_bounds__(kBlock) void k_generate_mixed(const GenArgs a) {
+  const int lane = __lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < a.n; base += stride) {
+    const uint64_t k0 = base + (uint64_t)wave * kWave;
+    const uint64_t k = k0 + (uint64_t)lane;
+    const bool live = k < a.n;
+    EventWordsM<N, ENGINE> words;
+    words.init(a.d, a.first + k0, lane);
+    float p[N][4];
+    const double wt = genbod_mixed<N, ENGINE>(a.d, words, p);
+    if constexpr (MOTHERS) {
+      const uint64_t km = live ? k : 0;
+      const float e = (float)a.mothers[km];
+      const float bx = (float)a.mothers[a.mld + km] / e;
+      const float by = (float)a.mothers[2 * a.mld + km] / e;
+      const float bz = (float)a.mothers[3 * a.mld + km] / e;
+#pragma unroll
+      for (int i = 0; i < N; ++i) boost_general_f32(p[i], bx, by, bz);
+    } else {
+      final_boost_f32<N>(p, a.d);
+    }
+    if (live) {
+      if (a.out_f32) {
+        float* c = static_cast<float*>(a.cols) + k0;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            __builtin_nontemporal_store(p[i][q], uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane);
+      } else {
+        double* c = static_cast<double*>(a.cols) + k0;
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            __builtin_nontemporal_store((double)p[i][q],
+                                        uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane);
+      }
+      if (a.wt) __builtin_nontemporal_store(wt, uniform_ptr(a.wt + k0) + lane);
+      if (a.evw) {
+        double pp[N][4];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) pp[i][q] = (double)p[i][q];
+        __builtin_nontemporal_store(event_weight_f64<N>(pp, a.d.m, a.d.maxw),
+                                    uniform_ptr(a.evw + k0) + lane);
+      }
+    }
+  }
+}
+
+template <int N, int ENGINE, bool MOTHERS>
+__global__ __launch_bounds__(kBlock) void k_average_mixed(const AvgArgs a) {
+  __shared__ Stats sh[kBlock / kWave];
+  const int lane = __lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t nchunks = (a.n + kAvgChunk - 1) / kAvgChunk;
+  for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    Stats acc = {0.0, 0.0, 0.0};
+    for (int it = 0; it < kAvgChunk / kBlock; ++it) {
+      const uint64_t k0 = c * kAvgChunk + (uint64_t)it * kBlock + (uint64_t)wave * kWave;
+      const uint64_t k = k0 + (uint64_t)lane;
+      const bool live = k < a.n;
+      EventWordsM<N, ENGINE> words;
+      words.init(a.d, a.first + k0, lane);
+      float p[N + (MOTHERS ? 1 : 0)][4];
+      double wt;
+      if constexpr (MOTHERS) {
+        float q[N][4];
+        wt = genbod_mixed<N, ENGINE>(a.d, words, q);
+        const uint64_t km = live ? k : 0;
+        float m4[4];
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) m4[cc] = (float)a.mothers[(uint64_t)cc * a.mld + km];
+        const float bx = m4[1] / m4[0], by = m4[2] / m4[0], bz = m4[3] / m4[0];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          boost_general_f32(q[i], bx, by, bz);
+#pragma unroll
+          for (int cc = 0; cc < 4; ++cc) p[i + 1][cc] = q[i][cc];
+        }
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) p[0][cc] = m4[cc];
+      } else {
+        wt = genbod_mixed<N, ENGINE>(a.d, words, p);
+        final_boost_f32<N>(p, a.d);
+      }
+      const double f = functor_eval<float>(a.f, p);
+      if (live) stats_push(acc, f, wt);
+    }
+    acc = wave_reduce_stats(acc);
+    if (lane == 0) sh[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      a.partials[c] = stats_merge(stats_merge(sh[0], sh[1]), stats_merge(sh[2], sh[3]));
+    __syncthreads();
+  }
+}
+
+constexpr int kNumN = HPSP_MAX_N - 1;
+constexpr int kNumE = 6;
+
+template <int N, int E>
+static void gen_one(const GenArgs& a, hipStream_t s) {
+  const unsigned g = grid_for(a.n, 1u << 20);
+  if (a.mothers)
+    hipLaunchKernelGGL((k_generate_mixed<N, E, true>), dim3(g), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_generate_mixed<N, E, false>), dim3(g), dim3(kBlock), 0, s, a);
+}
+
+template <int N, int E>
+static void avg_one(const AvgArgs& a, hipStream_t s) {
+  const uint64_t nchunks = (a.n + kAvgChunk - 1) / kAvgChunk;
+  const unsigned g = (unsigned)(nchunks < 8192 ? (nchunks ? nchunks : 1) : 8192);
+  if (a.mothers)
+    hipLaunchKernelGGL((k_average_mixed<N, E, true>), dim3(g), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_average_mixed<N, E, false>), dim3(g), dim3(kBlock), 0, s, a);
+}
+
+template <int I>
+static constexpr auto gen_at = &gen_one<2 + I / kNumE, I % kNumE>;
+template <int I>
+static constexpr auto avg_at = &avg_one<2 + I / kNumE, I % kNumE>;
+
+template <int... I>
+static constexpr auto gen_table(std::integer_sequence<int, I...>) {
+  return std::array<void (*)(const GenArgs&, hipStream_t), sizeof...(I)>{gen_at<I>...};
+}
+template <int... I>
+static constexpr auto avg_table(std::integer_sequence<int, I...>) {
+  return std::array<void (*)(const AvgArgs&, hipStream_t), sizeof...(I)>{avg_at<I>...};
+}
+
+static const auto kGen = gen_table(std::make_integer_sequence<int, kNumN * kNumE>{});
+static const auto kAvg = avg_table(std::make_integer_sequence<int, kNumN * kNumE>{});
+
+int launch_generate_mixed(int N, int engine, const GenArgs& a, hipStream_t s) {
+  kGen[(N - 2) * kNumE + engine](a, s);
+  return (int)hipGetLastError();
+}
+
+int launch_average_mixed(int N, int engine, const AvgArgs& a, hipStream_t s) {
+  kAvg[(N - 2) * kNumE + engine](a, s);
+  return (int)hipGetLastError();
+}
+
+}  // namespace hpsp

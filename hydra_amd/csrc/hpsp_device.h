@@ -1,0 +1,504 @@
+// Device-side building blocks shared by the gfx950 kernels: counter-based
+// engines, the word -> [0,1) map, the GENBOD pieces and the StatsPHSP merge.
+// Written for CDNA4 wave64; nothing here is a CUDA translation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hpsp.h"
+
+namespace hpsp {
+
+constexpr int kBlock = 256;  // threads per block (4 waves of 64)
+constexpr int kWave = 64;
+
+// Minimum waves per SIMD requested from the register allocator for the
+// one-event-per-lane kernels: the FP64 chains are long and dependent, so
+// occupancy hides their latency; larger N needs more live momenta.
+constexpr int waves_for(int N) { return N <= 4 ? 4 : (N <= 6 ? 3 : (N <= 10 ? 2 : 1)); }
+
+// Kernel-side decay description; everything that depends only on the decay
+// is folded on the host (engine keys, final boost constants).
+struct DecayK {
+  double m[HPSP_MAX_N];
+  double ecm;
+  double maxw;
+  uint64_t key[4];  // engine key words, already splitmix-ed
+  // Final boost of a single mother (Vector4R.inl:312-369) with its derived
+  // constants, valid when do_boost != 0 (0 < b2 < 1).
+  double bx, by, bz, bxx, byy, bzz, gamma, gb2, gb2xy, gb2xz, gb2yz, gbx, gby, gbz;
+  int do_boost;
+};
+
+struct FunctorK {
+  int kind;
+  int nterms;
+  double m[HPSP_MAX_TERMS], g[HPSP_MAX_TERMS], cr[HPSP_MAX_TERMS], ci[HPSP_MAX_TERMS];
+  int a[HPSP_MAX_TERMS], b[HPSP_MAX_TERMS];
+};
+
+// A wave-uniform pointer pinned to SGPRs, so that per-lane accesses are
+// (scalar base) + lane offset and no 64-bit VGPR address is kept per column.
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+// ---------------------------------------------------------------------------
+// Engines
+
+__device__ __forceinline__ uint64_t rot32(uint64_t x) { return (x >> 32) | (x << 32); }
+
+// squares3 word for counter ctr (squares3.h:105-119)
+__device__ __forceinline__ uint32_t squares3(uint64_t key, uint64_t ctr) {
+  uint64_t y = key * ctr;
+  uint64_t x = y;
+  uint64_t z = y + key;
+  x = x * x + y;
+  x = rot32(x);
+  x = x * x + z;
+  x = rot32(x);
+  return (uint32_t)((x * x + y) >> 32);
+}
+
+// squares4 word (squares4.h:106-122)
+__device__ __forceinline__ uint32_t squares4(uint64_t key, uint64_t ctr) {
+  uint64_t y = key * ctr;
+  uint64_t x = y;
+  uint64_t z = y + key;
+  x = x * x + y;
+  x = rot32(x);
+  x = x * x + z;
+  x = rot32(x);
+  x = x * x + y;
+  x = rot32(x);
+  return (uint32_t)((x * x + z) >> 32);
+}
+
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, unsigned n) {
+  return (x << n) | (x >> ((64 - n) & 63));
+}
+
+// Philox2x64, R rounds (Salmon et al. SC'11): multiply-hi/lo round, Weyl key bump.
+template <int R>
+__device__ __forceinline__ void philox2x64(uint64_t c0, uint64_t c1, uint64_t k,
+                                           uint64_t& o0, uint64_t& o1) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (r > 0) k += 0x9E3779B97F4A7C15ULL;
+    uint64_t lo = 0xD2B74407B1CE6E93ULL * c0;
+    uint64_t hi = __umul64hi(0xD2B74407B1CE6E93ULL, c0);
+    c0 = hi ^ k ^ c1;
+    c1 = lo;
+  }
+  o0 = c0;
+  o1 = c1;
+}
+
+template <int R>
+__device__ __forceinline__ void philox4x64(uint64_t (&c)[4], uint64_t k0, uint64_t k1) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B97F4A7C15ULL;
+      k1 += 0xBB67AE8584CAA73BULL;
+    }
+    uint64_t lo0 = 0xD2E7470EE14C6C93ULL * c[0];
+    uint64_t hi0 = __umul64hi(0xD2E7470EE14C6C93ULL, c[0]);
+    uint64_t lo1 = 0xCA5A826395121157ULL * c[2];
+    uint64_t hi1 = __umul64hi(0xCA5A826395121157ULL, c[2]);
+    uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+  }
+}
+
+constexpr uint64_t kSkeinParity = 0x1BD11BDAA9FC1A22ULL;
+
+// Threefry2x64, R rounds (Threefish-style mix with key injection every 4 rounds).
+template <int R>
+__device__ __forceinline__ void threefry2x64(uint64_t c0, uint64_t c1, uint64_t k0, uint64_t k1,
+                                             uint64_t& o0, uint64_t& o1) {
+  constexpr unsigned rot[8] = {16, 42, 12, 31, 16, 32, 24, 21};
+  const uint64_t ks[3] = {k0, k1, kSkeinParity ^ k0 ^ k1};
+  uint64_t x0 = c0 + ks[0], x1 = c1 + ks[1];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    x0 += x1;
+    x1 = rotl64(x1, rot[r % 8]);
+    x1 ^= x0;
+    if ((r % 4) == 3) {
+      const int s = (r + 1) / 4;
+      x0 += ks[s % 3];
+      x1 += ks[(s + 1) % 3];
+      x1 += (uint64_t)s;
+    }
+  }
+  o0 = x0;
+  o1 = x1;
+}
+
+template <int R>
+__device__ __forceinline__ void threefry4x64(uint64_t (&x)[4], const uint64_t (&k)[4]) {
+  constexpr unsigned rot[8][2] = {{14, 16}, {52, 57}, {23, 40}, {5, 37},
+                                  {25, 33}, {46, 12}, {58, 22}, {32, 32}};
+  const uint64_t ks[5] = {k[0], k[1], k[2], k[3], kSkeinParity ^ k[0] ^ k[1] ^ k[2] ^ k[3]};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) x[i] += ks[i];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if ((r % 2) == 0) {
+      x[0] += x[1]; x[1] = rotl64(x[1], rot[r % 8][0]); x[1] ^= x[0];
+      x[2] += x[3]; x[3] = rotl64(x[3], rot[r % 8][1]); x[3] ^= x[2];
+    } else {
+      x[0] += x[3]; x[3] = rotl64(x[3], rot[r % 8][0]); x[3] ^= x[0];
+      x[2] += x[1]; x[1] = rotl64(x[1], rot[r % 8][1]); x[1] ^= x[2];
+    }
+    if ((r % 4) == 3) {
+      const int s = (r + 1) / 4;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[i] += ks[(s + i) % 5];
+      x[3] += (uint64_t)s;
+    }
+  }
+}
+
+// Multi-word counter increment with carry (array.h:151-183 semantics).
+template <int W>
+__device__ __forceinline__ void ctr_add(uint64_t* v, uint64_t n) {
+  v[0] += n;
+  bool carry = v[0] < n;
+#pragma unroll
+  for (int i = 1; i < W; ++i) {
+    v[i] += carry ? 1ULL : 0ULL;
+    carry = carry && (v[i] == 0);
+  }
+}
+
+// One 2- or 4-word block of a Random123 engine for a counter value.
+template <int ENGINE>
+__device__ __forceinline__ void r123_block(const uint64_t* key, const uint64_t (&ctr)[4],
+                                           uint64_t (&out)[4]) {
+  if constexpr (ENGINE == HPSP_PHILOX) {
+    philox2x64<7>(ctr[0], ctr[1], key[0], out[0], out[1]);
+  } else if constexpr (ENGINE == HPSP_PHILOX_LONG) {
+    uint64_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox4x64<7>(c, key[0], key[1]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[i] = c[i];
+  } else if constexpr (ENGINE == HPSP_THREEFRY) {
+    threefry2x64<12>(ctr[0], ctr[1], key[0], key[1], out[0], out[1]);
+  } else {
+    uint64_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    const uint64_t k[4] = {key[0], key[1], key[2], key[3]};
+    threefry4x64<12>(c, k);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[i] = c[i];
+  }
+}
+
+template <int ENGINE>
+constexpr int engine_arity() {
+  return (ENGINE == HPSP_PHILOX || ENGINE == HPSP_THREEFRY) ? 2
+         : (ENGINE == HPSP_PHILOX_LONG || ENGINE == HPSP_THREEFRY_LONG) ? 4
+                                                                          : 1;
+}
+
+template <int ENGINE>
+constexpr bool is_squares() {
+  return ENGINE == HPSP_SQUARES3 || ENGINE == HPSP_SQUARES4;
+}
+
+template <int ENGINE>
+__device__ __forceinline__ uint32_t squares_word(uint64_t key, uint64_t ctr) {
+  if constexpr (ENGINE == HPSP_SQUARES3)
+    return squares3(key, ctr);
+  else
+    return squares4(key, ctr);
+}
+
+// Word j of the stream after GRND eng(seed); eng.discard(d):
+// squares: counter d + j; Random123: block counter d + 1 + j / arity
+// (pre-incremented), word j % arity.  Words of one event are produced
+// together so the block evaluations are shared.
+template <int ENGINE, int W>
+__device__ __forceinline__ void r123_words(const uint64_t* key, uint64_t d, uint64_t (&w)[W]) {
+  constexpr int A = engine_arity<ENGINE>();
+  constexpr int NB = (W + A - 1) / A;
+  uint64_t ctr[4] = {0, 0, 0, 0};
+  ctr_add<A>(ctr, d);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    ctr_add<A>(ctr, 1);
+    uint64_t out[4];
+    r123_block<ENGINE>(key, ctr, out);
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+      if (b * A + i < W) w[b * A + i] = out[i];
+  }
+}
+
+// word -> double in [0,1) (uniform_real_distribution.inl:62-80).
+template <int ENGINE>
+__device__ __forceinline__ double to_unit(uint64_t w) {
+#pragma clang fp contract(off)
+  if constexpr (is_squares<ENGINE>())
+    return (double)(uint32_t)w * 0x1p-32;
+  else
+    return (double)w * 0x1p-64;
+}
+
+// In-register ascending sort of a small fixed-size array (odd-even
+// transposition network; any correct sort yields the same values).
+template <typename T, int K>
+__device__ __forceinline__ void sort_net(T (&v)[K]) {
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+#pragma unroll
+    for (int i = (r & 1); i + 1 < K; i += 2) {
+      T a = v[i], b = v[i + 1];
+      v[i] = a < b ? a : b;
+      v[i + 1] = a < b ? b : a;
+    }
+  }
+}
+
+// Wave-level sharing of squares words.  Lane L's event e0+L needs the words
+// at counters c0 + L + k, k = 0..W-1 (W <= 64).  Each lane evaluates the two
+// words at c0 + L ("lo") and c0 + 64 + L ("hi").  For word k, lane L reads
+// source lane s = (L+k) mod 64; if L+k < 64 it needs lo[s] and then s >= k,
+// otherwise hi[s] and then s < k.  So each source lane offers
+// (s >= k ? lo : hi) and one ds_bpermute per word suffices.
+struct WaveWords {
+  uint32_t lo, hi;
+  int lane;
+  __device__ __forceinline__ uint32_t get(int k) const {
+    if (k == 0) return lo;
+    const uint32_t src = (lane >= k) ? lo : hi;
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(((lane + k) & (kWave - 1)) << 2, (int)src);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// sin and cos of 2*pi*u for u in [0, 1].  The quadrant is taken in u itself
+// (r = u - q/4 is exact because u and q/4 are multiples of one power of two
+// and within a factor two of each other), the remaining angle x = 2*pi*r with
+// |x| <= pi/4 is formed with a two-term 2*pi, and sin/cos of x come from
+// their Taylor series (terms to x^17 / x^16; the first omitted terms are
+// below 1e-19 on |x| <= pi/4), evaluated by Horner's rule with FMA.
+// The reference evaluates sin/cos of the rounded angle fl(2*pi*u); the two
+// agree to a few 1e-16 absolute.
+__device__ __forceinline__ void sincos_2pi(double u, double& s, double& c) {
+  const double qf = __builtin_rint(4.0 * u);
+  const double r = __builtin_fma(-0.25, qf, u);  // exact
+  const double x = __builtin_fma(r, 6.283185307179586, r * 2.4492935982947064e-16);
+  const double x2 = x * x;
+  double ps = 2.8114572543455206e-15;
+  ps = __builtin_fma(ps, x2, -7.647163731819816e-13);
+  ps = __builtin_fma(ps, x2, 1.6059043836821613e-10);
+  ps = __builtin_fma(ps, x2, -2.505210838544172e-08);
+  ps = __builtin_fma(ps, x2, 2.7557319223985893e-06);
+  ps = __builtin_fma(ps, x2, -0.0001984126984126984);
+  ps = __builtin_fma(ps, x2, 0.008333333333333333);
+  ps = __builtin_fma(ps, x2, -0.16666666666666666);
+  const double sx = __builtin_fma(x * x2, ps, x);
+  double pc = 4.779477332387385e-14;
+  pc = __builtin_fma(pc, x2, -1.1470745597729725e-11);
+  pc = __builtin_fma(pc, x2, 2.08767569878681e-09);
+  pc = __builtin_fma(pc, x2, -2.755731922398589e-07);
+  pc = __builtin_fma(pc, x2, 2.48015873015873e-05);
+  pc = __builtin_fma(pc, x2, -0.001388888888888889);
+  pc = __builtin_fma(pc, x2, 0.041666666666666664);
+  pc = __builtin_fma(pc, x2, -0.5);
+  const double cx = __builtin_fma(x2, pc, 1.0);
+  const int q = ((int)qf) & 3;
+  const double a = (q & 1) ? cx : sx;   // |sin| candidate
+  const double b = (q & 1) ? sx : cx;   // |cos| candidate
+  s = (q & 2) ? -a : a;
+  c = ((q + 1) & 2) ? -b : b;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 kinematics with the reference's operation order.  Each helper turns
+// contraction off itself, so the operation sequence holds in any translation
+// unit.
+
+__device__ __forceinline__ double pdk(double a, double b, double c) {
+#pragma clang fp contract(off)
+  return sqrt((a - b - c) * (a + b + c) * (a - b + c) * (a + b - c)) / (2 * a);
+}
+
+__device__ __forceinline__ double mass_of(double e, double x, double y, double z) {
+#pragma clang fp contract(off)
+  const double m2 = e * e - x * x - y * y - z * z;
+  return (m2 > 0.0) ? sqrt(m2) : 0.0;
+}
+
+// Vector4R::applyBoostTo(bx,by,bz) forward (Vector4R.inl:312-369) for a
+// per-event boost vector.
+__device__ __forceinline__ void boost_general(double (&v)[4], double bx, double by, double bz) {
+#pragma clang fp contract(off)
+  const double bxx = bx * bx;
+  const double byy = by * by;
+  const double bzz = bz * bz;
+  const double b2 = bxx + byy + bzz;
+  if (b2 > 0.0 && b2 < 1.0) {
+    const double gamma = 1.0 / sqrt(1.0 - b2);
+    const double gb2 = (gamma - 1.0) / b2;
+    const double gb2xy = gb2 * bx * by;
+    const double gb2xz = gb2 * bx * bz;
+    const double gb2yz = gb2 * by * bz;
+    const double gbx = gamma * bx;
+    const double gby = gamma * by;
+    const double gbz = gamma * bz;
+    const double e2 = v[0], px2 = v[1], py2 = v[2], pz2 = v[3];
+    v[0] = gamma * e2 + gbx * px2 + gby * py2 + gbz * pz2;
+    v[1] = gbx * e2 + gb2 * bxx * px2 + px2 + gb2xy * py2 + gb2xz * pz2;
+    v[2] = gby * e2 + gb2 * byy * py2 + py2 + gb2xy * px2 + gb2yz * pz2;
+    v[3] = gbz * e2 + gb2 * bzz * pz2 + pz2 + gb2yz * py2 + gb2xz * px2;
+  }
+}
+
+// The same boost with host-folded constants (single-mother final boost).
+__device__ __forceinline__ void boost_folded(double (&v)[4], const DecayK& d) {
+#pragma clang fp contract(off)
+  const double e2 = v[0], px2 = v[1], py2 = v[2], pz2 = v[3];
+  v[0] = d.gamma * e2 + d.gbx * px2 + d.gby * py2 + d.gbz * pz2;
+  v[1] = d.gbx * e2 + d.gb2 * d.bxx * px2 + px2 + d.gb2xy * py2 + d.gb2xz * pz2;
+  v[2] = d.gby * e2 + d.gb2 * d.byy * py2 + py2 + d.gb2xy * px2 + d.gb2yz * pz2;
+  v[3] = d.gbz * e2 + d.gb2 * d.bzz * pz2 + pz2 + d.gb2yz * py2 + d.gb2xz * px2;
+}
+
+// PhaseSpaceWeight::Evaluate on register momenta (Decays.inl:215-231).
+// R + p_i of one iteration and R += p_i round identically, so the mass of the
+// running sum is computed once and reused as R.mass() of the next iteration.
+template <int N>
+__device__ __forceinline__ double event_weight_f64(const double (&p)[N][4], const double* m,
+                                                   double maxw) {
+#pragma clang fp contract(off)
+  double R0 = p[0][0], R1 = p[0][1], R2 = p[0][2], R3 = p[0][3];
+  double w = maxw;
+  double mR = mass_of(R0, R1, R2, R3);
+#pragma unroll
+  for (int i = 1; i < N; ++i) {
+    R0 += p[i][0];
+    R1 += p[i][1];
+    R2 += p[i][2];
+    R3 += p[i][3];
+    const double mS = mass_of(R0, R1, R2, R3);
+    w *= pdk(mS, m[i], mR);
+    mR = mS;
+  }
+  return w;
+}
+
+// Functor evaluation (same arithmetic sequence as the oracle's).  P is the
+// number of four-vectors visible to the functor.
+template <typename Real, int P>
+__device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p)[P][4]) {
+  if (f.kind == HPSP_F_ONE) return 1.0;
+  if (f.kind == HPSP_F_MASS2_01) {
+    const Real s0 = p[0][0] + p[1][0], s1 = p[0][1] + p[1][1];
+    const Real s2 = p[0][2] + p[1][2], s3 = p[0][3] + p[1][3];
+    return (double)(s0 * s0 - s1 * s1 - s2 * s2 - s3 * s3);
+  }
+  Real Ar = 0, Ai = 0;
+  for (int k = 0; k < f.nterms; ++k) {
+    const int a = f.a[k], b = f.b[k];
+    Real v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      // runtime particle index: select through the unrolled register array
+      Real pa = p[0][c], pb = p[0][c];
+#pragma unroll
+      for (int i = 1; i < P; ++i) {
+        pa = (a == i) ? p[i][c] : pa;
+        pb = (b == i) ? p[i][c] : pb;
+      }
+      v[c] = pa + pb;
+    }
+    const Real s = v[0] * v[0] - v[1] * v[1] - v[2] * v[2] - v[3] * v[3];
+    const Real M = (Real)f.m[k], G = (Real)f.g[k];
+    const Real dr = M * M - s;
+    const Real di = -(M * G);
+    const Real den = dr * dr + di * di;
+    const Real br = dr / den;
+    const Real bi = -di / den;
+    const Real cr = (Real)f.cr[k], ci = (Real)f.ci[k];
+    const Real tr = cr * br - ci * bi;
+    const Real ti = cr * bi + ci * br;
+    Ar += tr;
+    Ai += ti;
+  }
+  return (double)(Ar * Ar + Ai * Ai);
+}
+
+// ---------------------------------------------------------------------------
+// StatsPHSP
+
+struct Stats {
+  double w, mean, m2;
+};
+
+// AddStatsPHSP (StatsPHSP.h:62-86); an empty side (w == 0) leaves the other.
+__device__ __forceinline__ Stats stats_merge(const Stats& x, const Stats& y) {
+#pragma clang fp contract(off)
+  if (x.w == 0.0) return y;
+  if (y.w == 0.0) return x;
+  Stats r;
+  const double w = x.w + y.w;
+  const double delta = y.mean - x.mean;
+  const double delta2 = delta * delta;
+  r.w = w;
+  r.mean = (x.mean * x.w + y.mean * y.w) / w;
+  r.m2 = x.m2 + y.m2;
+  r.m2 += delta2 * x.w * y.w / w;
+  return r;
+}
+
+// Fold one event {f, wt, 0} into a running accumulator (weighted West
+// update; equal to AddStatsPHSP with a single-point right side).
+__device__ __forceinline__ void stats_push(Stats& s, double f, double wt) {
+#pragma clang fp contract(off)
+  if (s.w == 0.0) {
+    s.w = wt;
+    s.mean = f;
+    s.m2 = 0.0;
+    return;
+  }
+  const double w = s.w + wt;
+  const double delta = f - s.mean;
+  const double r = wt / w;
+  const double mean = s.mean + delta * r;
+  s.m2 = s.m2 + wt * delta * (f - mean);
+  s.mean = mean;
+  s.w = w;
+}
+
+__device__ __forceinline__ Stats shfl_stats(const Stats& s, int src) {
+  Stats r;
+  r.w = __shfl(s.w, src, kWave);
+  r.mean = __shfl(s.mean, src, kWave);
+  r.m2 = __shfl(s.m2, src, kWave);
+  return r;
+}
+
+// Deterministic wave reduction: lane 0 ends with the merge tree
+// ((l0 l1)(l2 l3))... over the 64 lanes.
+__device__ __forceinline__ Stats wave_reduce_stats(Stats s) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    Stats o = shfl_stats(s, lane + off);
+    if ((lane & (2 * off - 1)) == 0) s = stats_merge(s, o);
+  }
+  return s;
+}
+
+}  // namespace hpsp

@@ -1,0 +1,84 @@
+// Host-side launch interface between the C ABI (capi.hip) and the kernel
+// translation units.  Internal to libhpsp.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hpsp_device.h"
+
+namespace hpsp {
+
+constexpr int kAvgChunk = 4096;  // events per AverageOn reduction chunk
+constexpr int kUwTile = 1024;    // events per Unweight tile (256 threads x 4)
+
+struct GenArgs {
+  DecayK d;
+  uint64_t first, n;
+  void* cols;
+  uint64_t ld;
+  int out_f32;
+  double* wt;
+  double* evw;
+  const double* mothers;  // FP64 mother columns (E,px,py,pz) or null
+  uint64_t mld;
+};
+
+struct AvgArgs {
+  DecayK d;
+  FunctorK f;
+  uint64_t first, n;
+  const double* mothers;
+  uint64_t mld;
+  Stats* partials;  // one per kAvgChunk events
+};
+
+struct WeightArgs {
+  DecayK d;
+  FunctorK f;
+  int use_f;
+  const double* cols;
+  uint64_t ld, n;
+  double* w_out;
+};
+
+struct UnweightArgs {
+  DecayK d;
+  FunctorK f;
+  int use_f;
+  double maxv;          // divisor of the acceptance test
+  const double* maxv_dev;  // if non-null, divisor read from device memory
+  uint64_t uw_key;      // splitmix64(seed) of the default engine
+  const double* cols_in;
+  uint64_t ld_in, n;
+  const double* w_in;
+  double* cols_out;
+  uint64_t ld_out;
+  uint64_t* idx_out;
+  double* w_out;
+  uint64_t* flags;      // one bit per event, 64 per word
+  uint32_t* tile_count; // accepted per tile
+  uint64_t* tile_off;   // exclusive prefix of tile_count
+  uint64_t* n_acc;      // total accepted (device)
+};
+
+// Kernel launchers; each returns a hipError_t value.
+int launch_generate_f64(int N, int engine, const GenArgs& a, hipStream_t s);
+int launch_generate_mixed(int N, int engine, const GenArgs& a, hipStream_t s);
+int launch_average_f64(int N, int engine, const AvgArgs& a, hipStream_t s);
+int launch_average_mixed(int N, int engine, const AvgArgs& a, hipStream_t s);
+int launch_stats_finalize(const Stats* partials, uint64_t m, Stats* out, hipStream_t s);
+int launch_event_weight(int N, const WeightArgs& a, hipStream_t s);
+int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
+                      double* out, hipStream_t s);
+int launch_unweight(int N, const UnweightArgs& a, hipStream_t s);
+int launch_rng_words(int engine, const uint64_t* key, uint64_t discard, uint64_t n,
+                     uint64_t* out, hipStream_t s);
+
+// Blocks for a grid-stride launch over `items` work items of kBlock each.
+inline unsigned grid_for(uint64_t items, unsigned cap = 65536u) {
+  const uint64_t b = (items + kBlock - 1) / kBlock;
+  return (unsigned)(b == 0 ? 1 : (b < cap ? b : cap));
+}
+
+}  // namespace hpsp

@@ -1,0 +1,237 @@
+// Accept-reject unweighting as a wave-ballot stream compaction (gfx950).
+//
+// Decays::Unweight (Decays.inl:415-455, 457-511) is a stable_partition of
+// (index, event) by FlagDaugthers (Decays.inl:75-89).  Here:
+//   1. k_uw_flags   -- one 1024-event tile per block: weight (read from a
+//                      stored column or recomputed from the momenta), the
+//                      default engine's word for the container index, the
+//                      test  w / max > u; a 64-bit __ballot per wave gives
+//                      the flag word, popcounts give the tile count.
+//   2. k_uw_scan    -- one block: exclusive prefix of the tile counts and
+//                      the accepted total.
+//   3. k_uw_scatter -- each event's destination from its tile offset and the
+//                      popcount of the lower lanes of its ballot word:
+//                      accepted -> off_acc, rejected -> n_acc + (k - off_acc);
+//                      all 4N columns (and optional index/weight) move once.
+// Event columns are read twice at most (flags need them only when no weight
+// column is supplied) and written once.
+#include <array>
+#include <utility>
+
+#include "hpsp_launch.h"
+
+namespace hpsp {
+
+constexpr int kItems = kUwTile / kBlock;  // 4 groups of 64 events per wave
+
+template <int N>
+__device__ __forceinline__ void load_event_uw(const double* cols, uint64_t ld, uint64_t k,
+                                              double (&p)[N][4]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) p[i][q] = cols[(uint64_t)(4 * i + q) * ld + k];
+}
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
+#pragma clang fp contract(off)
+  __shared__ uint32_t cnt[kBlock / kWave];
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  const uint64_t tile = blockIdx.x;
+  const double maxv = a.maxv_dev ? *a.maxv_dev : a.maxv;
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint64_t g = tile * (kUwTile / kWave) + (uint64_t)j * (kBlock / kWave) + wave;
+    const uint64_t k = g * kWave + (uint64_t)lane;
+    bool acc = false;
+    if (k < a.n) {
+      double w;
+      if (a.w_in && !a.use_f) {
+        w = a.w_in[k];
+      } else {
+        double p[N][4];
+        load_event_uw<N>(a.cols_in, a.ld_in, k, p);
+        w = a.w_in ? a.w_in[k] : event_weight_f64<N>(p, a.d.m, a.d.maxw);
+        if (a.use_f) w = w * functor_eval<double>(a.f, p);
+      }
+      const double u = (double)squares3(a.uw_key, k) * 0x1p-32;
+      acc = (w / maxv) > u;
+    }
+    const uint64_t ball = __ballot(acc);
+    if (lane == 0) a.flags[g] = ball;
+    c += (uint32_t)__popcll(ball);
+  }
+  if (lane == 0) cnt[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) a.tile_count[tile] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+}
+
+// Exclusive scan of the tile counts by one block of 256 threads: each thread
+// sums a contiguous slice, the 256 slice sums are scanned in LDS, then each
+// thread writes its slice's prefixes.
+__global__ __launch_bounds__(kBlock) void k_uw_scan(const uint32_t* tile_count, uint64_t ntiles,
+                                                    uint64_t* tile_off, uint64_t* n_acc) {
+  __shared__ uint64_t sh[kBlock];
+  const uint64_t per = (ntiles + kBlock - 1) / kBlock;
+  const uint64_t b = (uint64_t)threadIdx.x * per;
+  const uint64_t e = (b + per < ntiles) ? b + per : ntiles;
+  uint64_t s = 0;
+  for (uint64_t i = b; i < e; ++i) s += tile_count[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t run = 0;
+    for (int t = 0; t < kBlock; ++t) {
+      const uint64_t v = sh[t];
+      sh[t] = run;
+      run += v;
+    }
+    *n_acc = run;
+  }
+  __syncthreads();
+  uint64_t run = sh[threadIdx.x];
+  for (uint64_t i = b; i < e; ++i) {
+    tile_off[i] = run;
+    run += tile_count[i];
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_uw_scatter(const UnweightArgs a) {
+  __shared__ uint64_t goff[kUwTile / kWave];
+  const int lane = __lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t tile = blockIdx.x;
+  const uint64_t ngroups = kUwTile / kWave;
+  const uint64_t tile_base = a.tile_off[tile];
+  const uint64_t nacc = *a.n_acc;
+  if (threadIdx.x == 0) {
+    uint64_t run = tile_base;
+    for (uint64_t g = 0; g < ngroups; ++g) {
+      const uint64_t gi = tile * ngroups + g;
+      goff[g] = run;
+      if (gi * kWave < a.n) run += (uint64_t)__popcll(a.flags[gi]);
+    }
+  }
+  __syncthreads();
+  const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint64_t gl = (uint64_t)j * (kBlock / kWave) + wave;
+    const uint64_t g = tile * ngroups + gl;
+    const uint64_t k0 = g * kWave;
+    if (k0 >= a.n) continue;  // wave-uniform
+    const uint64_t k = k0 + (uint64_t)lane;
+    const bool live = k < a.n;
+    const uint64_t fw = a.flags[g];
+    // accepted lanes go to goff[gl] + (accepted lanes below),
+    // rejected lanes to nacc + (k0 - goff[gl]) + (rejected lanes below):
+    // two wave-uniform bases plus a lane offset below 64
+    const uint32_t acc_below = (uint32_t)__popcll(fw & below);
+    const bool acc = (fw >> lane) & 1ULL;
+    const uint64_t base_a = goff[gl];
+    const uint64_t base_r = nacc + k0 - goff[gl];
+    const uint32_t off = acc ? acc_below : (uint32_t)lane - acc_below;
+#pragma unroll
+    for (int c = 0; c < 4 * N; ++c) {
+      const double v = live ? __builtin_nontemporal_load(uniform_ptr(a.cols_in + (uint64_t)c * a.ld_in + k0) + lane) : 0.0;
+      double* col = a.cols_out + (uint64_t)c * a.ld_out;
+      if (live) {
+        if (acc)
+          uniform_ptr(col + base_a)[off] = v;
+        else
+          uniform_ptr(col + base_r)[off] = v;
+      }
+    }
+    if (live) {
+      const uint64_t dst = (acc ? base_a : base_r) + off;
+      if (a.idx_out) a.idx_out[dst] = k;
+      if (a.w_out) a.w_out[dst] = a.w_in[k];
+    }
+  }
+}
+
+__global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
+
+constexpr int kNumN = HPSP_MAX_N - 1;
+
+template <int N>
+static void uw_one(const UnweightArgs& a, hipStream_t s) {
+  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
+  hipLaunchKernelGGL(k_uw_flags<N>, dim3((unsigned)ntiles), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_uw_scan, dim3(1), dim3(kBlock), 0, s, a.tile_count, ntiles, a.tile_off,
+                     a.n_acc);
+  if (a.cols_out || a.idx_out)
+    hipLaunchKernelGGL(k_uw_scatter<N>, dim3((unsigned)ntiles), dim3(kBlock), 0, s, a);
+}
+
+template <int... I>
+static constexpr auto uw_table(std::integer_sequence<int, I...>) {
+  return std::array<void (*)(const UnweightArgs&, hipStream_t), sizeof...(I)>{&uw_one<2 + I>...};
+}
+static const auto kUw = uw_table(std::make_integer_sequence<int, kNumN>{});
+
+int launch_unweight(int N, const UnweightArgs& a, hipStream_t s) {
+  if (a.n == 0) {
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, a.n_acc, (uint64_t)0);
+    return (int)hipGetLastError();
+  }
+  kUw[N - 2](a, s);
+  return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Engine words (inspection / parity): thread t produces word t (squares) or
+// block t (Random123, A words) of the stream after discard(d).
+
+template <int ENGINE>
+__global__ __launch_bounds__(kBlock) void k_rng_words(const uint64_t* key_dev, uint64_t k0,
+                                                      uint64_t k1, uint64_t k2, uint64_t k3,
+                                                      uint64_t d, uint64_t n, uint64_t* out) {
+  (void)key_dev;
+  const uint64_t key[4] = {k0, k1, k2, k3};
+  constexpr int A = engine_arity<ENGINE>();
+  const uint64_t nitems = (n + A - 1) / A;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < nitems; t += stride) {
+    if constexpr (is_squares<ENGINE>()) {
+      out[t] = squares_word<ENGINE>(key[0], d + t);
+    } else {
+      uint64_t ctr[4] = {0, 0, 0, 0};
+      ctr_add<A>(ctr, d);
+      ctr_add<A>(ctr, t + 1);
+      uint64_t blk[4];
+      r123_block<ENGINE>(key, ctr, blk);
+#pragma unroll
+      for (int i = 0; i < A; ++i)
+        if (t * A + i < n) out[t * A + i] = blk[i];
+    }
+  }
+}
+
+int launch_rng_words(int engine, const uint64_t* key, uint64_t discard, uint64_t n,
+                     uint64_t* out, hipStream_t s) {
+  const unsigned g = grid_for(n);
+  switch (engine) {
+#define HPSP_RNG_CASE(E)                                                                   \
+  case E:                                                                                  \
+    hipLaunchKernelGGL(k_rng_words<E>, dim3(g), dim3(kBlock), 0, s, nullptr, key[0], key[1], \
+                       key[2], key[3], discard, n, out);                                   \
+    break;
+    HPSP_RNG_CASE(0)
+    HPSP_RNG_CASE(1)
+    HPSP_RNG_CASE(2)
+    HPSP_RNG_CASE(3)
+    HPSP_RNG_CASE(4)
+    HPSP_RNG_CASE(5)
+#undef HPSP_RNG_CASE
+    default:
+      return (int)hipErrorInvalidValue;
+  }
+  return (int)hipGetLastError();
+}
+
+}  // namespace hpsp

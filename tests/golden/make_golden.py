@@ -1,0 +1,134 @@
+"""Write the golden vectors of the phase-space path from the reference itself.
+
+Runs the reference harness (oracle/_ref/libhydra_ref.so, compiled from the
+reference headers by oracle/Makefile) and stores inputs and outputs as small
+.npz fixtures next to this script.  Only runs where the reference is mounted
+and the harness is built:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from oracle import ENGINES, Reference  # noqa: E402
+
+# Decays of the workloads (3-body B -> J/psi K pi; 10-body flat; near-threshold
+# 10-body; 2-body; 12-body with unequal masses).
+DECAYS = {
+    "c1_3body": (5.279, [3.0969, 0.493677, 0.13957]),
+    "c3_10body": (10.0, [0.13957] * 10),
+    "c3_10body_threshold": (10.0, [0.999] * 10),
+    "n2": (3.0, [1.0, 0.5]),
+    "n12": (20.0, [0.1 + 0.1 * i for i in range(12)]),
+}
+SEED = 159753
+UW_SEED = 0x180EC6D33CFD0ABA
+UWF_SEED = 0x39ABDC4529B1661C
+# Dalitz functor: three relativistic Breit-Wigners (M, Gamma, cr, ci, a, b)
+BW3 = [(0.892, 0.05, 0.4967141530, -0.1382643012, 1, 2),
+       (1.43, 0.11, 0.6476885381, 1.5230298564, 1, 2),
+       (0.775, 0.149, -0.2341533747, -0.2341368981, 0, 2)]
+
+
+def bw_par(terms):
+    flat = [float(len(terms))]
+    for t in terms:
+        flat.extend(float(x) for x in t)
+    return np.array(flat)
+
+
+def main():
+    R = Reference()
+    procs = os.cpu_count()
+    out = {}
+
+    # engine words: start of the stream and after a large discard
+    for name, e in ENGINES.items():
+        for disc in (0, 2**40 + 12345):
+            out[f"rng_{name}_d{disc}"] = R.rng_words(e, SEED, disc, 64)
+
+    # constructor products and the default seed
+    for key, (M, m) in DECAYS.items():
+        rc, ecm, mw, seed_full, seed_get = R.params(M, m)
+        out[f"params_{key}"] = np.array([ecm, mw])
+        out[f"seed_{key}"] = np.array([seed_full, np.uint64(seed_get & 0xFFFFFFFFFFFFFFFF)],
+                                      dtype=np.uint64)
+
+    # per-event GENBOD output: first events and sparse high indices, all engines
+    evts = np.concatenate([np.arange(256, dtype=np.uint64),
+                           np.array([2**20 + 7, 2**26 - 1, 2**27 - 1, 2**30 + 3, 2**40 + 5],
+                                    dtype=np.uint64)])
+    out["evts"] = evts
+    for key, (M, m) in DECAYS.items():
+        for name, e in ENGINES.items():
+            if e != 0 and key not in ("c1_3body", "c3_10body"):
+                continue
+            p, wt = R.decay_events(M, m, SEED, evts, engine=e)
+            out[f"decay_{key}_{name}_p"] = p
+            out[f"decay_{key}_{name}_wt"] = wt
+    # a moving mother (the single-mother boost as the reference builds it)
+    mom = [6.0, 0.3, -0.2, 0.5]
+    p, wt = R.decay_events(5.279, DECAYS["c1_3body"][1], SEED, evts[:64], mother4=mom)
+    out["decay_moving_p"] = p
+    out["decay_moving_wt"] = wt
+
+    # Generate into Decays + GetEventWeightFunctor, 4096 events
+    M, m = DECAYS["c1_3body"]
+    p, w, _ = R.generate(M, m, SEED, 4096)
+    out["gen_c1_p"] = p
+    out["gen_c1_w"] = w
+    M10, m10 = DECAYS["c3_10body_threshold"]
+    p, w, _ = R.generate(M10, m10, SEED, 1024)
+    out["gen_thr_p"] = p
+    out["gen_thr_w"] = w
+    # default seed (no SetSeed)
+    p, w, _ = R.generate(M, m, None, 64)
+    out["gen_c1_defaultseed_p"] = p
+
+    # Generate(mothers, daughters)
+    rng = np.random.default_rng(7)
+    mothers = np.zeros((4, 256))
+    pm = rng.normal(0.0, 2.0, size=(3, 256))
+    mothers[1:] = pm
+    mothers[0] = np.sqrt(M * M + (pm ** 2).sum(axis=0))
+    p, w = R.generate_mothers(M, m, SEED, mothers)
+    out["genm_mothers"] = mothers
+    out["genm_p"] = p
+    out["genm_w"] = w
+
+    # Unweight(): plain and with a functor (f = (p0+p1)^2), 4096 events
+    nacc, p, idx, _ = R.unweight(M, m, SEED, 4096, UW_SEED)
+    out["uw_c1_nacc"] = np.array([nacc], dtype=np.uint64)
+    out["uw_c1_idx"] = idx
+    out["uw_c1_p"] = p
+    nacc, p, idx, _ = R.unweight(M, m, SEED, 4096, UWF_SEED, fid=2, fpar=np.zeros(1),
+                                 max_weight=-1.0)
+    out["uwf_c1_nacc"] = np.array([nacc], dtype=np.uint64)
+    out["uwf_c1_idx"] = idx
+    nacc, p, idx, _ = R.unweight(M, m, SEED, 4096, UWF_SEED, fid=1, fpar=bw_par(BW3),
+                                 max_weight=40.0)
+    out["uwbw_c1_nacc"] = np.array([nacc], dtype=np.uint64)
+    out["uwbw_c1_idx"] = idx
+
+    # AverageOn (OMP fold over omp_get_num_procs() intervals)
+    for fid, fp, tag in ((0, np.zeros(1), "one"), (2, np.zeros(1), "m2"),
+                         (1, bw_par(BW3), "bw3")):
+        mean, sig, _ = R.average_on(M, m, SEED, 100000, fid=fid, fpar=fp)
+        out[f"avg_c1_{tag}"] = np.array([mean, sig])
+    mean, sig, _ = R.average_on(M10, m10, SEED, 20000, fid=2, fpar=np.zeros(1))
+    out["avg_thr_m2"] = np.array([mean, sig])
+    out["avg_procs"] = np.array([procs])
+    out["bw3_par"] = bw_par(BW3)
+
+    np.savez_compressed(os.path.join(HERE, "phsp_golden.npz"), **out)
+    print("wrote", os.path.join(HERE, "phsp_golden.npz"), len(out), "arrays")
+
+
+if __name__ == "__main__":
+    main()

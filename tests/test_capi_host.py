@@ -1,0 +1,90 @@
+"""Host-side checks of libhpsp.so that need no GPU: the library loads, exports
+every entry point declared in include/hpsp.h, and its host-only set-up
+functions reproduce the reference's constructor arithmetic and errors."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from hydra_amd import capi
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def test_library_exports_declared_symbols():
+    L = capi.lib()
+    names = capi.exported_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+
+
+@pytest.mark.parametrize("M,m", [(5.279, [3.0969, 0.493677, 0.13957]), (10.0, [0.13957] * 10),
+                                 (10.0, [0.999] * 10), (3.0, [1.0, 0.5]),
+                                 (20.0, [0.1 + 0.1 * i for i in range(12)])])
+def test_decay_init_matches_oracle(oracle, M, m):
+    d = capi.make_decay(M, m)
+    rc, ecm, mw = oracle.params(M, m)
+    assert bits(np.array([d.ecm, d.max_weight])).tolist() == bits(np.array([ecm, mw])).tolist()
+    assert d.n == len(m) and list(d.beta) == [0.0, 0.0, 0.0]
+
+
+def test_decay_init_errors():
+    L = capi.lib()
+    d = capi.Decay()
+    m = (ctypes.c_double * 3)(1.0, 0.5, 0.2)
+    assert L.hpsp_decay_init(ctypes.byref(d), 3, 0.0, m, 0) == capi.ERR_MASS
+    assert L.hpsp_decay_init(ctypes.byref(d), 3, -1.0, m, 0) == capi.ERR_MASS
+    bad = (ctypes.c_double * 3)(1.0, 0.0, 0.2)
+    assert L.hpsp_decay_init(ctypes.byref(d), 3, 5.0, bad, 0) == capi.ERR_MASS
+    assert L.hpsp_decay_init(ctypes.byref(d), 3, 1.7, m, 0) == capi.ERR_ENERGY
+    assert L.hpsp_decay_init(ctypes.byref(d), 1, 5.0, m, 0) == capi.ERR_INVALID_ARGUMENT
+    assert L.hpsp_decay_init(ctypes.byref(d), 13, 5.0, m, 0) == capi.ERR_INVALID_ARGUMENT
+    assert L.hpsp_decay_init(ctypes.byref(d), 3, 5.0, m, 6) == capi.ERR_INVALID_ARGUMENT
+    assert L.hpsp_decay_init(None, 3, 5.0, m, 0) == capi.ERR_INVALID_ARGUMENT
+    for code in range(7):
+        assert L.hpsp_error_string(code)
+
+
+@pytest.mark.parametrize("mom", [[5.279, 0, 0, 0], [6.0, 0.3, -0.2, 0.5], [7.0, 0.0, 0.0, 3.0]])
+def test_set_mother_matches_oracle(oracle, mom):
+    d = capi.make_decay(5.279, [3.0969, 0.493677, 0.13957], mother4=mom)
+    assert bits(np.array(list(d.beta))).tolist() == bits(oracle.mother_beta(mom)).tolist()
+
+
+def test_stats_merge_matches_oracle(oracle):
+    x = capi.Stats(2.5, 0.7, 0.1)
+    y = capi.Stats(1.25, -0.3, 0.05)
+    r = capi.Stats()
+    capi.lib().hpsp_stats_merge(x, y, r)
+    o = oracle.stats_merge([2.5, 0.7, 0.1], [1.25, -0.3, 0.05])
+    assert bits(np.array([r.w, r.mean, r.m2])).tolist() == bits(o).tolist()
+
+
+def test_workspace_sizes_grow_with_n():
+    L = capi.lib()
+    assert L.hpsp_unweight_workspace_size(0) > 0
+    assert L.hpsp_unweight_workspace_size(1 << 27) >= (1 << 27) // 8
+    assert L.hpsp_average_on_workspace_size(1 << 30) >= 24 * ((1 << 30) // 4096)
+
+
+def test_calls_validate_before_touching_the_device():
+    L = capi.lib()
+    d = capi.make_decay(5.279, [3.0969, 0.493677, 0.13957])
+    # n > 0 with null columns, ld < n, bad precision, bad engine: rejected on the host
+    assert L.hpsp_generate(ctypes.byref(d), 1, 0, 10, 0, None, 10, 0, None, None, None) == 1
+    assert L.hpsp_generate(ctypes.byref(d), 1, 0, 10, 0, ctypes.c_void_p(64), 5, 0, None, None,
+                           None) == 1
+    assert L.hpsp_generate(ctypes.byref(d), 1, 0, 10, 7, ctypes.c_void_p(64), 10, 0, None, None,
+                           None) == 1
+    assert L.hpsp_rng_words(9, 1, 0, 10, ctypes.c_void_p(64), None) == 1
+    assert L.hpsp_unweight(ctypes.byref(d), None, -1.0, 1, ctypes.c_void_p(64), 10, 10, None,
+                           None, 0, None, None, None, 0, ctypes.c_void_p(64), None) == \
+        capi.ERR_WORKSPACE
+    assert L.hpsp_average_on(ctypes.byref(d), None, 1, 0, 10, 0, None, 0, ctypes.c_void_p(64),
+                             None) == capi.ERR_WORKSPACE
+    # n = 0 is a no-op
+    assert L.hpsp_generate(ctypes.byref(d), 1, 0, 0, 0, None, 0, 0, None, None, None) == 0

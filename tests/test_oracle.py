@@ -1,0 +1,280 @@
+"""The oracle (oracle/phsp_oracle.c) pinned before it is trusted.
+
+* bit-for-bit against the golden vectors written by the reference itself
+  (tests/golden/make_golden.py);
+* bit-for-bit against the live reference harness where it is built;
+* against published known-answer vectors (SplitMix64, Random123) and
+  closed forms of the phase-space algebra that a dropped term, a wrong sign
+  or a swapped index would break.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ENGINES, cols_to_rows, rows_to_cols
+
+SEED = 159753
+UW_SEED = 0x180EC6D33CFD0ABA
+UWF_SEED = 0x39ABDC4529B1661C
+DECAYS = {
+    "c1_3body": (5.279, [3.0969, 0.493677, 0.13957]),
+    "c3_10body": (10.0, [0.13957] * 10),
+    "c3_10body_threshold": (10.0, [0.999] * 10),
+    "n2": (3.0, [1.0, 0.5]),
+    "n12": (20.0, [0.1 + 0.1 * i for i in range(12)]),
+}
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+# ---------------------------------------------------------------- engines
+def test_splitmix64_published_sequence(oracle):
+    # SplitMix64 (Steele, Lea, Flood 2014) started from state 0.
+    x = 0
+    got = []
+    for _ in range(3):
+        r, x = oracle.splitmix64(x)
+        got.append(r)
+    assert got == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+@pytest.mark.parametrize("name,rounds,ctr,key,expect", [
+    ("philox2x64", 10, [0, 0], [0], [0xCA00A0459843D731, 0x66C24222C9A845B5]),
+    ("philox4x64", 10, [0, 0, 0, 0], [0, 0],
+     [0x16554D9ECA36314C, 0xDB20FE9D672D0FDC, 0xD7E772CEE186176B, 0x7E68B68AEC7BA23B]),
+    ("threefry2x64", 20, [0, 0], [0, 0], [0xC2B6E3A8C2C69865, 0x6F81ED42F350084D]),
+    ("threefry4x64", 20, [0, 0, 0, 0], [0, 0, 0, 0],
+     [0x09218EBDE6C85537, 0x55941F5266D86105, 0x4BD25E16282434DC, 0xEE29EC846BD2E40B]),
+])
+def test_random123_known_answers(oracle, name, rounds, ctr, key, expect):
+    # Random123 known-answer vectors at the library's default round counts;
+    # the reference runs the same bijections at 7 (Philox) / 12 (Threefry)
+    # rounds, pinned by the golden words below.
+    out = oracle.bijection(name, rounds, ctr, key)
+    assert [int(v) for v in out] == expect
+
+
+@pytest.mark.parametrize("engine", list(ENGINES))
+@pytest.mark.parametrize("disc", [0, 2**40 + 12345])
+def test_engine_words_golden(oracle, golden, engine, disc):
+    got = oracle.rng_words(engine, SEED, disc, 64)
+    assert np.array_equal(got, golden[f"rng_{engine}_d{disc}"])
+
+
+def test_discard_equals_draws(oracle):
+    # discard(n) is n draws for squares; n blocks for Random123.
+    for e, arity in (("squares3", 1), ("squares4", 1), ("philox", 2), ("philox_long", 4),
+                     ("threefry", 2), ("threefry_long", 4)):
+        full = oracle.rng_words(e, 42, 0, 64 * arity)
+        part = oracle.rng_words(e, 42, 8, 16)
+        assert np.array_equal(part, full[8 * arity: 8 * arity + 16])
+
+
+def test_uniform_map(oracle):
+    assert oracle.uniform(0, 0) == 0.0
+    assert oracle.uniform(0, 2**32 - 1) == (2**32 - 1) / 2**32
+    # 64-bit words round in the conversion and can reach 1.0
+    assert oracle.uniform(2, 2**64 - 1) == 1.0
+    assert oracle.uniform(2, 2**63) == 0.5
+
+
+# ---------------------------------------------------------------- set-up
+@pytest.mark.parametrize("key", list(DECAYS))
+def test_params_golden(oracle, golden, key):
+    M, m = DECAYS[key]
+    rc, ecm, mw = oracle.params(M, m)
+    assert rc == 0
+    assert np.array_equal(bits(np.array([ecm, mw])), bits(golden[f"params_{key}"]))
+
+
+def test_params_closed_form(oracle):
+    # N = 2: wtmax = PDK(M, m1, m2) = |p*| of the two-body decay.
+    M, m1, m2 = 3.0, 1.0, 0.5
+    rc, ecm, mw = oracle.params(M, [m1, m2])
+    pstar = math.sqrt((M * M - (m1 + m2) ** 2) * (M * M - (m1 - m2) ** 2)) / (2 * M)
+    assert abs(1.0 / mw - pstar) <= 4e-16 * pstar
+    assert ecm == M - m1 - m2
+    assert oracle.params(1.0, [0.6, 0.5])[0] == -2  # not enough energy
+    assert oracle.params(0.0, [0.1, 0.1])[0] == -1
+    assert oracle.params(1.0, [0.1, -0.1])[0] == -1
+
+
+# ---------------------------------------------------------------- GENBOD
+@pytest.mark.parametrize("key", list(DECAYS))
+def test_decay_events_golden_default_engine(oracle, golden, key):
+    M, m = DECAYS[key]
+    p, wt = oracle.decay_events(M, m, SEED, golden["evts"])
+    assert np.array_equal(bits(p), bits(golden[f"decay_{key}_squares3_p"]))
+    assert np.array_equal(bits(wt), bits(golden[f"decay_{key}_squares3_wt"]))
+
+
+@pytest.mark.parametrize("key", ["c1_3body", "c3_10body"])
+@pytest.mark.parametrize("engine", list(ENGINES))
+def test_decay_events_golden_engines(oracle, golden, key, engine):
+    M, m = DECAYS[key]
+    p, wt = oracle.decay_events(M, m, SEED, golden["evts"], engine=engine)
+    assert np.array_equal(bits(p), bits(golden[f"decay_{key}_{engine}_p"]))
+    assert np.array_equal(bits(wt), bits(golden[f"decay_{key}_{engine}_wt"]))
+
+
+def test_decay_moving_mother_golden(oracle, golden):
+    M, m = DECAYS["c1_3body"]
+    p, wt = oracle.decay_events(M, m, SEED, golden["evts"][:64], mother4=[6.0, 0.3, -0.2, 0.5])
+    assert np.array_equal(bits(p), bits(golden["decay_moving_p"]))
+    assert np.array_equal(bits(wt), bits(golden["decay_moving_wt"]))
+
+
+def test_generate_and_event_weight_golden(oracle, golden):
+    M, m = DECAYS["c1_3body"]
+    cols, wt = oracle.generate(M, m, SEED, 4096)
+    assert np.array_equal(bits(cols), bits(rows_to_cols(golden["gen_c1_p"])))
+    w = oracle.event_weight(M, m, cols)
+    assert np.array_equal(bits(w), bits(golden["gen_c1_w"]))
+    # the weight recomputed from stored momenta (masses of running sums)
+    # equals the generation weight only up to cancellation in mass()
+    assert np.max(np.abs(w - wt) / wt) < 1e-10
+    M, m = DECAYS["c3_10body_threshold"]
+    cols, wt = oracle.generate(M, m, SEED, 1024)
+    assert np.array_equal(bits(cols), bits(rows_to_cols(golden["gen_thr_p"])))
+    assert np.array_equal(bits(oracle.event_weight(M, m, cols)), bits(golden["gen_thr_w"]))
+
+
+def test_generate_mothers_golden(oracle, golden):
+    M, m = DECAYS["c1_3body"]
+    cols, wt = oracle.generate_mothers(M, m, SEED, golden["genm_mothers"])
+    assert np.array_equal(bits(cols), bits(rows_to_cols(golden["genm_p"])))
+    assert np.array_equal(bits(oracle.event_weight(M, m, cols)), bits(golden["genm_w"]))
+
+
+def test_chunked_generation_is_one_call(oracle):
+    # event k depends only on (seed, k): chunks concatenate to one run
+    M, m = DECAYS["c1_3body"]
+    a, _ = oracle.generate(M, m, SEED, 1000)
+    b1, _ = oracle.generate(M, m, SEED, 300, first=0)
+    b2, _ = oracle.generate(M, m, SEED, 700, first=300)
+    assert np.array_equal(a, np.concatenate([b1, b2], axis=1))
+
+
+@pytest.mark.parametrize("key", list(DECAYS))
+def test_kinematic_invariants(oracle, key):
+    M, m = DECAYS[key]
+    N = len(m)
+    cols, wt = oracle.generate(M, m, 99, 2000)
+    p = cols_to_rows(cols, N)
+    tot = p.sum(axis=1)
+    assert np.max(np.abs(tot[:, 0] - M)) < 1e-12 * M
+    assert np.max(np.abs(tot[:, 1:])) < 1e-12 * M
+    m2 = p[..., 0] ** 2 - (p[..., 1:] ** 2).sum(axis=-1)
+    E2 = p[..., 0] ** 2
+    assert np.max(np.abs(m2 - np.array(m) ** 2) / E2) < 1e-12
+    assert np.all(wt > 0) and np.all(wt <= 1.0 + 1e-12)
+
+
+def test_two_body_closed_form(oracle):
+    M, m1, m2 = 3.0, 1.0, 0.5
+    cols, wt = oracle.generate(M, [m1, m2], 5, 500)
+    p = cols_to_rows(cols, 2)
+    pstar = math.sqrt((M * M - (m1 + m2) ** 2) * (M * M - (m1 - m2) ** 2)) / (2 * M)
+    pm = np.sqrt((p[:, 0, 1:] ** 2).sum(axis=-1))
+    assert np.max(np.abs(pm - pstar)) < 1e-14 * M
+    assert np.max(np.abs(wt - 1.0)) < 1e-14
+    assert np.max(np.abs(p[:, 0, 1:] + p[:, 1, 1:])) < 1e-14 * M
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 5])
+def test_massless_phase_space_volume(oracle, N):
+    # E[prod pd] over GENBOD's sorted uniforms equals the massless N-body
+    # phase-space integral in GENBOD's normalisation: M^(N-1) / (2^(2N-3) (N-1)!).
+    M = 1.0
+    m = [1e-9] * N
+    st = oracle.average_on(M, m, 1234, 400000, fid=0, intervals=8)
+    rc, ecm, mw = oracle.params(M, m)
+    mean_prod = st[0] / 400000 / mw
+    expect = M ** (N - 1) / (2 ** (2 * N - 3) * math.factorial(N - 1))
+    sd = 3.0 / math.sqrt(400000)  # relative statistical scale (weights are O(mean))
+    assert abs(mean_prod / expect - 1.0) < 5 * sd
+
+
+def test_isotropy(oracle):
+    M, m = DECAYS["c1_3body"]
+    cols, wt = oracle.generate(M, m, 77, 200000)
+    p = cols_to_rows(cols, 3)
+    for i in range(3):
+        c = p[:, i, 3] / np.sqrt((p[:, i, 1:] ** 2).sum(axis=-1))
+        assert abs(c.mean()) < 5 / math.sqrt(3 * len(c))
+
+
+# ---------------------------------------------------------------- weights / unweighting
+def test_unweight_golden(oracle, golden):
+    M, m = DECAYS["c1_3body"]
+    cols, _ = oracle.generate(M, m, SEED, 4096)
+    nacc, out, idx, flags = oracle.unweight(M, m, cols, UW_SEED)
+    assert nacc == int(golden["uw_c1_nacc"][0])
+    assert np.array_equal(idx, golden["uw_c1_idx"])
+    assert np.array_equal(bits(out), bits(rows_to_cols(golden["uw_c1_p"])))
+    nacc, out, idx, flags = oracle.unweight(M, m, cols, UWF_SEED, fid=2, max_value=-1.0)
+    assert nacc == int(golden["uwf_c1_nacc"][0])
+    assert np.array_equal(idx, golden["uwf_c1_idx"])
+    nacc, out, idx, flags = oracle.unweight(M, m, cols, UWF_SEED, fid=1,
+                                            fpar=golden["bw3_par"], max_value=40.0)
+    assert nacc == int(golden["uwbw_c1_nacc"][0])
+    assert np.array_equal(idx, golden["uwbw_c1_idx"])
+
+
+def test_unweight_is_stable_partition(oracle):
+    M, m = DECAYS["c1_3body"]
+    cols, _ = oracle.generate(M, m, 3, 3000)
+    nacc, out, idx, flags = oracle.unweight(M, m, cols, 11)
+    acc = np.nonzero(flags)[0]
+    rej = np.nonzero(flags == 0)[0]
+    assert np.array_equal(idx, np.concatenate([acc, rej]))
+    assert np.array_equal(out, cols[:, idx.astype(np.int64)])
+
+
+# ---------------------------------------------------------------- AverageOn
+@pytest.mark.parametrize("tag,fid", [("one", 0), ("m2", 2), ("bw3", 1)])
+def test_average_on_golden(oracle, golden, tag, fid):
+    M, m = DECAYS["c1_3body"]
+    P = int(golden["avg_procs"][0])
+    st = oracle.average_on(M, m, SEED, 100000, fid=fid, fpar=golden["bw3_par"], intervals=P)
+    mean, sig = st[1], math.sqrt(st[2]) / st[0]
+    assert np.array_equal(bits(np.array([mean, sig])), bits(golden[f"avg_c1_{tag}"]))
+
+
+def test_average_on_constant_functor(oracle):
+    M, m = DECAYS["c3_10body"]
+    st = oracle.average_on(M, m, 5, 5000, fid=0, intervals=3)
+    assert st[1] == 1.0 and st[2] == 0.0
+
+
+def test_stats_merge_closed_form(oracle):
+    # two points {f=1,w=1} and {f=3,w=3}: W=4, mean=(1+9)/4, M2 = sum w (f-mean)^2
+    r = oracle.stats_merge([1.0, 1.0, 0.0], [3.0, 3.0, 0.0])
+    mean = 10.0 / 4.0
+    assert r[0] == 4.0 and r[1] == mean
+    assert abs(r[2] - (1 * (1 - mean) ** 2 + 3 * (3 - mean) ** 2)) < 1e-15
+
+
+# ---------------------------------------------------------------- live reference
+@pytest.mark.parametrize("N", range(2, 13))
+def test_oracle_equals_reference_random_decays(oracle, reference, N):
+    rng = np.random.default_rng(N)
+    m = list(rng.uniform(0.1, 1.0, N))
+    M = sum(m) * rng.uniform(1.05, 3.0)
+    seed = int(rng.integers(0, 2**63))
+    evts = np.array(list(range(40)) + [2**33 + 1, 2**50], dtype=np.uint64)
+    for e in range(6):
+        p1, w1 = oracle.decay_events(M, m, seed, evts, engine=e)
+        p2, w2 = reference.decay_events(M, m, seed, evts, engine=e)
+        assert np.array_equal(bits(p1), bits(p2))
+        assert np.array_equal(bits(w1), bits(w2))
+    # Unweight over a generated container
+    pr, wr, _ = reference.generate(M, m, 31, 700)
+    cols = rows_to_cols(pr)
+    assert np.array_equal(bits(oracle.event_weight(M, m, cols)), bits(wr))
+    nacc, pu, idx, _ = reference.unweight(M, m, 31, 700, seed)
+    n2, out, idx2, _ = oracle.unweight(M, m, cols, seed)
+    assert nacc == n2 and np.array_equal(idx, idx2)
