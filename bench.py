@@ -59,7 +59,9 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 27, help="events per step (headline)")
     ap.add_argument("--extras", type=int, default=1, help="also time the other workloads")
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--only", default="", help="run one workload: c2, c3, c4 (profiling)")
+    ap.add_argument("--only", default="",
+                    help="run one workload: c2, c3, c4, or copy (a plain 1 GiB device copy "
+                         "that calibrates the profiler's byte counters)")
     return ap.parse_args()
 
 
@@ -279,6 +281,19 @@ def main():
     if args.only == "c4":
         extras["c4"] = run_c4(torch, capi)
         print(json.dumps(extras), flush=True)
+        return
+    if args.only == "copy":
+        # plain device copy of a known size: calibrates the HBM byte counters
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        t = Timer(torch)
+        for _ in range(3):
+            t.start()
+            y.copy_(x)
+            t.stop()
+        print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8,
+                          "copy_ms": t.avg_ms,
+                          "copy_GBs": 2 * x.numel() * 8 / (t.avg_ms * 1e6)}), flush=True)
         return
 
     c2 = run_c2(torch, capi, args.n, args.steps, args.warmup)

@@ -92,6 +92,8 @@ _SIGS = {
     "hpsp_memcpy_to_host": (_I, [_P, _P, _SZ]),
     "hpsp_memcpy_device": (_I, [_P, _P, _SZ, _P]),
     "hpsp_stream_synchronize": (_I, [_P]),
+    "hpsp_memcpy2d_to_host": (_I, [_P, _SZ, _P, _SZ, _SZ, _SZ]),
+    "hpsp_memcpy2d_to_device": (_I, [_P, _SZ, _P, _SZ, _SZ, _SZ]),
 }
 
 

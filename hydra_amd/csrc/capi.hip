@@ -414,4 +414,16 @@ int hpsp_stream_synchronize(void* stream) {
   return hip_status((int)hipStreamSynchronize(as_stream(stream)));
 }
 
+int hpsp_memcpy2d_to_host(void* dst, size_t dpitch, const void* src, size_t spitch,
+                          size_t width, size_t height) {
+  return hip_status(
+      (int)hipMemcpy2D(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToHost));
+}
+
+int hpsp_memcpy2d_to_device(void* dst, size_t dpitch, const void* src, size_t spitch,
+                            size_t width, size_t height) {
+  return hip_status(
+      (int)hipMemcpy2D(dst, dpitch, src, spitch, width, height, hipMemcpyHostToDevice));
+}
+
 }  // extern "C"

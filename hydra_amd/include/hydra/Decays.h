@@ -1,0 +1,371 @@
+// hydra::Decays -- the N-particle event container of the phase-space path
+// (reference: hydra/Decays.h:67-651, hydra/detail/Decays.inl:114-511).
+//
+// Device storage is structure-of-arrays by component: 4N FP64 columns of
+// capacity() events each (column 4*i + c holds component c of particle i),
+// so every kernel access is coalesced.  Rows are assembled into tuples of
+// Vector4R on access.  The container also keeps the PhaseSpaceWeight of each
+// event when it is known exactly (written by PhaseSpace::Generate, permuted by
+// Unweight), so Unweight need not re-read the momenta to flag events.
+//
+// Storage is allocated without zero-filling (the reference value-initialises,
+// Decays.h:92; the difference is not observable after Generate).
+#pragma once
+
+#include <array>
+#include <cstring>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include <hydra/Function.h>
+#include <hydra/PhaseSpaceFunctors.h>
+#include <hydra/Range.h>
+#include <hydra/Tuple.h>
+#include <hydra/Types.h>
+#include <hydra/Vector4R.h>
+#include <hydra/detail/Runtime.h>
+#include <hydra/device/System.h>
+
+#include "hpsp.h"
+
+namespace hydra {
+
+namespace placeholders {
+template <unsigned I>
+struct placeholder {};
+constexpr placeholder<0> _0{};
+constexpr placeholder<1> _1{};
+constexpr placeholder<2> _2{};
+constexpr placeholder<3> _3{};
+constexpr placeholder<4> _4{};
+constexpr placeholder<5> _5{};
+constexpr placeholder<6> _6{};
+constexpr placeholder<7> _7{};
+constexpr placeholder<8> _8{};
+constexpr placeholder<9> _9{};
+constexpr placeholder<10> _10{};
+constexpr placeholder<11> _11{};
+}  // namespace placeholders
+
+namespace detail {
+
+// PhaseSpace<N> constructor products (PhaseSpace.inl:67-89), also computed
+// by Decays and PhaseSpaceWeight (Decays.h:100-118, Decays.inl:139-157).
+template <size_t N>
+inline hpsp_decay make_decay(double mother_mass, const std::array<double, N>& masses,
+                             const char* where, int engine = HPSP_SQUARES3) {
+  hpsp_decay d;
+  hpsp_check(hpsp_decay_init(&d, (int)N, mother_mass, masses.data(), engine), where);
+  return d;
+}
+
+}  // namespace detail
+
+// Weights of a container's events, computed on the device
+// (events | events.GetEventWeightFunctor(), Range2.inl:213-238).
+class EventWeights {
+ public:
+  explicit EventWeights(size_t n) : w_(n) {}
+  size_t size() const { return w_.size(); }
+  double operator[](size_t i) const {
+    double v = 0;
+    detail::hpsp_check(hpsp_memcpy_to_host(&v, w_.data() + i, sizeof(double)), "EventWeights");
+    return v;
+  }
+  std::vector<double> to_host() const {
+    std::vector<double> h(w_.size());
+    if (!h.empty())
+      detail::hpsp_check(hpsp_memcpy_to_host(h.data(), w_.data(), h.size() * sizeof(double)),
+                         "EventWeights");
+    return h;
+  }
+  double* data() { return w_.data(); }
+  const double* data() const { return w_.data(); }
+
+ private:
+  detail::device_buffer<double> w_;
+};
+
+// PhaseSpaceWeight<P...> (Decays.inl:114-254): w = fMaxWeight * prod pdk over
+// masses of running sums, evaluated on the device.
+template <typename... Particles>
+class PhaseSpaceWeight {
+ public:
+  static constexpr size_t N = sizeof...(Particles);
+  PhaseSpaceWeight(double motherMass, std::array<double, N> const& masses)
+      : d_(detail::make_decay<N>(motherMass, masses, "PhaseSpaceWeight")) {}
+  double GetMaxWeight() const { return d_.max_weight; }
+  const double* GetMasses() const { return d_.masses; }
+  const hpsp_decay& descriptor() const { return d_; }
+
+ private:
+  hpsp_decay d_;
+};
+
+// PhaseSpaceReweight<F, P...> (Decays.inl:258-411): PhaseSpaceWeight * f.
+template <typename Functor, typename... Particles>
+class PhaseSpaceReweight {
+ public:
+  static constexpr size_t N = sizeof...(Particles);
+  PhaseSpaceReweight(Functor const& f, double motherMass, std::array<double, N> const& masses)
+      : d_(detail::make_decay<N>(motherMass, masses, "PhaseSpaceReweight")),
+        f_(detail::to_hpsp(f)) {}
+  double GetMaxWeight() const { return d_.max_weight; }
+  const hpsp_decay& descriptor() const { return d_; }
+  const hpsp_functor& functor() const { return f_; }
+
+ private:
+  hpsp_decay d_;
+  hpsp_functor f_;
+};
+
+template <typename Tuple, typename Backend>
+class Decays;
+
+template <typename... Particles>
+class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
+ public:
+  static constexpr size_t N = sizeof...(Particles);
+  enum { nparticles = N };
+  typedef hydra::tuple<Particles...> value_type;
+  typedef hydra::tuple<Particles...> tuple_type;
+  typedef hydra::device::sys_t system_type;
+
+  // Random-access iterator over rows; dereferencing copies one row to the host.
+  class iterator {
+   public:
+    typedef std::ptrdiff_t difference_type;
+    iterator() = default;
+    iterator(const Decays* c, size_t i) : c_(c), i_(i) {}
+    value_type operator*() const { return (*c_)[i_]; }
+    iterator& operator++() { ++i_; return *this; }
+    iterator operator+(difference_type d) const { return iterator(c_, i_ + d); }
+    iterator operator-(difference_type d) const { return iterator(c_, i_ - d); }
+    difference_type operator-(const iterator& o) const { return (difference_type)i_ - (difference_type)o.i_; }
+    bool operator==(const iterator& o) const { return i_ == o.i_ && c_ == o.c_; }
+    bool operator!=(const iterator& o) const { return !(*this == o); }
+    size_t index() const { return i_; }
+
+   private:
+    const Decays* c_ = nullptr;
+    size_t i_ = 0;
+  };
+  typedef iterator const_iterator;
+
+  // One particle's column view (GetDaugtherRange, Decays.h:279-284).
+  class ParticleRange {
+   public:
+    ParticleRange(const Decays* c, unsigned i) : c_(c), i_(i) {}
+    size_t size() const { return c_->size(); }
+    Vector4R operator[](size_t k) const {
+      double v[4];
+      detail::hpsp_check(hpsp_memcpy2d_to_host(v, sizeof(double), c_->column(i_, 0) + k,
+                                               c_->ld() * sizeof(double), sizeof(double), 4),
+                         "Decays::GetDaugtherRange");
+      return Vector4R(v[0], v[1], v[2], v[3]);
+    }
+    // device pointer of component c (0:E 1:px 2:py 3:pz)
+    const double* data(int c) const { return c_->column(i_, c); }
+
+   private:
+    const Decays* c_;
+    unsigned i_;
+  };
+
+  Decays() = delete;
+
+  Decays(double motherMass, std::array<double, N> const& daughtersMasses, size_t nentries = 0)
+      : fMotherMass(motherMass), fMasses(daughtersMasses) {
+    init(nentries);
+  }
+
+  Decays(double motherMass, const double (&daughtersMasses)[N], size_t nentries = 0)
+      : fMotherMass(motherMass) {
+    for (size_t i = 0; i < N; ++i) fMasses[i] = daughtersMasses[i];
+    init(nentries);
+  }
+
+  Decays(const Decays& other) = default;
+  Decays(Decays&& other) noexcept = default;
+  Decays& operator=(const Decays& other) = default;
+  Decays& operator=(Decays&& other) noexcept = default;
+
+  // ---- STL-like interface (Decays.h:298-394)
+  size_t size() const { return fSize; }
+  size_t capacity() const { return fCap; }
+  bool empty() const { return fSize == 0; }
+  void clear() { fSize = 0; fWValid = false; }
+  void reserve(size_t n) {
+    if (n > fCap) regrow(n);
+  }
+  void resize(size_t n) {
+    if (n > fCap) regrow(n);
+    fSize = n;
+    fWValid = false;
+  }
+  void shrink_to_fit() {
+    if (fCap > fSize) regrow(fSize);
+  }
+
+  value_type operator[](size_t k) const {
+    std::array<double, 4 * N> v;
+    if (fSize)
+      detail::hpsp_check(hpsp_memcpy2d_to_host(v.data(), sizeof(double), fCols.data() + k,
+                                               fCap * sizeof(double), sizeof(double), 4 * N),
+                         "Decays::operator[]");
+    return make_row(v, std::make_index_sequence<N>{});
+  }
+  value_type GetEntry(size_t k) const { return (*this)[k]; }
+
+  iterator begin() const { return iterator(this, 0); }
+  iterator end() const { return iterator(this, fSize); }
+  iterator cbegin() const { return begin(); }
+  iterator cend() const { return end(); }
+
+  template <unsigned I>
+  ParticleRange GetDaugtherRange(placeholders::placeholder<I>) const {
+    static_assert(I < N, "particle index out of range");
+    return ParticleRange(this, I);
+  }
+
+  // ---- weights (Decays.h:235-250)
+  PhaseSpaceWeight<Particles...> GetEventWeightFunctor() const {
+    return PhaseSpaceWeight<Particles...>(fMotherMass, fMasses);
+  }
+  template <typename Functor>
+  PhaseSpaceReweight<Functor, Particles...> GetEventWeightFunctor(Functor const& f) const {
+    static_assert(is_phase_space_functor<Functor>::value,
+                  "functor must be one of the device-described phase-space functors");
+    return PhaseSpaceReweight<Functor, Particles...>(f, fMotherMass, fMasses);
+  }
+
+  // ---- unweighting (Decays.inl:415-511)
+  Range<iterator> Unweight(size_t seed = 0x180ec6d33cfd0aba) {
+    const hpsp_decay d = detail::make_decay<N>(fMotherMass, fMasses, "Decays::Unweight");
+    return unweight(d, nullptr, -1.0, seed);
+  }
+
+  template <typename Functor>
+  Range<iterator> Unweight(Functor const& functor, double weight = -1.0,
+                           size_t seed = 0x39abdc4529b1661c) {
+    static_assert(is_phase_space_functor<Functor>::value,
+                  "functor must be one of the device-described phase-space functors");
+    const hpsp_decay d = detail::make_decay<N>(fMotherMass, fMasses, "Decays::Unweight");
+    const hpsp_functor f = detail::to_hpsp(functor);
+    return unweight(d, &f, weight, seed);
+  }
+
+  std::array<double, N> GetMasses() const { return fMasses; }
+  double GetMaxWeight() const { return fMaxWeight; }
+  double GetMotherMass() const { return fMotherMass; }
+
+  // ---- raw device access (SoA)
+  double* column(size_t particle, int component) {
+    return fCols.data() + (4 * particle + component) * fCap;
+  }
+  const double* column(size_t particle, int component) const {
+    return fCols.data() + (4 * particle + component) * fCap;
+  }
+  double* data() { return fCols.data(); }
+  const double* data() const { return fCols.data(); }
+  size_t ld() const { return fCap; }
+
+  // PhaseSpaceWeight column kept alongside the momenta (null when unknown).
+  const double* weights_cache() const { return fWValid ? fW.data() : nullptr; }
+  double* weights_cache_for_write() { return fW.data(); }
+  void set_weights_cache_valid(bool v) { fWValid = v; }
+
+ private:
+  template <size_t... I>
+  static value_type make_row(const std::array<double, 4 * N>& v, std::index_sequence<I...>) {
+    // column (4*i + c) -> row entry i
+    return value_type(Vector4R(v[4 * I], v[4 * I + 1], v[4 * I + 2], v[4 * I + 3])...);
+  }
+
+  void init(size_t n) {
+    const hpsp_decay d = detail::make_decay<N>(fMotherMass, fMasses, "Decays");
+    fMaxWeight = d.max_weight;
+    regrow(n);
+    fSize = n;
+  }
+
+  void regrow(size_t cap) {
+    detail::device_buffer<double> cols(4 * N * cap);
+    detail::device_buffer<double> w(cap);
+    const size_t keep = fSize < cap ? fSize : cap;
+    if (keep && fCols.data())
+      for (size_t c = 0; c < 4 * N; ++c)
+        detail::hpsp_check(hpsp_memcpy_device(cols.data() + c * cap, fCols.data() + c * fCap,
+                                              keep * sizeof(double), nullptr),
+                           "Decays::regrow");
+    if (keep && fWValid)
+      detail::hpsp_check(hpsp_memcpy_device(w.data(), fW.data(), keep * sizeof(double), nullptr),
+                         "Decays::regrow");
+    detail::hpsp_sync("Decays::regrow");
+    fCols.swap(cols);
+    fW.swap(w);
+    fCap = cap;
+  }
+
+  Range<iterator> unweight(const hpsp_decay& d, const hpsp_functor* f, double max_value,
+                           size_t seed) {
+    const size_t n = fSize;
+    detail::device_buffer<double> out(4 * N * (n ? n : 1));
+    detail::device_buffer<double> wout(n ? n : 1);
+    const size_t wsb = hpsp_unweight_workspace_size(n);
+    detail::device_buffer<unsigned char> ws(wsb);
+    detail::device_buffer<uint64_t> nacc(1);
+    const double* win = weights_cache();
+    detail::hpsp_check(hpsp_unweight(&d, f, max_value, (uint64_t)seed, fCols.data(), fCap, n,
+                                     win, out.data(), n ? n : 1, nullptr,
+                                     win ? wout.data() : nullptr, ws.data(), wsb, nacc.data(),
+                                     nullptr),
+                       "Decays::Unweight");
+    uint64_t acc = 0;
+    detail::hpsp_check(hpsp_memcpy_to_host(&acc, nacc.data(), sizeof(acc)), "Decays::Unweight");
+    fCols.swap(out);
+    fW.swap(wout);
+    fCap = n ? n : 1;
+    if (!n) fCap = 1;
+    return Range<iterator>(begin(), begin() + (std::ptrdiff_t)acc);
+  }
+
+  double fMotherMass = 0.0;
+  double fMaxWeight = 0.0;
+  std::array<double, N> fMasses{};
+  size_t fSize = 0;
+  size_t fCap = 0;
+  detail::device_buffer<double> fCols;
+  detail::device_buffer<double> fW;
+  bool fWValid = false;
+};
+
+// events | weight functor  (Range2.inl:213-238): weights computed on the device.
+template <typename... Particles>
+EventWeights operator|(const Decays<hydra::tuple<Particles...>, hydra::device::sys_t>& ev,
+                       const PhaseSpaceWeight<Particles...>& w) {
+  EventWeights out(ev.size());
+  if (ev.size()) {
+    detail::hpsp_check(hpsp_event_weight(&w.descriptor(), nullptr, ev.data(), ev.ld(),
+                                         ev.size(), out.data(), nullptr),
+                       "PhaseSpaceWeight");
+    detail::hpsp_sync("PhaseSpaceWeight");
+  }
+  return out;
+}
+
+template <typename Functor, typename... Particles>
+EventWeights operator|(const Decays<hydra::tuple<Particles...>, hydra::device::sys_t>& ev,
+                       const PhaseSpaceReweight<Functor, Particles...>& w) {
+  EventWeights out(ev.size());
+  if (ev.size()) {
+    detail::hpsp_check(hpsp_event_weight(&w.descriptor(), &w.functor(), ev.data(), ev.ld(),
+                                         ev.size(), out.data(), nullptr),
+                       "PhaseSpaceReweight");
+    detail::hpsp_sync("PhaseSpaceReweight");
+  }
+  return out;
+}
+
+}  // namespace hydra

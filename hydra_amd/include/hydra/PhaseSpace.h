@@ -1,0 +1,280 @@
+// hydra::PhaseSpace<N, GRND> -- the phase-space generator
+// (reference: hydra/PhaseSpace.h:86-345, hydra/detail/PhaseSpace.inl:42-455).
+//
+// Same constructor checks, seed handling and member functions as the
+// reference; Generate, AverageOn and the container's Unweight run the gfx950
+// kernels of libhpsp.so through the C ABI (include/hpsp.h).  Every call is
+// synchronous to the caller, as in the reference.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <functional>
+#include <initializer_list>
+#include <stdexcept>
+#include <utility>
+
+#include <hydra/Decays.h>
+#include <hydra/PhaseSpaceFunctors.h>
+#include <hydra/Types.h>
+#include <hydra/Vector4R.h>
+#include <hydra/detail/Runtime.h>
+#include <hydra/detail/random/engines.h>
+#include <hydra/device/System.h>
+#include <hydra/device/vector.h>
+
+#include "hpsp.h"
+
+namespace hydra {
+
+namespace detail {
+
+// boost-style hash_combine over std::hash<double> (hydra/detail/Hash.h:45-60).
+template <class It>
+inline std::size_t hash_range(It first, It last) {
+  std::size_t seed = 0;
+  for (; first != last; ++first) {
+    std::hash<typename std::decay<decltype(*first)>::type> hasher;
+    seed ^= hasher(*first) + 0x9e3779b9 + (seed << 6) + (seed >> 2);
+  }
+  return seed;
+}
+
+}  // namespace detail
+
+template <size_t N, typename GRND = hydra::default_random_engine>
+class PhaseSpace {
+  static_assert(N >= 2 && N <= HPSP_MAX_N, "PhaseSpace<N>: N must be in 2..HPSP_MAX_N");
+
+ public:
+  PhaseSpace(double motherMass, const double (&daughtersMasses)[N])
+      : fSeed(1), fMotherMass(motherMass), fMaxWeight(0.), fECM(0.) {
+    for (size_t i = 0; i < N; ++i) fMasses[i] = daughtersMasses[i];
+    init();
+  }
+  PhaseSpace(double motherMass, std::array<GReal_t, N> const& daughtersMasses)
+      : fSeed(1), fMotherMass(motherMass), fMaxWeight(0.), fECM(0.) {
+    for (size_t i = 0; i < N; ++i) fMasses[i] = daughtersMasses[i];
+    init();
+  }
+  PhaseSpace(double motherMass, std::initializer_list<GReal_t> const& daughtersMasses)
+      : fSeed(1), fMotherMass(motherMass), fMaxWeight(0.), fECM(0.) {
+    if (daughtersMasses.size() < N)
+      throw std::invalid_argument("[hydra::PhaseSpace]: too few daughter masses");
+    for (size_t i = 0; i < N; ++i) fMasses[i] = *(daughtersMasses.begin() + i);
+    init();
+  }
+
+  // The copy goes through GetSeed(), which returns an int (PhaseSpace.inl:194-216).
+  PhaseSpace(PhaseSpace const& other)
+      : fSeed((size_t)other.GetSeed()), fMotherMass(other.GetMotherMass()),
+        fMaxWeight(other.GetMaxWeight()), fECM(other.GetECM()) {
+    for (size_t i = 0; i < N; ++i) fMasses[i] = other.GetMasses()[i];
+  }
+  template <typename GRND2>
+  PhaseSpace(PhaseSpace<N, GRND2> const& other)
+      : fSeed((size_t)other.GetSeed()), fMotherMass(other.GetMotherMass()),
+        fMaxWeight(other.GetMaxWeight()), fECM(other.GetECM()) {
+    for (size_t i = 0; i < N; ++i) fMasses[i] = other.GetMasses()[i];
+  }
+  PhaseSpace& operator=(PhaseSpace const& other) {
+    if (this == &other) return *this;
+    fSeed = (size_t)other.GetSeed();
+    fMotherMass = other.GetMotherMass();
+    fMaxWeight = other.GetMaxWeight();
+    fECM = other.GetECM();
+    for (size_t i = 0; i < N; ++i) fMasses[i] = other.GetMasses()[i];
+    return *this;
+  }
+
+  // ---- AverageOn (PhaseSpace.inl:255-287) -------------------------------
+  template <typename FUNCTOR>
+  std::pair<GReal_t, GReal_t> AverageOn(hydra::device::sys_t const&, Vector4R const& mother,
+                                        FUNCTOR const& functor, size_t n) {
+    hpsp_stats st = AverageOnStats(mother, functor, 0, n, HPSP_FP64);
+    return std::make_pair(st.mean, ::sqrt(st.m2) / st.w);
+  }
+
+  // Over a container of mothers; the functor sees the mother as particle 0.
+  template <typename FUNCTOR>
+  std::pair<GReal_t, GReal_t> AverageOn(hydra::device::vector<Vector4R> const& mothers,
+                                        FUNCTOR const& functor) {
+    static_assert(is_phase_space_functor<FUNCTOR>::value,
+                  "functor must be one of the device-described phase-space functors");
+    const hpsp_decay d = descriptor(Vector4R(fMotherMass, 0, 0, 0));
+    const hpsp_functor f = detail::to_hpsp(functor);
+    const size_t n = mothers.size();
+    const size_t wsb = hpsp_average_on_workspace_size(n);
+    detail::device_buffer<unsigned char> ws(wsb);
+    detail::device_buffer<hpsp_stats> out(1);
+    detail::hpsp_check(hpsp_average_on_mothers(&d, &f, fSeed, mothers.data(), mothers.ld(), n,
+                                               HPSP_FP64, ws.data(), wsb, out.data(), nullptr),
+                       "PhaseSpace::AverageOn");
+    hpsp_stats st;
+    detail::hpsp_check(hpsp_memcpy_to_host(&st, out.data(), sizeof(st)), "PhaseSpace::AverageOn");
+    return std::make_pair(st.mean, ::sqrt(st.m2) / st.w);
+  }
+
+  // Partial statistics of events first..first+n-1 at a chosen precision; the
+  // partials of consecutive ranges merge with hpsp_stats_merge into the result
+  // of one call over their union.
+  template <typename FUNCTOR>
+  hpsp_stats AverageOnStats(Vector4R const& mother, FUNCTOR const& functor, size_t first,
+                            size_t n, int precision = HPSP_FP64) {
+    static_assert(is_phase_space_functor<FUNCTOR>::value,
+                  "functor must be one of the device-described phase-space functors");
+    const hpsp_decay d = descriptor(mother);
+    const hpsp_functor f = detail::to_hpsp(functor);
+    const size_t wsb = hpsp_average_on_workspace_size(n);
+    detail::device_buffer<unsigned char> ws(wsb);
+    detail::device_buffer<hpsp_stats> out(1);
+    detail::hpsp_check(hpsp_average_on(&d, &f, fSeed, first, n, precision, ws.data(), wsb,
+                                       out.data(), nullptr),
+                       "PhaseSpace::AverageOn");
+    hpsp_stats st;
+    detail::hpsp_check(hpsp_memcpy_to_host(&st, out.data(), sizeof(st)), "PhaseSpace::AverageOn");
+    return st;
+  }
+
+  // ---- Generate (PhaseSpace.inl:351-444) --------------------------------
+  template <typename... P>
+  Range<typename Decays<hydra::tuple<P...>, hydra::device::sys_t>::iterator> Generate(
+      Vector4R const& mother, Decays<hydra::tuple<P...>, hydra::device::sys_t>& events) {
+    static_assert(sizeof...(P) == N, "container arity differs from PhaseSpace<N>");
+    const hpsp_decay d = descriptor(mother);
+    // keep the PhaseSpaceWeight column when the container's weight functor is
+    // this generator's (same masses and fMaxWeight, bit for bit)
+    const bool same = same_decay(events.GetMotherMass(), events.GetMasses()) &&
+                      events.GetMaxWeight() == fMaxWeight;
+    detail::hpsp_check(hpsp_generate(&d, fSeed, 0, events.size(), HPSP_FP64, events.data(),
+                                     events.ld(), 0, nullptr,
+                                     same ? events.weights_cache_for_write() : nullptr, nullptr),
+                       "PhaseSpace::Generate");
+    detail::hpsp_sync("PhaseSpace::Generate");
+    events.set_weights_cache_valid(same && events.size() > 0);
+    return make_range(events.begin(), events.end());
+  }
+
+  // Generate(mothers, daughters): event k boosted by mother k (DecayMothers).
+  template <typename... P>
+  Range<typename Decays<hydra::tuple<P...>, hydra::device::sys_t>::iterator> Generate(
+      hydra::device::vector<Vector4R> const& mothers,
+      Decays<hydra::tuple<P...>, hydra::device::sys_t>& daughters) {
+    static_assert(sizeof...(P) == N, "container arity differs from PhaseSpace<N>");
+    if (daughters.size() < mothers.size()) daughters.resize(mothers.size());
+    const hpsp_decay d = descriptor(Vector4R(fMotherMass, 0, 0, 0));
+    const bool same = same_decay(daughters.GetMotherMass(), daughters.GetMasses()) &&
+                      daughters.GetMaxWeight() == fMaxWeight;
+    detail::hpsp_check(
+        hpsp_generate_mothers(&d, fSeed, 0, mothers.size(), mothers.data(), mothers.ld(),
+                              HPSP_FP64, daughters.data(), daughters.ld(), 0, nullptr,
+                              same ? daughters.weights_cache_for_write() : nullptr, nullptr),
+        "PhaseSpace::Generate");
+    detail::hpsp_sync("PhaseSpace::Generate");
+    daughters.set_weights_cache_valid(same && mothers.size() > 0);
+    return make_range(daughters.begin(), daughters.begin() + (std::ptrdiff_t)mothers.size());
+  }
+
+  // ---- seeds and parameters (PhaseSpace.h:281-326) -----------------------
+  // GetSeed/SetSeed go through int (GInt_t), truncating 64-bit seeds.
+  GInt_t GetSeed() const { return (GInt_t)fSeed; }
+  void SetSeed(GInt_t seed) { fSeed = (size_t)seed; }
+  size_t GetFullSeed() const { return fSeed; }
+
+  const GReal_t* GetMasses() const { return fMasses; }
+  double GetMaxWeight() const { return fMaxWeight; }
+  void SetMaxWeight(double w) { fMaxWeight = w; }
+  double GetMotherMass() const { return fMotherMass; }
+  void SetMotherMass(double m) { fMotherMass = m; }
+  double GetECM() const { return fECM; }
+  void SetECM(double e) { fECM = e; }
+
+  static double PDK(double a, double b, double c) {
+    double x = (a - b - c) * (a + b + c) * (a - b + c) * (a + b - c);
+    x = ::sqrt(x) / (2 * a);
+    return x;
+  }
+
+  // C ABI description of this generator for a mother four-vector.
+  hpsp_decay descriptor(Vector4R const& mother) const {
+    hpsp_decay d{};
+    d.n = (int32_t)N;
+    d.engine = GRND::engine_id;
+    d.mother_mass = fMotherMass;
+    for (size_t i = 0; i < N; ++i) d.masses[i] = fMasses[i];
+    d.ecm = fECM;
+    d.max_weight = fMaxWeight;
+    const double m4[4] = {mother.get(0), mother.get(1), mother.get(2), mother.get(3)};
+    detail::hpsp_check(hpsp_decay_set_mother(&d, m4), "PhaseSpace");
+    return d;
+  }
+
+ private:
+  void init() {
+    if (fMotherMass <= 0.0)
+      throw std::invalid_argument(
+          "[hydra::PhaseSpace]: Mass of mother particle is negative or zero. (fMotherMass <=0)");
+    for (size_t i = 0; i < N; ++i)
+      if (fMasses[i] <= 0.0)
+        throw std::invalid_argument(
+            "[hydra::PhaseSpace]: Mass of daughter particle is negative or zero. (fMass[] <=0)");
+    fSeed *= (1 + detail::hash_range(&fMasses[0], &fMasses[0] + N));
+    hpsp_decay d;
+    const int rc = hpsp_decay_init(&d, (int)N, fMotherMass, fMasses, GRND::engine_id);
+    if (rc == HPSP_ERR_ENERGY)
+      throw std::runtime_error(
+          "[hydra::PhaseSpace]: Not enough energy to generate this decay. ( fECM <=0.");
+    detail::hpsp_check(rc, "PhaseSpace");
+    fECM = d.ecm;
+    fMaxWeight = d.max_weight;
+  }
+
+  bool same_decay(double M, std::array<double, N> const& m) const {
+    if (M != fMotherMass) return false;
+    for (size_t i = 0; i < N; ++i)
+      if (m[i] != fMasses[i]) return false;
+    return true;
+  }
+
+  size_t fSeed;
+  GReal_t fMotherMass;
+  GReal_t fMaxWeight;
+  GReal_t fECM;
+  GReal_t fMasses[N];
+};
+
+// PhaseSpaceIntegrator<N, Backend, GRND>::Integrate (PhaseSpaceIntegrator.h:57-160,
+// PhaseSpaceIntegrator.inl:37-40): AverageOn with the mother at rest.
+template <size_t N, typename Backend = hydra::device::sys_t,
+          typename GRND = hydra::default_random_engine>
+class PhaseSpaceIntegrator {
+ public:
+  PhaseSpaceIntegrator(const GReal_t motherMass, const GReal_t (&daughtersMasses)[N], size_t n)
+      : fGenerator(motherMass, daughtersMasses), fMother(motherMass, 0, 0, 0), fNSamples(n) {}
+  PhaseSpaceIntegrator(const GReal_t motherMass, std::array<GReal_t, N> const& daughtersMasses,
+                       size_t n)
+      : fGenerator(motherMass, daughtersMasses), fMother(motherMass, 0, 0, 0), fNSamples(n) {}
+  PhaseSpaceIntegrator(const GReal_t motherMass,
+                       std::initializer_list<GReal_t> const& daughtersMasses, size_t n)
+      : fGenerator(motherMass, daughtersMasses), fMother(motherMass, 0, 0, 0), fNSamples(n) {}
+
+  const PhaseSpace<N, GRND>& GetGenerator() const { return fGenerator; }
+  PhaseSpace<N, GRND>& GetGenerator() { return fGenerator; }
+  void SetGenerator(const PhaseSpace<N, GRND>& g) { fGenerator = g; }
+  const Vector4R& GetMother() const { return fMother; }
+  void SetMother(const Vector4R& m) { fMother = m; }
+  size_t GetNSamples() const { return fNSamples; }
+  void SetNSamples(size_t n) { fNSamples = n; }
+
+  template <typename FUNCTOR>
+  std::pair<GReal_t, GReal_t> Integrate(FUNCTOR const& functor) {
+    return fGenerator.AverageOn(hydra::device::sys, fMother, functor, fNSamples);
+  }
+
+ private:
+  PhaseSpace<N, GRND> fGenerator;
+  Vector4R fMother;
+  size_t fNSamples;
+};
+
+}  // namespace hydra

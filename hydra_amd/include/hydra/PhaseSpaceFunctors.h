@@ -1,0 +1,86 @@
+// Per-event functors for AverageOn and Unweight(functor).
+//
+// The reference compiles any callable of the N daughters into its Thrust
+// kernels (hydra/Lambda.h:133-200).  Here the kernels are prebuilt for gfx950,
+// so a functor is a description the kernels evaluate on the device (the
+// hpsp_functor of include/hpsp.h):
+//   ConstantFunctor          f = 1
+//   InvariantMass2           f = (p0 + p1)^2
+//   BreitWignerSum           f = |sum_k c_k / (M_k^2 - s_k - i M_k G_k)|^2,
+//                            s_k = (p[a_k] + p[b_k])^2
+// Any other functor type is rejected at compile time.
+#pragma once
+
+#include <complex>
+#include <stdexcept>
+#include <initializer_list>
+#include <type_traits>
+#include <vector>
+
+#include "hpsp.h"
+
+namespace hydra {
+
+struct ConstantFunctor {};
+
+struct InvariantMass2 {};
+
+// One relativistic Breit-Wigner term c / (M^2 - s - i M G) of the pair (a, b).
+struct BreitWignerTerm {
+  double mass;
+  double width;
+  std::complex<double> coupling;
+  int a;
+  int b;
+};
+
+class BreitWignerSum {
+ public:
+  BreitWignerSum(std::initializer_list<BreitWignerTerm> terms) : terms_(terms) {}
+  explicit BreitWignerSum(std::vector<BreitWignerTerm> terms) : terms_(std::move(terms)) {}
+  const std::vector<BreitWignerTerm>& terms() const { return terms_; }
+
+ private:
+  std::vector<BreitWignerTerm> terms_;
+};
+
+template <typename F>
+struct is_phase_space_functor
+    : std::integral_constant<bool, std::is_same<F, ConstantFunctor>::value ||
+                                       std::is_same<F, InvariantMass2>::value ||
+                                       std::is_same<F, BreitWignerSum>::value> {};
+
+namespace detail {
+
+inline hpsp_functor to_hpsp(const ConstantFunctor&) {
+  hpsp_functor f{};
+  f.kind = HPSP_F_ONE;
+  return f;
+}
+
+inline hpsp_functor to_hpsp(const InvariantMass2&) {
+  hpsp_functor f{};
+  f.kind = HPSP_F_MASS2_01;
+  return f;
+}
+
+inline hpsp_functor to_hpsp(const BreitWignerSum& s) {
+  hpsp_functor f{};
+  f.kind = HPSP_F_BW_SUM;
+  if (s.terms().size() > HPSP_MAX_TERMS)
+    throw std::invalid_argument("[hydra::BreitWignerSum]: too many terms");
+  f.nterms = (int32_t)s.terms().size();
+  for (size_t k = 0; k < s.terms().size(); ++k) {
+    const BreitWignerTerm& t = s.terms()[k];
+    f.m[k] = t.mass;
+    f.g[k] = t.width;
+    f.cr[k] = t.coupling.real();
+    f.ci[k] = t.coupling.imag();
+    f.a[k] = t.a;
+    f.b[k] = t.b;
+  }
+  return f;
+}
+
+}  // namespace detail
+}  // namespace hydra

@@ -208,6 +208,11 @@ int hpsp_memcpy_to_device(void* dst, const void* src, size_t bytes);
 int hpsp_memcpy_to_host(void* dst, const void* src, size_t bytes);
 int hpsp_memcpy_device(void* dst, const void* src, size_t bytes, void* stream);
 int hpsp_stream_synchronize(void* stream);
+/* Strided copies (rows of SoA columns): height rows of width bytes. */
+int hpsp_memcpy2d_to_host(void* dst, size_t dpitch, const void* src, size_t spitch,
+                          size_t width, size_t height);
+int hpsp_memcpy2d_to_device(void* dst, size_t dpitch, const void* src, size_t spitch,
+                            size_t width, size_t height);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,162 @@
+// Drives the C++ drop-in API (hydra_amd/include/hydra) the way the
+// reference's phase-space examples do (examples/phase_space/phsp_basic.inl,
+// phsp_unweighting.inl, phsp_averaging_functor.inl) and prints what it sees as
+// one JSON object; tests/test_cpp_api.py checks it against the golden vectors
+// and the oracle.
+//   test_api host   -- host-only parts (engines, constructor, seeds, errors)
+//   test_api gpu    -- plus Generate / weights / Unweight / AverageOn on the GPU
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <hydra/Decays.h>
+#include <hydra/Function.h>
+#include <hydra/PhaseSpace.h>
+#include <hydra/Vector4R.h>
+#include <hydra/device/System.h>
+
+declarg(A, hydra::Vector4R)
+declarg(B, hydra::Vector4R)
+declarg(C, hydra::Vector4R)
+
+using namespace hydra::arguments;
+
+static void emit_array(const char* key, const std::vector<double>& v, bool comma = true) {
+  std::printf("\"%s\": [", key);
+  for (size_t i = 0; i < v.size(); ++i) std::printf("%s%.17g", i ? ", " : "", v[i]);
+  std::printf("]%s\n", comma ? "," : "");
+}
+
+static void emit_words(const char* key, const std::vector<unsigned long long>& v) {
+  std::printf("\"%s\": [", key);
+  for (size_t i = 0; i < v.size(); ++i) std::printf("%s\"%llu\"", i ? ", " : "", v[i]);
+  std::printf("],\n");
+}
+
+template <typename E>
+static std::vector<unsigned long long> words(unsigned long long seed, unsigned long long disc, int n) {
+  E eng(seed);
+  eng.discard(disc);
+  std::vector<unsigned long long> w;
+  for (int i = 0; i < n; ++i) w.push_back((unsigned long long)eng());
+  return w;
+}
+
+int main(int argc, char** argv) {
+  const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
+  const double P_mass = 5.279, A_mass = 3.0969, B_mass = 0.493677, C_mass = 0.13957;
+  double masses[3]{A_mass, B_mass, C_mass};
+
+  std::printf("{\n");
+  // ---- engines
+  const unsigned long long seed = 159753ULL, big = (1ULL << 40) + 12345ULL;
+  emit_words("rng_squares3_d0", words<hydra::random::squares3>(seed, 0, 64));
+  emit_words("rng_squares3_dbig", words<hydra::random::squares3>(seed, big, 64));
+  emit_words("rng_squares4_d0", words<hydra::random::squares4>(seed, 0, 64));
+  emit_words("rng_philox_d0", words<hydra::random::philox>(seed, 0, 64));
+  emit_words("rng_philox_long_dbig", words<hydra::random::philox_long>(seed, big, 64));
+  emit_words("rng_threefry_d0", words<hydra::random::threefry>(seed, 0, 64));
+  emit_words("rng_threefry_long_d0", words<hydra::random::threefry_long>(seed, 0, 64));
+  emit_words("rng_default_d0", words<hydra::default_random_engine>(seed, 0, 64));
+
+  // ---- constructor products, default seed and its int truncation
+  hydra::PhaseSpace<3> phsp{P_mass, masses};
+  std::printf("\"ecm\": %.17g,\n\"max_weight\": %.17g,\n", phsp.GetECM(), phsp.GetMaxWeight());
+  std::printf("\"seed_full\": \"%llu\",\n\"seed_get\": %d,\n",
+              (unsigned long long)phsp.GetFullSeed(), phsp.GetSeed());
+  hydra::PhaseSpace<3> copy(phsp);
+  std::printf("\"copy_seed_full\": \"%llu\",\n", (unsigned long long)copy.GetFullSeed());
+
+  // ---- constructor errors (PhaseSpace.inl:49-76)
+  int err_mass = 0, err_energy = 0, err_mother = 0;
+  try {
+    double bad[3]{A_mass, -1.0, C_mass};
+    hydra::PhaseSpace<3> p(P_mass, bad);
+  } catch (std::invalid_argument const&) {
+    err_mass = 1;
+  }
+  try {
+    hydra::PhaseSpace<3> p(3.5, masses);
+  } catch (std::runtime_error const&) {
+    err_energy = 1;
+  }
+  try {
+    hydra::PhaseSpace<3> p(0.0, masses);
+  } catch (std::invalid_argument const&) {
+    err_mother = 1;
+  }
+  std::printf("\"err_mass\": %d,\n\"err_energy\": %d,\n\"err_mother\": %d,\n", err_mass,
+              err_energy, err_mother);
+
+  if (gpu) {
+    hydra::Vector4R Parent(P_mass, 0.0, 0.0, 0.0);
+    phsp.SetSeed(159753);
+    const size_t n = 4096;
+    hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> Events(P_mass, masses, n);
+    phsp.Generate(Parent, Events);
+    std::vector<double> rows;
+    for (size_t i = 0; i < 16; ++i) {
+      auto ev = Events[i];
+      const hydra::Vector4R a = std::get<0>(ev), b = std::get<1>(ev), c = std::get<2>(ev);
+      for (const hydra::Vector4R* v : {&a, &b, &c})
+        for (int q = 0; q < 4; ++q) rows.push_back(v->get(q));
+    }
+    emit_array("gen_rows16", rows);
+    auto weights = Events | Events.GetEventWeightFunctor();
+    emit_array("gen_weights", weights.to_host());
+    auto pi_range = Events.GetDaugtherRange(hydra::placeholders::_2);
+    const hydra::Vector4R pi7 = pi_range[7];
+    emit_array("pion_event7", {pi7.get(0), pi7.get(1), pi7.get(2), pi7.get(3)});
+
+    // Unweight() on a copy, Unweight(functor) on another
+    auto E2 = Events;
+    auto accepted = E2.Unweight();
+    std::printf("\"unweight_nacc\": %zu,\n", accepted.size());
+    std::vector<double> acc_rows;
+    for (size_t i = 0; i < 4; ++i) {
+      auto ev = accepted[i];
+      const hydra::Vector4R a = std::get<0>(ev);
+      for (int q = 0; q < 4; ++q) acc_rows.push_back(a.get(q));
+    }
+    emit_array("unweight_first_A", acc_rows);
+    auto E3 = Events;
+    auto acc_f = E3.Unweight(hydra::InvariantMass2{}, -1.0, 0x39abdc4529b1661cULL);
+    std::printf("\"unweight_f_nacc\": %zu,\n", acc_f.size());
+
+    // AverageOn with the three-resonance Dalitz functor, and its integrator
+    hydra::BreitWignerSum bw{{0.892, 0.05, {0.4967141530, -0.1382643012}, 1, 2},
+                             {1.43, 0.11, {0.6476885381, 1.5230298564}, 1, 2},
+                             {0.775, 0.149, {-0.2341533747, -0.2341368981}, 0, 2}};
+    auto avg = phsp.AverageOn(hydra::device::sys, Parent, bw, 100000);
+    emit_array("avg_bw3", {avg.first, avg.second});
+    auto avg1 = phsp.AverageOn(hydra::device::sys, Parent, hydra::ConstantFunctor{}, 10000);
+    emit_array("avg_one", {avg1.first, avg1.second});
+    hydra::PhaseSpaceIntegrator<3, hydra::device::sys_t> integrator(P_mass, masses, 100000);
+    integrator.GetGenerator().SetSeed(159753);
+    auto integ = integrator.Integrate(hydra::InvariantMass2{});
+    emit_array("integrate_m2", {integ.first, integ.second});
+
+    // Generate(mothers, daughters)
+    std::vector<hydra::Vector4R> moms;
+    for (int k = 0; k < 256; ++k) {
+      const double px = 0.01 * k, py = -0.02 * k, pz = 0.005 * k;
+      moms.emplace_back(std::sqrt(P_mass * P_mass + px * px + py * py + pz * pz), px, py, pz);
+    }
+    hydra::device::vector<hydra::Vector4R> mothers(moms);
+    hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> Daughters(P_mass, masses, 256);
+    phsp.Generate(mothers, Daughters);
+    std::vector<double> mrows;
+    for (size_t i : {0u, 100u, 255u}) {
+      auto ev = Daughters[i];
+      const hydra::Vector4R c = std::get<2>(ev);
+      for (int q = 0; q < 4; ++q) mrows.push_back(c.get(q));
+    }
+    emit_array("mothers_rows_C", mrows);
+  }
+  std::printf("\"ok\": 1\n}\n");
+  return 0;
+}

@@ -1,0 +1,78 @@
+"""The C++ drop-in API (hydra_amd/include/hydra) as a user of the reference
+would drive it (tests/cpp/test_api.cpp), checked against the reference's
+golden vectors and the oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import rows_to_cols
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "bin", "test_api")
+M, MASSES = 5.279, [3.0969, 0.493677, 0.13957]
+
+
+def run(mode):
+    if not os.path.exists(BIN):
+        pytest.fail(f"{BIN} not built (make -C tests/cpp)")
+    out = subprocess.run([BIN, mode], check=True, capture_output=True, text=True, timeout=120)
+    return json.loads(out.stdout)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def test_host_engines_constructor_and_seeds(golden):
+    r = run("host")
+    big = 2**40 + 12345
+    pairs = [("rng_squares3_d0", "rng_squares3_d0"), ("rng_squares3_dbig", f"rng_squares3_d{big}"),
+             ("rng_squares4_d0", "rng_squares4_d0"), ("rng_philox_d0", "rng_philox_d0"),
+             ("rng_philox_long_dbig", f"rng_philox_long_d{big}"),
+             ("rng_threefry_d0", "rng_threefry_d0"),
+             ("rng_threefry_long_d0", "rng_threefry_long_d0"),
+             ("rng_default_d0", "rng_squares3_d0")]
+    for ours, ref in pairs:
+        assert [int(x) for x in r[ours]] == [int(x) for x in golden[ref]], ours
+    assert bits([r["ecm"], r["max_weight"]]).tolist() == bits(golden["params_c1_3body"]).tolist()
+    # default seed 1*(1+hash_range(masses)) and GetSeed()'s int truncation
+    assert int(r["seed_full"]) == int(golden["seed_c1_3body"][0])
+    assert np.int32(r["seed_get"]).astype(np.uint64) == golden["seed_c1_3body"][1]
+    assert int(r["copy_seed_full"]) == r["seed_get"] & 0xFFFFFFFFFFFFFFFF
+    assert r["err_mass"] == 1 and r["err_energy"] == 1 and r["err_mother"] == 1
+
+
+@pytest.mark.gpu
+def test_gpu_generate_unweight_average(golden, oracle):
+    r = run("gpu")
+    ref = golden["decay_c1_3body_squares3_p"][:16]
+    got = np.array(r["gen_rows16"]).reshape(16, 3, 4)
+    E = np.abs(ref[..., 0:1])
+    assert np.max(np.abs(got - ref) / E) <= 1e-12
+    w = np.array(r["gen_weights"])
+    assert w.shape == (4096,)
+    assert np.max(np.abs(w - golden["gen_c1_w"]) / golden["gen_c1_w"]) <= 1e-12
+    pi7 = np.array(r["pion_event7"])
+    assert np.max(np.abs(pi7 - golden["gen_c1_p"][7, 2]) / pi7[0]) <= 1e-12
+    assert r["unweight_nacc"] == int(golden["uw_c1_nacc"][0])
+    first = np.array(r["unweight_first_A"]).reshape(4, 4)
+    refA = golden["uw_c1_p"][:4, 0]
+    assert np.max(np.abs(first - refA) / refA[:, 0:1]) <= 1e-12
+    assert r["unweight_f_nacc"] == int(golden["uwf_c1_nacc"][0])
+    for key, gkey in (("avg_bw3", "avg_c1_bw3"), ("integrate_m2", "avg_c1_m2")):
+        mean, sig = r[key]
+        gm, gs = golden[gkey]
+        assert abs(mean - gm) <= 1e-12 * abs(gm), key
+        assert abs(sig - gs) <= 1e-10 * abs(gs), key
+    assert r["avg_one"] == [1.0, 0.0]
+    # Generate(mothers, daughters) against the oracle
+    k = np.arange(256)
+    px, py, pz = 0.01 * k, -0.02 * k, 0.005 * k
+    mothers = np.stack([np.sqrt(M * M + px * px + py * py + pz * pz), px, py, pz])
+    cols, _ = oracle.generate_mothers(M, MASSES, 159753, mothers)
+    exp = cols.reshape(3, 4, 256)[2][:, [0, 100, 255]].T
+    got = np.array(r["mothers_rows_C"]).reshape(3, 4)
+    assert np.max(np.abs(got - exp) / exp[:, 0:1]) <= 1e-12
