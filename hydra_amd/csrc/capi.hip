@@ -103,7 +103,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 struct UwLayout {
-  size_t flags, tile_count, tile_off, maxv, block_max, total;
+  size_t flags, tile_count, tile_off, run_sum, maxv, block_max, total;
 };
 
 UwLayout uw_layout(uint64_t n) {
@@ -114,7 +114,8 @@ UwLayout uw_layout(uint64_t n) {
   L.flags = 0;
   L.tile_count = L.flags + align_up(ngroups * sizeof(uint64_t));
   L.tile_off = L.tile_count + align_up(ntiles * sizeof(uint32_t));
-  L.maxv = L.tile_off + align_up(ntiles * sizeof(uint64_t));
+  L.run_sum = L.tile_off + align_up(ntiles * sizeof(uint64_t));
+  L.maxv = L.run_sum + align_up((ntiles / 1024 + 2) * sizeof(uint64_t));
   L.block_max = L.maxv + align_up(sizeof(double));
   L.total = L.block_max + align_up(nblocks * sizeof(double));
   return L;
@@ -338,6 +339,7 @@ int hpsp_unweight(const hpsp_decay* d, const hpsp_functor* f, double max_value, 
   a.flags = reinterpret_cast<uint64_t*>(base + L.flags);
   a.tile_count = reinterpret_cast<uint32_t*>(base + L.tile_count);
   a.tile_off = reinterpret_cast<uint64_t*>(base + L.tile_off);
+  a.run_sum = reinterpret_cast<uint64_t*>(base + L.run_sum);
   a.n_acc = n_acc_dev;
   return hip_status(launch_unweight(d->n, a, s));
 }

@@ -1,9 +1,13 @@
 // FP64 phase-space kernels for gfx950: GENBOD generation into SoA columns,
 // AverageOn (fused generation + functor + StatsPHSP reduction) and the
-// per-event weight kernels.  Built with -ffp-contract=off so that every
-// double operation is the reference's (DecayMother.h:163-276,
-// Vector4R.inl:312-369, Decays.inl:215-231), apart from terms that are
-// exactly zero and can only change the sign of a zero result.
+// per-event weight kernels.  Built with -ffp-contract=off.  The weight chain
+// (uniforms, invMas, pdk, the weight product) and PhaseSpaceWeight repeat the
+// reference's double operations one for one with correctly rounded sqrt and
+// division, so weights agree bit for bit (DecayMother.h:170-212,
+// Decays.inl:215-231).  The momenta (DecayMother.h:218-269) are formed with
+// the same operations except: exactly-zero terms are dropped, sin/cos and
+// square roots outside the weight chain are evaluated to about one ulp, and
+// the boost factors use gamma = E*/M, gamma*beta = p/M.
 #include <array>
 #include <type_traits>
 #include <utility>
@@ -47,7 +51,7 @@ __device__ __forceinline__ double genbod_f64(const DecayK& d, const WordFn& word
   }
 
   // Daughter 0 along +y; it is rotated for the first time at i = 1.
-  p[0][0] = sqrt(pd[0] * pd[0] + d.m[0] * d.m[0]);
+  p[0][0] = sqrt_fast(pd[0] * pd[0] + d.m[0] * d.m[0]);
   p[0][1] = 0.0;
   p[0][2] = pd[0];
   p[0][3] = 0.0;
@@ -57,13 +61,13 @@ __device__ __forceinline__ double genbod_f64(const DecayK& d, const WordFn& word
     // keep the scheduler from hoisting later steps' angles and boosts
     // above this one: their results would all be live at once
     if constexpr (N > 4) __builtin_amdgcn_sched_barrier(0);
-    p[i][0] = sqrt(pd[i - 1] * pd[i - 1] + d.m[i] * d.m[i]);
+    p[i][0] = sqrt_fast(pd[i - 1] * pd[i - 1] + d.m[i] * d.m[i]);
     p[i][1] = 0.0;
     p[i][2] = -pd[i - 1];
     p[i][3] = 0.0;
 
     const double cZ = 2 * to_unit<ENGINE>(word(N - 2 + 2 * (i - 1))) - 1;
-    const double sZ = sqrt(1 - cZ * cZ);
+    const double sZ = sqrt_fast(1 - cZ * cZ);
     // sin/cos of angY = 2*pi*u
     double sY, cY;
     sincos_2pi(to_unit<ENGINE>(word(N - 2 + 2 * (i - 1) + 1)), sY, cY);
@@ -96,8 +100,8 @@ __device__ __forceinline__ double genbod_f64(const DecayK& d, const WordFn& word
     // E*/M, pd/M and gamma - 1; the momenta are formed as
     // E' = gamma E + gamma beta py and py' = gamma beta E + gamma py.
     if (pd[i] > 0.0) {
-      const double inv_m = 1.0 / invMas[i];
-      const double estar = sqrt(pd[i] * pd[i] + invMas[i] * invMas[i]);
+      const double inv_m = rcp_fast(invMas[i]);
+      const double estar = sqrt_fast(pd[i] * pd[i] + invMas[i] * invMas[i]);
       const double gamma = estar * inv_m;
       const double gby = pd[i] * inv_m;
 #pragma unroll
@@ -242,7 +246,7 @@ void k_average_f64(const AvgArgs a) {
           for (int i = 0; i < N; ++i) boost_folded(p[i], a.d);
         }
       }
-      f = functor_eval<double>(a.f, p);
+      f = functor_eval<double, false>(a.f, p);
       if (live) stats_push(acc, f, wt);
     }
     acc = wave_reduce_stats(acc);

@@ -1,10 +1,11 @@
 // Mixed-precision phase-space kernels for gfx950.  The weight chain stays
-// FP64 and follows the reference operation for operation (the uniforms of
-// the invariant masses, invMas, pdk -- where a - b - c cancels near
-// threshold -- and the weight product), so generation weights are the FP64
-// ones bit for bit.  The kinematics after pdk (energies, rotations, boosts,
-// functor) run in FP32 with FMA contraction, hardware sin/cos in revolutions
-// and reciprocal square roots.  Accumulation of AverageOn stays FP64.
+// FP64 in the reference's order (the uniforms of the invariant masses,
+// invMas, pdk -- where a - b - c cancels near threshold -- and the weight
+// product), with one-ulp square root and reciprocal, so generation weights
+// agree with the FP64 ones to a few ulp.  The kinematics after pdk
+// (energies, rotations, boosts, functor) run in FP32 with FMA contraction,
+// hardware sin/cos in revolutions, square roots and reciprocals.
+// Accumulation of AverageOn stays FP64.
 #include <array>
 #include <type_traits>
 #include <utility>
@@ -31,12 +32,21 @@ __device__ __forceinline__ void polar_f32(uint32_t w, float& cZ, float& sZ) {
   cZ = (float)t * 0x1p-32f;
   const float a = (float)w;
   const float b = (float)((1ULL << 32) - (uint64_t)w);
-  sZ = __builtin_sqrtf(a * b) * 0x1p-31f;
+  sZ = __builtin_amdgcn_sqrtf(a * b) * 0x1p-31f;
+}
+
+// pdk in FP64 with the reference's subtraction order (so the a - b - c
+// cancellation near threshold is the FP64 one) and a one-ulp square root and
+// reciprocal instead of correctly rounded ones.
+__device__ __forceinline__ double pdk_fast(double a, double b, double c) {
+#pragma clang fp contract(off)
+  const double x = (a - b - c) * (a + b + c) * (a - b + c) * (a + b - c);
+  return sqrt_fast(x) * rcp_fast(2 * a);
 }
 
 // FP64 weight chain of DecayMother::process (DecayMother.h:170-212):
-// sorted uniforms, invMas, pdk and the weight product, operation for
-// operation; the momenta pd are handed on in FP32.
+// sorted uniforms, invMas, pdk and the weight product in the reference's
+// order; the momenta pd are handed on in FP32.
 template <int N, int ENGINE, typename WordFn>
 __device__ __forceinline__ double weight_chain(const DecayK& d, const WordFn& word,
                                                double (&invMas)[N], float (&pd)[N - 1]) {
@@ -62,7 +72,7 @@ __device__ __forceinline__ double weight_chain(const DecayK& d, const WordFn& wo
   double wt = d.maxw;
 #pragma unroll
   for (int n = 0; n < N - 1; ++n) {
-    const double q = pdk(invMas[n + 1], invMas[n], d.m[n + 1]);
+    const double q = pdk_fast(invMas[n + 1], invMas[n], d.m[n + 1]);
     wt *= q;
     pd[n] = (float)q;
   }
@@ -77,7 +87,7 @@ __device__ __forceinline__ double genbod_mixed(const DecayK& d, const WordFn& wo
   const double wt = weight_chain<N, ENGINE>(d, word, invMas, pd);
 
   const float m0 = (float)d.m[0];
-  p[0][0] = __builtin_sqrtf(pd[0] * pd[0] + m0 * m0);
+  p[0][0] = __builtin_amdgcn_sqrtf(pd[0] * pd[0] + m0 * m0);
   p[0][1] = 0.0f;
   p[0][2] = pd[0];
   p[0][3] = 0.0f;
@@ -85,7 +95,7 @@ __device__ __forceinline__ double genbod_mixed(const DecayK& d, const WordFn& wo
 #pragma unroll
   for (int i = 1; i < N; ++i) {
     const float mi = (float)d.m[i];
-    p[i][0] = __builtin_sqrtf(pd[i - 1] * pd[i - 1] + mi * mi);
+    p[i][0] = __builtin_amdgcn_sqrtf(pd[i - 1] * pd[i - 1] + mi * mi);
     p[i][1] = 0.0f;
     p[i][2] = -pd[i - 1];
     p[i][3] = 0.0f;
@@ -126,7 +136,7 @@ __device__ __forceinline__ double genbod_mixed(const DecayK& d, const WordFn& wo
     const float beta = pd[i] * __builtin_amdgcn_rsqf(pi2 + M * M);
     const float b2 = beta * beta;
     const float gamma = __builtin_amdgcn_rsqf(1.0f - b2);
-    const float gb2yy = gamma * gamma / (gamma + 1.0f) * b2;
+    const float gb2yy = gamma * gamma * __builtin_amdgcn_rcpf(gamma + 1.0f) * b2;
     const float gby = gamma * beta;
 #pragma unroll
     for (int j = 0; j <= i; ++j) {
@@ -142,7 +152,7 @@ __device__ __forceinline__ void boost_general_f32(float (&v)[4], float bx, float
   const float b2 = bx * bx + by * by + bz * bz;
   if (b2 > 0.0f && b2 < 1.0f) {
     const float gamma = __builtin_amdgcn_rsqf(1.0f - b2);
-    const float gb2 = gamma * gamma / (gamma + 1.0f);
+    const float gb2 = gamma * gamma * __builtin_amdgcn_rcpf(gamma + 1.0f);
     const float e2 = v[0], px2 = v[1], py2 = v[2], pz2 = v[3];
     const float bp = bx * px2 + by * py2 + bz * pz2;
     v[0] = gamma * (e2 + bp);
@@ -199,10 +209,10 @@ __global__ __launch_bounds__(kBlock) void k_generate_mixed(const GenArgs a) {
     const double wt = genbod_mixed<N, ENGINE>(a.d, words, p);
     if constexpr (MOTHERS) {
       const uint64_t km = live ? k : 0;
-      const float e = (float)a.mothers[km];
-      const float bx = (float)a.mothers[a.mld + km] / e;
-      const float by = (float)a.mothers[2 * a.mld + km] / e;
-      const float bz = (float)a.mothers[3 * a.mld + km] / e;
+      const float ie = __builtin_amdgcn_rcpf((float)a.mothers[km]);
+      const float bx = (float)a.mothers[a.mld + km] * ie;
+      const float by = (float)a.mothers[2 * a.mld + km] * ie;
+      const float bz = (float)a.mothers[3 * a.mld + km] * ie;
 #pragma unroll
       for (int i = 0; i < N; ++i) boost_general_f32(p[i], bx, by, bz);
     } else {
@@ -262,7 +272,8 @@ __global__ __launch_bounds__(kBlock) void k_average_mixed(const AvgArgs a) {
         float m4[4];
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc) m4[cc] = (float)a.mothers[(uint64_t)cc * a.mld + km];
-        const float bx = m4[1] / m4[0], by = m4[2] / m4[0], bz = m4[3] / m4[0];
+        const float ie = __builtin_amdgcn_rcpf(m4[0]);
+        const float bx = m4[1] * ie, by = m4[2] * ie, bz = m4[3] * ie;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           boost_general_f32(q[i], bx, by, bz);
@@ -275,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_average_mixed(const AvgArgs a) {
         wt = genbod_mixed<N, ENGINE>(a.d, words, p);
         final_boost_f32<N>(p, a.d);
       }
-      const double f = functor_eval<float>(a.f, p);
+      const double f = functor_eval<float, false>(a.f, p);
       if (live) stats_push(acc, f, wt);
     }
     acc = wave_reduce_stats(acc);

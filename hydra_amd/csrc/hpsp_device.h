@@ -329,15 +329,59 @@ __device__ __forceinline__ void sincos_2pi(double u, double& s, double& c) {
 // contraction off itself, so the operation sequence holds in any translation
 // unit.
 
+// Correctly rounded FP64 square root: reciprocal-square-root estimate, one
+// Newton step on (g ~ sqrt(x), h ~ 1/(2 sqrt(x))), then two residual
+// corrections g += (x - g*g) * h (Markstein's scheme, as the AMDGPU backend
+// expands llvm.sqrt.f64).  The backend's rescaling of tiny inputs and its
+// zero/infinity fix-ups are replaced by sending inputs outside
+// [2^-767, 2^1000] to the library sqrt.
+__device__ __forceinline__ double sqrt_cr(double x) {
+  if (!(x >= 0x1p-767 && x <= 0x1p+1000)) return __builtin_sqrt(x);
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = 0.5 * y;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
+
+// Faithful (within about one ulp) square root for x >= 0 where only a
+// tolerance applies (energies, sin of the polar angle, boost factors): the
+// same scheme with one residual correction; zero stays zero because the
+// estimate is taken of max(x, 2^-1000).
+__device__ __forceinline__ double sqrt_fast(double x) {
+  const double y = __builtin_amdgcn_rsq(__builtin_fmax(x, 0x1p-1000));
+  double g = x * y;
+  double h = 0.5 * y;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  const double d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
+
+// Reciprocal within about one ulp: estimate and two Newton steps.
+__device__ __forceinline__ double rcp_fast(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
 __device__ __forceinline__ double pdk(double a, double b, double c) {
 #pragma clang fp contract(off)
-  return sqrt((a - b - c) * (a + b + c) * (a - b + c) * (a + b - c)) / (2 * a);
+  return sqrt_cr((a - b - c) * (a + b + c) * (a - b + c) * (a + b - c)) / (2 * a);
 }
 
 __device__ __forceinline__ double mass_of(double e, double x, double y, double z) {
 #pragma clang fp contract(off)
   const double m2 = e * e - x * x - y * y - z * z;
-  return (m2 > 0.0) ? sqrt(m2) : 0.0;
+  return (m2 > 0.0) ? sqrt_cr(m2) : 0.0;
 }
 
 // Vector4R::applyBoostTo(bx,by,bz) forward (Vector4R.inl:312-369) for a
@@ -399,9 +443,13 @@ __device__ __forceinline__ double event_weight_f64(const double (&p)[N][4], cons
 }
 
 // Functor evaluation (same arithmetic sequence as the oracle's).  P is the
-// number of four-vectors visible to the functor.
-template <typename Real, int P>
+// number of four-vectors visible to the functor.  Exact = true keeps the
+// divisions correctly rounded (Unweight(functor) and PhaseSpaceReweight must
+// reproduce the reference's bits); otherwise the Breit-Wigner divisions use
+// a reciprocal (AverageOn, compared within a tolerance).
+template <typename Real, bool Exact = true, int P>
 __device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p)[P][4]) {
+#pragma clang fp contract(off)
   if (f.kind == HPSP_F_ONE) return 1.0;
   if (f.kind == HPSP_F_MASS2_01) {
     const Real s0 = p[0][0] + p[1][0], s1 = p[0][1] + p[1][1];
@@ -428,8 +476,19 @@ __device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p
     const Real dr = M * M - s;
     const Real di = -(M * G);
     const Real den = dr * dr + di * di;
-    const Real br = dr / den;
-    const Real bi = -di / den;
+    Real br, bi;
+    if constexpr (Exact) {
+      br = dr / den;
+      bi = -di / den;
+    } else {
+      Real inv;
+      if constexpr (sizeof(Real) == 4)
+        inv = __builtin_amdgcn_rcpf(den);
+      else
+        inv = rcp_fast(den);
+      br = dr * inv;
+      bi = -di * inv;
+    }
     const Real cr = (Real)f.cr[k], ci = (Real)f.ci[k];
     const Real tr = cr * br - ci * bi;
     const Real ti = cr * bi + ci * br;
@@ -474,7 +533,7 @@ __device__ __forceinline__ void stats_push(Stats& s, double f, double wt) {
   }
   const double w = s.w + wt;
   const double delta = f - s.mean;
-  const double r = wt / w;
+  const double r = wt * rcp_fast(w);
   const double mean = s.mean + delta * r;
   s.m2 = s.m2 + wt * delta * (f - mean);
   s.mean = mean;

@@ -59,6 +59,7 @@ struct UnweightArgs {
   uint64_t* flags;      // one bit per event, 64 per word
   uint32_t* tile_count; // accepted per tile
   uint64_t* tile_off;   // exclusive prefix of tile_count
+  uint64_t* run_sum;    // per-1024-tile sums, then their exclusive prefix
   uint64_t* n_acc;      // total accepted (device)
 };
 

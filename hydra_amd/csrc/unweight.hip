@@ -69,33 +69,80 @@ __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
   if (threadIdx.x == 0) a.tile_count[tile] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
 }
 
-// Exclusive scan of the tile counts by one block of 256 threads: each thread
-// sums a contiguous slice, the 256 slice sums are scanned in LDS, then each
-// thread writes its slice's prefixes.
-__global__ __launch_bounds__(kBlock) void k_uw_scan(const uint32_t* tile_count, uint64_t ntiles,
-                                                    uint64_t* tile_off, uint64_t* n_acc) {
-  __shared__ uint64_t sh[kBlock];
-  const uint64_t per = (ntiles + kBlock - 1) / kBlock;
-  const uint64_t b = (uint64_t)threadIdx.x * per;
-  const uint64_t e = (b + per < ntiles) ? b + per : ntiles;
-  uint64_t s = 0;
-  for (uint64_t i = b; i < e; ++i) s += tile_count[i];
-  sh[threadIdx.x] = s;
+// Exclusive scan of the tile counts in three fully parallel steps:
+// k_uw_scan_blocks sums each run of kScanRun counts, k_uw_scan_top scans those
+// sums in one block (and writes the accepted total), k_uw_scan_tiles scans
+// each run again starting from its offset.
+constexpr int kScanRun = 1024;  // tile counts per scan block (4 per thread)
+
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* sh, uint64_t* total) {
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  uint64_t x = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint64_t o = __shfl_up(x, off, kWave);
+    if (lane >= off) x += o;
+  }
+  if (lane == kWave - 1) sh[wave] = x;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t run = 0;
-    for (int t = 0; t < kBlock; ++t) {
-      const uint64_t v = sh[t];
-      sh[t] = run;
-      run += v;
-    }
-    *n_acc = run;
+  uint64_t base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    base += (w < wave) ? sh[w] : 0;
+    tot += sh[w];
   }
   __syncthreads();
-  uint64_t run = sh[threadIdx.x];
-  for (uint64_t i = b; i < e; ++i) {
-    tile_off[i] = run;
-    run += tile_count[i];
+  *total = tot;
+  return base + x - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_uw_scan_blocks(const uint32_t* tile_count,
+                                                           uint64_t ntiles, uint64_t* run_sum) {
+  __shared__ uint64_t sh[kBlock / kWave];
+  const uint64_t b = (uint64_t)blockIdx.x * kScanRun + (uint64_t)threadIdx.x * 4;
+  uint64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) s += (b + i < ntiles) ? tile_count[b + i] : 0u;
+  uint64_t tot;
+  (void)block_exclusive_scan(s, sh, &tot);
+  if (threadIdx.x == 0) run_sum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_uw_scan_top(uint64_t* run_sum, uint64_t nruns,
+                                                        uint64_t* n_acc) {
+  __shared__ uint64_t sh[kBlock / kWave];
+  uint64_t carry = 0;
+  for (uint64_t b = 0; b < nruns; b += kBlock) {
+    const uint64_t i = b + threadIdx.x;
+    const uint64_t v = (i < nruns) ? run_sum[i] : 0;
+    uint64_t tot;
+    const uint64_t ex = block_exclusive_scan(v, sh, &tot);
+    if (i < nruns) run_sum[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *n_acc = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_uw_scan_tiles(const uint32_t* tile_count,
+                                                          uint64_t ntiles,
+                                                          const uint64_t* run_off,
+                                                          uint64_t* tile_off) {
+  __shared__ uint64_t sh[kBlock / kWave];
+  const uint64_t b = (uint64_t)blockIdx.x * kScanRun + (uint64_t)threadIdx.x * 4;
+  uint32_t c[4];
+  uint64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    c[i] = (b + i < ntiles) ? tile_count[b + i] : 0u;
+    s += c[i];
+  }
+  uint64_t tot;
+  uint64_t run = run_off[blockIdx.x] + block_exclusive_scan(s, sh, &tot);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (b + i < ntiles) tile_off[b + i] = run;
+    run += c[i];
   }
 }
 
@@ -108,13 +155,19 @@ __global__ __launch_bounds__(kBlock) void k_uw_scatter(const UnweightArgs a) {
   const uint64_t ngroups = kUwTile / kWave;
   const uint64_t tile_base = a.tile_off[tile];
   const uint64_t nacc = *a.n_acc;
-  if (threadIdx.x == 0) {
-    uint64_t run = tile_base;
-    for (uint64_t g = 0; g < ngroups; ++g) {
-      const uint64_t gi = tile * ngroups + g;
-      goff[g] = run;
-      if (gi * kWave < a.n) run += (uint64_t)__popcll(a.flags[gi]);
+  // group offsets inside the tile: lanes 0..15 of wave 0 each load one flag
+  // word and an inclusive shuffle scan turns the popcounts into offsets
+  if (wave == 0) {
+    const uint64_t gi = tile * ngroups + (uint64_t)lane;
+    const uint32_t own =
+        (lane < (int)ngroups && gi * kWave < a.n) ? (uint32_t)__popcll(a.flags[gi]) : 0u;
+    uint32_t c = own;
+#pragma unroll
+    for (int off = 1; off < (int)ngroups; off <<= 1) {
+      const uint32_t o = __shfl_up(c, off, kWave);
+      if (lane >= off) c += o;
     }
+    if (lane < (int)ngroups) goff[lane] = tile_base + (c - own);
   }
   __syncthreads();
   const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
@@ -162,8 +215,12 @@ template <int N>
 static void uw_one(const UnweightArgs& a, hipStream_t s) {
   const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
   hipLaunchKernelGGL(k_uw_flags<N>, dim3((unsigned)ntiles), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL(k_uw_scan, dim3(1), dim3(kBlock), 0, s, a.tile_count, ntiles, a.tile_off,
-                     a.n_acc);
+  const uint64_t nruns = (ntiles + kScanRun - 1) / kScanRun;
+  hipLaunchKernelGGL(k_uw_scan_blocks, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
+                     ntiles, a.run_sum);
+  hipLaunchKernelGGL(k_uw_scan_top, dim3(1), dim3(kBlock), 0, s, a.run_sum, nruns, a.n_acc);
+  hipLaunchKernelGGL(k_uw_scan_tiles, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
+                     ntiles, a.run_sum, a.tile_off);
   if (a.cols_out || a.idx_out)
     hipLaunchKernelGGL(k_uw_scatter<N>, dim3((unsigned)ntiles), dim3(kBlock), 0, s, a);
 }
