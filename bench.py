@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--extras", type=int, default=1, help="also time the other workloads")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--only", default="",
-                    help="run one workload: c2, c3, c4, or copy (a plain 1 GiB device copy "
+                    help="run one workload: c2, c3, c4, c5 (serving), or copy (a plain 1 GiB device copy "
                          "that calibrates the profiler's byte counters)")
     return ap.parse_args()
 
@@ -232,6 +232,131 @@ def run_c4(torch, capi, chunk=1 << 24, nchunks=64):
     return res
 
 
+BW2 = [(0.892, 0.05, 1.0, 0.3, 1, 2), (1.43, 0.11, -0.4, 0.8, 0, 1)]
+
+
+def serving_mix(count, seed=5):
+    """Request mix: N uniform in 2..12, daughter masses U(0.1, 1) GeV, mother
+    mass U(1.05, 3) x their sum, events log-uniform in 1e3..1e7, 30% AverageOn
+    of a two-Breit-Wigner functor, half FP64 and half mixed precision."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        N = int(rng.integers(2, 13))
+        m = [float(x) for x in rng.uniform(0.1, 1.0, N)]
+        M = sum(m) * float(rng.uniform(1.05, 3.0))
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(1e7))))
+        avg = bool(rng.uniform() < 0.3)
+        mixed = bool(rng.uniform() < 0.5)
+        out.append(dict(N=N, M=M, m=m, n=n, avg=avg, mixed=mixed,
+                        seed=int(rng.integers(1, 2**31 - 1))))
+    return out
+
+
+def run_c5(torch, capi, nreq=6000, rate=3000.0, seed=5):
+    """Poisson arrivals into the continuous-batching server."""
+    import numpy as np
+    mix = serving_mix(nreq, seed)
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    reqs = []
+    for i, r in enumerate(mix):
+        dec = capi.make_decay(r["M"], r["m"])
+        terms = BW2 if r["N"] >= 3 else [(0.892, 0.05, 1.0, 0.3, 0, 1)]
+        f = capi.make_functor(capi.F_BW_SUM, terms)
+        kind = capi.REQ_AVERAGE if r["avg"] else capi.REQ_GENERATE
+        prec = capi.MIXED if r["mixed"] else capi.FP64
+        reqs.append(capi.make_request(kind, dec, r["seed"], r["n"], prec=prec,
+                                      out_f32=r["mixed"], functor=f, tag=i))
+    srv = capi.Server(64 << 30, n_streams=4)
+    try:
+        # warm the code objects of every (N, precision, kind) once
+        warm = 0
+        for r in reqs[:200]:
+            w = capi.Request.from_buffer_copy(r)
+            w.n = 4096
+            srv.submit(w)
+            warm += 1
+        got = 0
+        while got < warm:
+            for c in srv.poll(wait_ms=100):
+                got += 1
+                if c.kind == capi.REQ_GENERATE:
+                    srv.release(c.ticket)
+        info0 = srv.info()
+        lat = np.zeros(nreq)
+        queue = np.zeros(nreq)
+        t0 = time.perf_counter()
+        i = done = 0
+        first_submit = last_done = None
+        while done < nreq:
+            now = time.perf_counter() - t0
+            while i < nreq and arrivals[i] <= now:
+                srv.submit(reqs[i])
+                i += 1
+            for c in srv.poll(wait_ms=0 if i < nreq else 2):
+                k = int(c.user_tag)
+                lat[k] = c.done_s - c.submit_s
+                queue[k] = c.admit_s - c.submit_s
+                first_submit = c.submit_s if first_submit is None else min(first_submit, c.submit_s)
+                last_done = c.done_s if last_done is None else max(last_done, c.done_s)
+                if c.kind == capi.REQ_GENERATE:
+                    srv.release(c.ticket)
+                done += 1
+            if i < nreq:
+                gap = arrivals[i] - (time.perf_counter() - t0)
+                if gap > 2e-4:
+                    time.sleep(min(gap, 1e-3))
+        info = srv.info()
+    finally:
+        srv.close()
+    events = sum(r["n"] for r in mix)
+    wall = last_done - first_submit
+    lat_ms = lat * 1e3
+    return {
+        "requests": nreq, "offered_rate_per_s": rate, "events": events,
+        "events_per_s": events / wall, "wall_s": wall,
+        "latency_ms_p50": float(np.percentile(lat_ms, 50)),
+        "latency_ms_p99": float(np.percentile(lat_ms, 99)),
+        "latency_ms_max": float(lat_ms.max()),
+        "admission_wait_ms_p99": float(np.percentile(queue * 1e3, 99)),
+        "batches": info["batches"] - info0["batches"],
+        "graph_batches": info["graph_batches"] - info0["graph_batches"],
+        "launches": info["launches"] - info0["launches"],
+        "arena_peak_GB": info["arena_peak"] / 1e9,
+        "mix": "N 2..12, m U(0.1,1) GeV, M U(1.05,3)*sum(m), n log-uniform 1e3..1e7, "
+               "30% AverageOn (2 Breit-Wigners), 50% mixed precision (FP32 columns)",
+    }
+
+
+def cpu_baseline_mix(count=40, cap=200000, seed=5):
+    """The reference CPU path on the first requests of the serving mix (FP64)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    if not os.path.exists(orc.REF_LIB):
+        return None
+    R = orc.Reference(orc.REF_LIB)
+    cores = len(os.sched_getaffinity(0))
+    R.set_threads(cores)
+    ev = 0
+    sec = 0.0
+    for r in serving_mix(count, seed):
+        n = min(r["n"], cap)
+        if r["avg"]:
+            terms = BW2 if r["N"] >= 3 else [(0.892, 0.05, 1.0, 0.3, 0, 1)]
+            par = orc.functor_params(1, terms)
+            _, _, s = R.average_on(r["M"], r["m"], r["seed"], n, fid=1, fpar=par)
+        else:
+            _, _, s = R.generate(r["M"], r["m"], r["seed"], n, want_output=False)
+        ev += n
+        sec += s
+    return {"value": ev / sec, "unit": "events/s", "cores": cores, "kind": "reference",
+            "sample": f"first {count} requests of the serving mix, events capped at {cap}, "
+                      f"FP64, reference OMP backend, {cores} threads"}
+
+
 def cpu_baseline(n=1 << 25):
     """The reference CPU path (OMP) on a bounded sample of the headline step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -282,6 +407,11 @@ def main():
         extras["c4"] = run_c4(torch, capi)
         print(json.dumps(extras), flush=True)
         return
+    if args.only == "c5":
+        extras["c5"] = run_c5(torch, capi)
+        extras["c5_saturated"] = run_c5(torch, capi, rate=50000.0)
+        print(json.dumps(extras), flush=True)
+        return
     if args.only == "copy":
         # plain device copy of a known size: calibrates the HBM byte counters
         x = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
@@ -303,6 +433,9 @@ def main():
         print("# c3 done", file=sys.stderr, flush=True)
         extras["c4"] = run_c4(torch, capi)
         print("# c4 done", file=sys.stderr, flush=True)
+        extras["c5"] = run_c5(torch, capi)
+        extras["c5_saturated"] = run_c5(torch, capi, rate=50000.0)
+        print("# c5 done", file=sys.stderr, flush=True)
     extras["c2"] = c2
 
     if c2["generate_ms"] >= c2["unweight_ms"]:
@@ -321,6 +454,8 @@ def main():
     cpu = None
     if args.cpu_baseline:
         cpu = cpu_baseline()
+        if args.extras and "c5" in extras:
+            extras["c5"]["cpu_baseline"] = cpu_baseline_mix()
         print("# cpu baseline done", file=sys.stderr, flush=True)
 
     line = {

@@ -49,6 +49,32 @@ class Stats(ctypes.Structure):
     _fields_ = [("w", ctypes.c_double), ("mean", ctypes.c_double), ("m2", ctypes.c_double)]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("prec", ctypes.c_int32),
+                ("out_f32", ctypes.c_int32), ("want_weights", ctypes.c_int32),
+                ("decay", Decay), ("functor", Functor),
+                ("seed", ctypes.c_uint64), ("first", ctypes.c_uint64), ("n", ctypes.c_uint64),
+                ("user_tag", ctypes.c_uint64)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("user_tag", ctypes.c_uint64),
+                ("status", ctypes.c_int32), ("kind", ctypes.c_int32), ("stats", Stats),
+                ("cols", ctypes.c_void_p), ("weights", ctypes.c_void_p),
+                ("n", ctypes.c_uint64), ("submit_s", ctypes.c_double),
+                ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+class ServerInfo(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in
+                ("submitted", "admitted", "completed", "rejected", "batches", "graph_batches",
+                 "launches", "arena_bytes", "arena_in_use", "arena_peak")]
+
+
+REQ_GENERATE, REQ_AVERAGE = 0, 1
+ERR_SHUTDOWN = 7
+
+
 class HpspError(RuntimeError):
     def __init__(self, code, what=""):
         self.code = code
@@ -94,6 +120,12 @@ _SIGS = {
     "hpsp_stream_synchronize": (_I, [_P]),
     "hpsp_memcpy2d_to_host": (_I, [_P, _SZ, _P, _SZ, _SZ, _SZ]),
     "hpsp_memcpy2d_to_device": (_I, [_P, _SZ, _P, _SZ, _SZ, _SZ]),
+    "hpsp_server_create": (_I, [ctypes.POINTER(_P), _SZ, _I, _U64]),
+    "hpsp_server_submit": (_I, [_P, ctypes.POINTER(Request), ctypes.POINTER(_U64)]),
+    "hpsp_server_poll": (_I, [_P, ctypes.POINTER(Completion), _I, ctypes.POINTER(_I), _I]),
+    "hpsp_server_release": (_I, [_P, _U64]),
+    "hpsp_server_info_get": (_I, [_P, ctypes.POINTER(ServerInfo)]),
+    "hpsp_server_destroy": (_I, [_P]),
 }
 
 
@@ -161,3 +193,49 @@ def exported_symbols():
     text = open(HEADER_PATH).read()
     return sorted(set(re.findall(r"^\s*(?:int|void|size_t|const char\*)\s+(hpsp_\w+)\s*\(",
                                  text, re.M)))
+
+
+class Server:
+    """Continuous-batching server (hpsp_server_* in include/hpsp.h)."""
+
+    def __init__(self, hbm_bytes, n_streams=4, chunk_events=0):
+        self.h = ctypes.c_void_p()
+        check(lib().hpsp_server_create(ctypes.byref(self.h), int(hbm_bytes), n_streams,
+                                       int(chunk_events)), "hpsp_server_create")
+
+    def submit(self, req):
+        t = ctypes.c_uint64()
+        rc = lib().hpsp_server_submit(self.h, ctypes.byref(req), ctypes.byref(t))
+        check(rc, "hpsp_server_submit")
+        return t.value
+
+    def poll(self, max_n=256, wait_ms=10):
+        buf = (Completion * max_n)()
+        n = ctypes.c_int()
+        check(lib().hpsp_server_poll(self.h, buf, max_n, ctypes.byref(n), wait_ms),
+              "hpsp_server_poll")
+        return [buf[i] for i in range(n.value)]
+
+    def release(self, ticket):
+        check(lib().hpsp_server_release(self.h, int(ticket)), "hpsp_server_release")
+
+    def info(self):
+        i = ServerInfo()
+        check(lib().hpsp_server_info_get(self.h, ctypes.byref(i)), "hpsp_server_info_get")
+        return {k: getattr(i, k) for k, _ in ServerInfo._fields_}
+
+    def close(self):
+        if self.h:
+            check(lib().hpsp_server_destroy(self.h), "hpsp_server_destroy")
+            self.h = ctypes.c_void_p()
+
+
+def make_request(kind, decay, seed, n, first=0, prec=FP64, out_f32=False, want_weights=False,
+                 functor=None, tag=0):
+    r = Request()
+    r.kind, r.prec, r.out_f32, r.want_weights = kind, prec, int(out_f32), int(want_weights)
+    r.decay = decay
+    if functor is not None:
+        r.functor = functor
+    r.seed, r.first, r.n, r.user_tag = seed, first, n, tag
+    return r

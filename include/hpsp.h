@@ -40,7 +40,8 @@ typedef enum {
   HPSP_ERR_ENERGY = 3,           /* M - sum(m) <= 0: std::runtime_error in the reference */
   HPSP_ERR_WORKSPACE = 4,        /* workspace missing or too small */
   HPSP_ERR_HIP = 5,              /* a HIP runtime call failed */
-  HPSP_ERR_NO_DEVICE = 6
+  HPSP_ERR_NO_DEVICE = 6,
+  HPSP_ERR_SHUTDOWN = 7          /* server is shutting down */
 } hpsp_status;
 
 /* Counter-based engines (hydra/detail/PRNGTypedefs.h:59 and the headers in hydra/detail/random/). */
@@ -200,6 +201,62 @@ int hpsp_average_on(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed, u
 int hpsp_average_on_mothers(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed,
                             const double* mothers, uint64_t mld, uint64_t n, int prec,
                             void* ws, size_t ws_bytes, hpsp_stats* out_dev, void* stream);
+
+/* ---- serving: continuous batching of generation and integration requests --
+ * No counterpart in the reference (its calls are synchronous, one at a time).
+ * A server owns an HBM arena and a worker thread.  Requests are admitted when
+ * their footprint fits the arena, split into event chunks, and packed with
+ * the chunks of all other admitted requests into batches launched over
+ * several streams (captured into a HIP graph when a batch holds many
+ * launches).  A request's result is bit-identical to the direct call
+ * (hpsp_generate / hpsp_average_on with the same arguments), whatever else
+ * runs beside it.  Generation outputs stay in the arena until released.     */
+typedef struct hpsp_server hpsp_server;
+
+typedef enum { HPSP_REQ_GENERATE = 0, HPSP_REQ_AVERAGE = 1 } hpsp_request_kind;
+
+typedef struct {
+  int32_t kind;         /* hpsp_request_kind */
+  int32_t prec;         /* HPSP_FP64 or HPSP_MIXED */
+  int32_t out_f32;      /* GENERATE: FP32 columns */
+  int32_t want_weights; /* GENERATE: also the PhaseSpaceWeight column */
+  hpsp_decay decay;
+  hpsp_functor functor; /* AVERAGE */
+  uint64_t seed, first, n;
+  uint64_t user_tag;
+} hpsp_request;
+
+typedef struct {
+  uint64_t ticket, user_tag;
+  int32_t status; /* hpsp_status of the request */
+  int32_t kind;
+  hpsp_stats stats; /* AVERAGE */
+  void* cols;       /* GENERATE: device columns (ld = n), valid until released */
+  double* weights;  /* GENERATE: device PhaseSpaceWeight column or NULL */
+  uint64_t n;
+  double submit_s, admit_s, done_s; /* host monotonic clock, seconds */
+} hpsp_completion;
+
+typedef struct {
+  uint64_t submitted, admitted, completed, rejected;
+  uint64_t batches, graph_batches, launches;
+  uint64_t arena_bytes, arena_in_use, arena_peak;
+} hpsp_server_info;
+
+/* hbm_bytes: arena size; n_streams: 1..16 concurrent streams;
+ * chunk_events: events per chunk (rounded up to a multiple of 4096; 0 = default). */
+int hpsp_server_create(hpsp_server** server, size_t hbm_bytes, int n_streams,
+                       uint64_t chunk_events);
+/* Non-blocking.  HPSP_ERR_WORKSPACE if the request can never fit the arena. */
+int hpsp_server_submit(hpsp_server* server, const hpsp_request* req, uint64_t* ticket);
+/* Up to max completions; waits up to wait_ms for the first one. */
+int hpsp_server_poll(hpsp_server* server, hpsp_completion* out, int max, int* n_out,
+                     int wait_ms);
+/* Frees a completed GENERATE request's arena space. */
+int hpsp_server_release(hpsp_server* server, uint64_t ticket);
+int hpsp_server_info_get(hpsp_server* server, hpsp_server_info* info);
+/* Waits for in-flight work, then frees everything. */
+int hpsp_server_destroy(hpsp_server* server);
 
 /* ---- device memory helpers (for callers without a HIP toolchain) -------- */
 int hpsp_malloc(void** ptr, size_t bytes);
