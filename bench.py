@@ -101,43 +101,54 @@ def run_c2(torch, capi, n, steps, warmup):
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     nacc = torch.zeros(1, dtype=torch.int64, device=dev)
     sh = capi.stream_handle()
-    tg, tu = Timer(torch), Timer(torch)
+    tg, tu, tf = Timer(torch), Timer(torch), Timer(torch)
 
-    def step(timed):
-        if timed:
-            tg.start()
+    def gen():
         capi.check(lib.hpsp_generate(ctypes.byref(dec), SEED, 0, n, capi.FP64, capi.ptr(cols), n,
                                      0, None, capi.ptr(evw), sh), "generate")
-        if timed:
-            tg.stop()
-            tu.start()
+
+    def unweight(with_output=True):
         capi.check(lib.hpsp_unweight(ctypes.byref(dec), None, -1.0, UW_SEED, capi.ptr(cols), n,
-                                     n, capi.ptr(evw), capi.ptr(out), n, None, None,
-                                     capi.ptr(ws), ws_bytes, capi.ptr(nacc), sh), "unweight")
-        if timed:
-            tu.stop()
+                                     n, capi.ptr(evw), capi.ptr(out) if with_output else None,
+                                     n, None, None, capi.ptr(ws), ws_bytes, capi.ptr(nacc), sh),
+                   "unweight")
 
     for _ in range(warmup):
-        step(False)
+        gen()
+        unweight()
     torch.cuda.synchronize()
-    # kernel-level timing (separate pass, events synchronise)
+    # kernel-level timing (separate pass; events synchronise).  The flag and
+    # scan kernels alone run when no output is requested, so the scatter
+    # kernel's time is the full Unweight minus that.
     for _ in range(max(3, min(steps, 10))):
-        step(True)
+        tg.start()
+        gen()
+        tg.stop()
+        tu.start()
+        unweight()
+        tu.stop()
+        tf.start()
+        unweight(with_output=False)
+        tf.stop()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        step(False)
+        gen()
+        unweight()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1000.0 / steps
+    scatter_ms = tu.avg_ms - tf.avg_ms
     res = {"ms_per_step": ms, "events_per_s": n / (ms / 1000.0), "n_acc": int(nacc.item()),
-           "generate_ms": tg.avg_ms, "unweight_ms": tu.avg_ms}
+           "generate_ms": tg.avg_ms, "unweight_ms": tu.avg_ms, "flags_scan_ms": tf.avg_ms,
+           "scatter_ms": scatter_ms}
     gen_bytes = n * (32 * N + 8)
     uw_bytes = n * 64 * N
     res["generate_GBs"] = gen_bytes / (tg.avg_ms * 1e6)
     res["unweight_GBs"] = uw_bytes / (tu.avg_ms * 1e6)
+    res["scatter_GBs"] = uw_bytes / (scatter_ms * 1e6)
     res["generate_bytes_per_launch"] = gen_bytes
-    res["unweight_bytes_per_launch"] = uw_bytes
+    res["scatter_bytes_per_launch"] = uw_bytes
     del cols, evw, out, ws
     torch.cuda.empty_cache()
     return res
@@ -322,7 +333,6 @@ def run_c5(torch, capi, nreq=6000, rate=3000.0, seed=5):
         "latency_ms_max": float(lat_ms.max()),
         "admission_wait_ms_p99": float(np.percentile(queue * 1e3, 99)),
         "batches": info["batches"] - info0["batches"],
-        "graph_batches": info["graph_batches"] - info0["graph_batches"],
         "launches": info["launches"] - info0["launches"],
         "arena_peak_GB": info["arena_peak"] / 1e9,
         "mix": "N 2..12, m U(0.1,1) GeV, M U(1.05,3)*sum(m), n log-uniform 1e3..1e7, "
@@ -357,24 +367,36 @@ def cpu_baseline_mix(count=40, cap=200000, seed=5):
                       f"FP64, reference OMP backend, {cores} threads"}
 
 
-def cpu_baseline(n=1 << 25):
-    """The reference CPU path (OMP) on a bounded sample of the headline step."""
+def cpu_baseline(n=1 << 25, min_s=10.0, max_reps=16):
+    """The reference CPU path (OMP) on a bounded sample of the headline step:
+    the harness built with the reference's own flags (-O3, AVX2/FMA), 2^25
+    events per repetition, repeated until about ten seconds of wall time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     cores = len(os.sched_getaffinity(0))
-    if os.path.exists(orc.REF_LIB):
-        R = orc.Reference(orc.REF_LIB)
+    lib = orc.REF_NATIVE_LIB if os.path.exists(orc.REF_NATIVE_LIB) else orc.REF_LIB
+    if os.path.exists(lib):
+        R = orc.Reference(lib)
         R.set_threads(cores)
         t0 = time.perf_counter()
-        _, _, gsec = R.generate(M3, MASSES3, SEED, n, want_output=False)
-        nacc, _, _, usec = R.unweight(M3, MASSES3, SEED, n, UW_SEED, want_output=False)
+        gsec = usec = 0.0
+        reps = 0
+        while reps < max_reps and (reps == 0 or time.perf_counter() - t0 < min_s):
+            _, _, g = R.generate(M3, MASSES3, SEED, n, want_output=False)
+            _, _, _, u = R.unweight(M3, MASSES3, SEED, n, UW_SEED, want_output=False)
+            gsec += g
+            usec += u
+            reps += 1
         t1 = time.perf_counter()
-        step = gsec + usec
-        return {"value": n / step, "unit": "events/s", "cores": cores, "kind": "reference",
-                "sample": f"{n} events of the same 3-body FP64 decay: PhaseSpace<3>::Generate "
-                          f"into Decays + Decays::Unweight() on the reference's OMP backend "
-                          f"(built from its headers, -O2 -ffp-contract=off), {cores} threads; "
-                          f"generate {gsec:.3f} s, unweight {usec:.3f} s",
+        flags = ("--std=c++20 -march=x86-64-v3 -O3" if lib == orc.REF_NATIVE_LIB
+                 else "-O2 -ffp-contract=off")
+        return {"value": reps * n / (gsec + usec), "unit": "events/s", "cores": cores,
+                "kind": "reference",
+                "sample": f"{reps} x {n} events of the same 3-body FP64 decay: "
+                          f"PhaseSpace<3>::Generate into Decays + Decays::Unweight() on the "
+                          f"reference's OMP backend (built from its headers, {flags}), "
+                          f"{cores} threads; generate {gsec / reps:.3f} s, unweight "
+                          f"{usec / reps:.3f} s per repetition",
                 "wall_s": t1 - t0}
     O = orc.Oracle()
     import numpy as np
@@ -434,20 +456,19 @@ def main():
         extras["c4"] = run_c4(torch, capi)
         print("# c4 done", file=sys.stderr, flush=True)
         extras["c5"] = run_c5(torch, capi)
-        extras["c5_saturated"] = run_c5(torch, capi, rate=50000.0)
         print("# c5 done", file=sys.stderr, flush=True)
     extras["c2"] = c2
 
-    if c2["generate_ms"] >= c2["unweight_ms"]:
-        dom = ("generate (k_generate_f64<3,squares3>)", c2["generate_GBs"], c2["generate_bytes_per_launch"])
+    if c2["generate_ms"] >= c2["scatter_ms"]:
+        dom = ("k_generate_f64<3, 0, false>", c2["generate_GBs"], c2["generate_bytes_per_launch"],
+               "generate")
     else:
-        dom = ("unweight (k_uw_flags + k_uw_scan + k_uw_scatter)", c2["unweight_GBs"],
-               c2["unweight_bytes_per_launch"])
+        dom = ("k_uw_scatter<3>", c2["scatter_GBs"], c2["scatter_bytes_per_launch"], "scatter")
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         with open(tf) as fh:
-            traffic = json.load(fh).get(dom[0].split()[0])
+            traffic = json.load(fh).get(dom[3], {}).get("bytes_per_launch")
     roof = {"bound": "hbm", "achieved": dom[1], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": dom[1] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom[0],
             "algorithmic_bytes_per_launch": dom[2]}

@@ -67,8 +67,8 @@ class Completion(ctypes.Structure):
 
 class ServerInfo(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in
-                ("submitted", "admitted", "completed", "rejected", "batches", "graph_batches",
-                 "launches", "arena_bytes", "arena_in_use", "arena_peak")]
+                ("submitted", "admitted", "completed", "rejected", "batches", "launches",
+                 "arena_bytes", "arena_in_use", "arena_peak")]
 
 
 REQ_GENERATE, REQ_AVERAGE = 0, 1

@@ -13,11 +13,15 @@
 //   any mix.
 // * Each batch takes one chunk from every active request (round robin), so
 //   requests that arrive while others run join the next batch.  A batch's
-//   launches are spread over n_streams default-priority streams; batches
-//   with many launches are captured into a HIP graph and launched once.
-//   Batches run one after another on the device (each waits for the previous
-//   one's completion event); the worker prepares the next batch while the
-//   current one runs.
+//   launches are spread over n_streams default-priority streams.  Batches
+//   run one after another on the device (each waits for the previous one's
+//   completion event); the worker prepares the next batch while the current
+//   one runs.
+// * Launches are direct, not captured into HIP graphs: a batch's launch list
+//   changes from one batch to the next, so a graph would be instantiated per
+//   batch, which costs more host time than the launches it replaces for
+//   requests of 10^3..10^7 events (a launch costs a few microseconds, a
+//   request's kernel tens of microseconds to milliseconds).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,7 +48,6 @@ double now_s() {
 }
 
 constexpr size_t kArenaAlign = 256;
-constexpr int kGraphMinLaunches = 8;
 constexpr int kMaxInflight = 2;
 constexpr int kMaxLaunchesPerBatch = 256;
 constexpr uint64_t kDefaultChunk = 1ULL << 22;
@@ -138,8 +141,6 @@ struct Item {
 struct Batch {
   std::vector<Item> items;
   hipEvent_t done = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
 };
 
 size_t footprint(const hpsp_request& r) {
@@ -287,49 +288,24 @@ struct hpsp_server {
     return e;
   }
 
-  // Returns the number of kernel launches; sets *graph when captured.
-  uint64_t launch_batch(Batch& b, bool* graph) {
+  // Launches a batch on the server's streams; returns the number of kernel
+  // launches.
+  uint64_t launch_batch(Batch& b) {
     uint64_t nl = 0;
     (void)hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
     const int K = (int)streams.size();
     // batches run in order: this one starts when the previous one is done
-    (void)hipStreamWaitEvent(main, last_done, 0);
-    const bool use_graph = (int)b.items.size() >= kGraphMinLaunches;
-    int err = 0;
-    if (use_graph) {
-      err = (int)hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal);
-      if (!err) {
-        hipEvent_t fork = joins[K];
-        (void)hipEventRecord(fork, main);
-        for (int k = 0; k < K; ++k) (void)hipStreamWaitEvent(streams[k], fork, 0);
-        for (const Item& it : b.items) {
-          const int e = launch_item(it, streams[it.q->stream], &nl);
-          if (e) it.q->status = HPSP_ERR_HIP;
-        }
-        for (int k = 0; k < K; ++k) {
-          (void)hipEventRecord(joins[k], streams[k]);
-          (void)hipStreamWaitEvent(main, joins[k], 0);
-        }
-        err = (int)hipStreamEndCapture(main, &b.graph);
-        if (!err) err = (int)hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0);
-        if (!err) err = (int)hipGraphLaunch(b.exec, main);
-      }
-      if (err)
-        for (const Item& it : b.items) it.q->status = HPSP_ERR_HIP;
-    } else {
-      for (int k = 0; k < K; ++k) (void)hipStreamWaitEvent(streams[k], last_done, 0);
-      for (const Item& it : b.items) {
-        const int e = launch_item(it, streams[it.q->stream], &nl);
-        if (e) it.q->status = HPSP_ERR_HIP;
-      }
-      for (int k = 0; k < K; ++k) {
-        (void)hipEventRecord(joins[k], streams[k]);
-        (void)hipStreamWaitEvent(main, joins[k], 0);
-      }
+    for (int k = 0; k < K; ++k) (void)hipStreamWaitEvent(streams[k], last_done, 0);
+    for (const Item& it : b.items) {
+      const int e = launch_item(it, streams[it.q->stream], &nl);
+      if (e) it.q->status = HPSP_ERR_HIP;
+    }
+    for (int k = 0; k < K; ++k) {
+      (void)hipEventRecord(joins[k], streams[k]);
+      (void)hipStreamWaitEvent(main, joins[k], 0);
     }
     (void)hipEventRecord(b.done, main);
     (void)hipEventRecord(last_done, main);
-    *graph = use_graph;
     return nl;
   }
 
@@ -364,8 +340,6 @@ struct hpsp_server {
       done.push_back(c);
       ++info.completed;
     }
-    if (b.exec) (void)hipGraphExecDestroy(b.exec);
-    if (b.graph) (void)hipGraphDestroy(b.graph);
     (void)hipEventDestroy(b.done);
     if (!active.empty()) rr %= active.size();
     else rr = 0;
@@ -385,11 +359,9 @@ struct hpsp_server {
         Batch b = build_batch_locked();
         if (!b.items.empty()) {
           lk.unlock();
-          bool graph = false;
-          const uint64_t nl = launch_batch(b, &graph);
+          const uint64_t nl = launch_batch(b);
           lk.lock();
           ++info.batches;
-          info.graph_batches += graph ? 1 : 0;
           info.launches += nl;
           inflight.push_back(std::move(b));
           continue;
@@ -436,7 +408,7 @@ int hpsp_server_create(hpsp_server** out, size_t hbm_bytes, int n_streams,
     if (hipStreamCreateWithFlags(&s->streams[k], hipStreamNonBlocking) != hipSuccess)
       return HPSP_ERR_HIP;
   if (hipStreamCreateWithFlags(&s->main, hipStreamNonBlocking) != hipSuccess) return HPSP_ERR_HIP;
-  s->joins.resize(n_streams + 1);
+  s->joins.resize(n_streams);
   for (auto& e : s->joins)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return HPSP_ERR_HIP;
   if (hipEventCreateWithFlags(&s->last_done, hipEventDisableTiming) != hipSuccess)

@@ -207,8 +207,7 @@ int hpsp_average_on_mothers(const hpsp_decay* d, const hpsp_functor* f, uint64_t
  * A server owns an HBM arena and a worker thread.  Requests are admitted when
  * their footprint fits the arena, split into event chunks, and packed with
  * the chunks of all other admitted requests into batches launched over
- * several streams (captured into a HIP graph when a batch holds many
- * launches).  A request's result is bit-identical to the direct call
+ * several streams.  A request's result is bit-identical to the direct call
  * (hpsp_generate / hpsp_average_on with the same arguments), whatever else
  * runs beside it.  Generation outputs stay in the arena until released.     */
 typedef struct hpsp_server hpsp_server;
@@ -239,7 +238,7 @@ typedef struct {
 
 typedef struct {
   uint64_t submitted, admitted, completed, rejected;
-  uint64_t batches, graph_batches, launches;
+  uint64_t batches, launches;
   uint64_t arena_bytes, arena_in_use, arena_peak;
 } hpsp_server_info;
 

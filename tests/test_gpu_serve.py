@@ -109,7 +109,8 @@ def test_mix_equals_direct_and_alone():
     finally:
         srv.close()
     assert info["completed"] == len(reqs) and info["arena_in_use"] == 0
-    assert info["graph_batches"] > 0
+    # chunks of different requests shared batches
+    assert info["launches"] > info["batches"]
     for i, d in enumerate(reqs):
         assert same(got[i], direct(d), d), i
     # each request alone on a fresh server gives the same bits
