@@ -48,6 +48,19 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
   return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
 }
 
+// Global-address-space views of a pointer: loads and stores through them are
+// global_* instructions (not flat), so they do not wait on LDS counters.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> glob(T* p) {
+  return (gptr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ gptr<T> glob_u(T* p) {
+  return (gptr<T>)uniform_ptr(p);
+}
+
 // ---------------------------------------------------------------------------
 // Engines
 

@@ -11,6 +11,7 @@ namespace hpsp {
 
 constexpr int kAvgChunk = 4096;  // events per AverageOn reduction chunk
 constexpr int kUwTile = 1024;    // events per Unweight tile (256 threads x 4)
+constexpr int kXcds = 8;         // MI355X: workgroups are dealt to 8 XCDs in turn
 
 struct GenArgs {
   DecayK d;

@@ -14,7 +14,8 @@
 //                      accepted -> off_acc, rejected -> n_acc + (k - off_acc);
 //                      all 4N columns (and optional index/weight) move once.
 // Event columns are read twice at most (flags need them only when no weight
-// column is supplied) and written once.
+// column is supplied) and written once.  The scatter keeps several groups'
+// loads in flight before storing and maps tiles to XCDs in contiguous ranges.
 #include <array>
 #include <utility>
 
@@ -41,6 +42,18 @@ __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
   const int wave = threadIdx.x / kWave;
   const uint64_t tile = blockIdx.x;
   const double maxv = a.maxv_dev ? *a.maxv_dev : a.maxv;
+  const bool w_only = a.w_in && !a.use_f;
+  // with a weight column, the wave's four loads are issued before any flag
+  // word is stored
+  double wcol[kItems];
+  if (w_only) {
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const uint64_t g = tile * (kUwTile / kWave) + (uint64_t)j * (kBlock / kWave) + wave;
+      const uint64_t k = g * kWave + (uint64_t)lane;
+      wcol[j] = (k < a.n) ? *(glob(a.w_in) + k) : 0.0;
+    }
+  }
   uint32_t c = 0;
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
@@ -49,8 +62,8 @@ __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
     bool acc = false;
     if (k < a.n) {
       double w;
-      if (a.w_in && !a.use_f) {
-        w = a.w_in[k];
+      if (w_only) {
+        w = wcol[j];
       } else {
         double p[N][4];
         load_event_uw<N>(a.cols_in, a.ld_in, k, p);
@@ -61,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
       acc = (w / maxv) > u;
     }
     const uint64_t ball = __ballot(acc);
-    if (lane == 0) a.flags[g] = ball;
+    if (lane == 0) *(glob(a.flags) + g) = ball;
     c += (uint32_t)__popcll(ball);
   }
   if (lane == 0) cnt[wave] = c;
@@ -146,12 +159,18 @@ __global__ __launch_bounds__(kBlock) void k_uw_scan_tiles(const uint32_t* tile_c
   }
 }
 
+// Tiles are mapped so that each XCD takes a contiguous range of them:
+// neighbouring tiles write neighbouring pieces of the accepted and rejected
+// runs, and the cache lines they share then meet in one L2.
 template <int N>
 __global__ __launch_bounds__(kBlock) void k_uw_scatter(const UnweightArgs a) {
   __shared__ uint64_t goff[kUwTile / kWave];
+  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
+  const uint64_t per_xcd = (ntiles + kXcds - 1) / kXcds;
+  const uint64_t tile = (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (tile >= ntiles) return;  // whole block
   const int lane = __lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const uint64_t tile = blockIdx.x;
   const uint64_t ngroups = kUwTile / kWave;
   const uint64_t tile_base = a.tile_off[tile];
   const uint64_t nacc = *a.n_acc;
@@ -171,38 +190,55 @@ __global__ __launch_bounds__(kBlock) void k_uw_scatter(const UnweightArgs a) {
   }
   __syncthreads();
   const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+  // G groups of 64 events are loaded (every column) before any is stored, so
+  // a wave keeps G*4N loads in flight; G is bounded by a 64-value budget.
+  constexpr int C = 4 * N;
+  constexpr int G = (4 * C <= 64) ? 4 : ((2 * C <= 64) ? 2 : 1);
+  const bool move_cols = a.cols_out != nullptr;
+  const bool move_w = a.w_out != nullptr;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const uint64_t gl = (uint64_t)j * (kBlock / kWave) + wave;
-    const uint64_t g = tile * ngroups + gl;
-    const uint64_t k0 = g * kWave;
-    if (k0 >= a.n) continue;  // wave-uniform
-    const uint64_t k = k0 + (uint64_t)lane;
-    const bool live = k < a.n;
-    const uint64_t fw = a.flags[g];
-    // accepted lanes go to goff[gl] + (accepted lanes below),
-    // rejected lanes to nacc + (k0 - goff[gl]) + (rejected lanes below):
-    // two wave-uniform bases plus a lane offset below 64
-    const uint32_t acc_below = (uint32_t)__popcll(fw & below);
-    const bool acc = (fw >> lane) & 1ULL;
-    const uint64_t base_a = goff[gl];
-    const uint64_t base_r = nacc + k0 - goff[gl];
-    const uint32_t off = acc ? acc_below : (uint32_t)lane - acc_below;
+  for (int j0 = 0; j0 < kItems; j0 += G) {
+    double v[G][C];
+    double wv[G];
+    uint64_t dst[G];
+    uint64_t src[G];
+    bool live[G];
 #pragma unroll
-    for (int c = 0; c < 4 * N; ++c) {
-      const double v = live ? __builtin_nontemporal_load(uniform_ptr(a.cols_in + (uint64_t)c * a.ld_in + k0) + lane) : 0.0;
-      double* col = a.cols_out + (uint64_t)c * a.ld_out;
-      if (live) {
-        if (acc)
-          uniform_ptr(col + base_a)[off] = v;
-        else
-          uniform_ptr(col + base_r)[off] = v;
+    for (int h = 0; h < G; ++h) {
+      const uint64_t gl = (uint64_t)(j0 + h) * (kBlock / kWave) + wave;
+      const uint64_t g = tile * ngroups + gl;
+      const uint64_t k0 = g * kWave;
+      src[h] = k0 + (uint64_t)lane;
+      live[h] = src[h] < a.n;
+      dst[h] = 0;
+      if (k0 < a.n) {  // wave-uniform
+        // accepted lanes go to goff[gl] + (accepted lanes below), rejected
+        // lanes to nacc + (k0 - goff[gl]) + (rejected lanes below)
+        const uint64_t fw = a.flags[g];
+        const uint32_t acc_below = (uint32_t)__popcll(fw & below);
+        const bool acc = (fw >> lane) & 1ULL;
+        dst[h] = acc ? goff[gl] + acc_below : nacc + k0 - goff[gl] + ((uint32_t)lane - acc_below);
+        if (live[h]) {
+          if (move_cols) {
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+              v[h][c] = *(glob_u(a.cols_in + (uint64_t)c * a.ld_in + k0) + lane);
+          }
+          if (move_w) wv[h] = *(glob(a.w_in) + src[h]);
+        }
       }
     }
-    if (live) {
-      const uint64_t dst = (acc ? base_a : base_r) + off;
-      if (a.idx_out) a.idx_out[dst] = k;
-      if (a.w_out) a.w_out[dst] = a.w_in[k];
+#pragma unroll
+    for (int h = 0; h < G; ++h) {
+      if (live[h]) {
+        if (move_cols) {
+#pragma unroll
+          for (int c = 0; c < C; ++c)
+            *(glob_u(a.cols_out + (uint64_t)c * a.ld_out) + dst[h]) = v[h][c];
+        }
+        if (a.idx_out) *(glob(a.idx_out) + dst[h]) = src[h];
+        if (move_w) *(glob(a.w_out) + dst[h]) = wv[h];
+      }
     }
   }
 }
@@ -221,8 +257,10 @@ static void uw_one(const UnweightArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_uw_scan_top, dim3(1), dim3(kBlock), 0, s, a.run_sum, nruns, a.n_acc);
   hipLaunchKernelGGL(k_uw_scan_tiles, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
                      ntiles, a.run_sum, a.tile_off);
-  if (a.cols_out || a.idx_out)
-    hipLaunchKernelGGL(k_uw_scatter<N>, dim3((unsigned)ntiles), dim3(kBlock), 0, s, a);
+  if (a.cols_out || a.idx_out) {
+    const uint64_t grid = (ntiles + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL(k_uw_scatter<N>, dim3((unsigned)grid), dim3(kBlock), 0, s, a);
+  }
 }
 
 template <int... I>
