@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--extras", type=int, default=1, help="also time the other workloads")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--only", default="",
-                    help="run one workload: c2, c3, c4, c5 (serving), or copy (a plain 1 GiB device copy "
+                    help="run one workload: c2, c3, c4, evaluate, c5 (serving), or copy (a plain 1 GiB device copy "
                          "that calibrates the profiler's byte counters)")
     return ap.parse_args()
 
@@ -243,6 +243,57 @@ def run_c4(torch, capi, chunk=1 << 24, nchunks=64):
     return res
 
 
+def run_evaluate(torch, capi, n=1 << 27, reps=5):
+    """PhaseSpace::Evaluate as examples/phase_space/phsp_evaluating_functor.inl
+    runs it: B0 -> J/psi K pi with the weight, M13^2, M23^2 and the helicity
+    cosine per event (4 FP64 columns, no momenta stored); then over 2^26
+    moving mothers (EvalMothers: one engine per event)."""
+    import numpy as np
+    lib = capi.lib()
+    dec = capi.make_decay(M3, MASSES3)
+    fs = capi.functor_array([capi.make_functor(capi.F_MASS2, (0, 2)),
+                             capi.make_functor(capi.F_MASS2, (1, 2)),
+                             capi.make_functor(capi.F_COS_HELICITY, (1, 2))])
+    out = torch.empty((4, n), dtype=torch.float64, device="cuda:0")
+    sh = capi.stream_handle()
+
+    def ev():
+        capi.check(lib.hpsp_evaluate(ctypes.byref(dec), fs, 3, SEED, 0, n, capi.ptr(out), n, sh),
+                   "evaluate")
+    ev()
+    t = Timer(torch)
+    for _ in range(reps):
+        t.start()
+        ev()
+        t.stop()
+    res = {"events": n, "functors": "M13^2, M23^2, cos(theta_K*)", "ms": t.avg_ms,
+           "events_per_s": n / (t.avg_ms / 1e3), "bytes_per_event": 32,
+           "GBs": n * 32 / (t.avg_ms * 1e6)}
+    nm = n // 2
+    rng = np.random.default_rng(11)
+    mt = np.zeros((4, nm))
+    mt[1:] = rng.normal(0, 2.0, (3, nm))
+    mt[0] = np.sqrt(M3 * M3 + (mt[1:] ** 2).sum(axis=0))
+    mth = torch.as_tensor(mt, device="cuda:0")
+    del mt
+    outm = torch.empty((2, nm), dtype=torch.float64, device="cuda:0")
+    fm = capi.functor_array([capi.make_functor(capi.F_MASS2, (1, 3))])
+
+    def evm():
+        capi.check(lib.hpsp_evaluate_mothers(ctypes.byref(dec), fm, 1, SEED, capi.ptr(mth), nm, 0,
+                                             nm, capi.ptr(outm), nm, sh), "evaluate_mothers")
+    evm()
+    tm = Timer(torch)
+    for _ in range(reps):
+        tm.start()
+        evm()
+        tm.stop()
+    res.update({"mothers_events": nm, "mothers_ms": tm.avg_ms,
+                "mothers_events_per_s": nm / (tm.avg_ms / 1e3),
+                "mothers_bytes_per_event": 48, "mothers_GBs": nm * 48 / (tm.avg_ms * 1e6)})
+    return res
+
+
 BW2 = [(0.892, 0.05, 1.0, 0.3, 1, 2), (1.43, 0.11, -0.4, 0.8, 0, 1)]
 
 
@@ -367,6 +418,25 @@ def cpu_baseline_mix(count=40, cap=200000, seed=5):
                       f"FP64, reference OMP backend, {cores} threads"}
 
 
+def cpu_baseline_evaluate(n=1 << 23):
+    """The reference's PhaseSpace::Evaluate (OMP, all host threads) on the
+    Evaluate workload's first 2^23 events."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    lib = orc.REF_NATIVE_LIB if os.path.exists(orc.REF_NATIVE_LIB) else orc.REF_LIB
+    if not os.path.exists(lib):
+        return None
+    R = orc.Reference(lib)
+    cores = len(os.sched_getaffinity(0))
+    R.set_threads(cores)
+    F = [(3, (0, 2)), (3, (1, 2))]
+    _, sec = R.evaluate(M3, MASSES3, SEED, n, F)
+    return {"value": n / sec, "unit": "events/s", "cores": cores, "kind": "reference",
+            "sample": f"{n} events, Evaluate(mother, result, M13^2, M23^2) on the reference's OMP "
+                      f"backend, {cores} threads (the harness binds two functors; the GPU line "
+                      f"evaluates three)"}
+
+
 def cpu_baseline(n=1 << 25, min_s=10.0, max_reps=16):
     """The reference CPU path (OMP) on a bounded sample of the headline step:
     the harness built with the reference's own flags (-O3, AVX2/FMA), 2^25
@@ -429,6 +499,10 @@ def main():
         extras["c4"] = run_c4(torch, capi)
         print(json.dumps(extras), flush=True)
         return
+    if args.only == "evaluate":
+        extras["evaluate"] = run_evaluate(torch, capi)
+        print(json.dumps(extras), flush=True)
+        return
     if args.only == "c5":
         extras["c5"] = run_c5(torch, capi)
         extras["c5_saturated"] = run_c5(torch, capi, rate=50000.0)
@@ -455,6 +529,8 @@ def main():
         print("# c3 done", file=sys.stderr, flush=True)
         extras["c4"] = run_c4(torch, capi)
         print("# c4 done", file=sys.stderr, flush=True)
+        extras["evaluate"] = run_evaluate(torch, capi)
+        print("# evaluate done", file=sys.stderr, flush=True)
         extras["c5"] = run_c5(torch, capi)
         print("# c5 done", file=sys.stderr, flush=True)
     extras["c2"] = c2
@@ -477,6 +553,8 @@ def main():
         cpu = cpu_baseline()
         if args.extras and "c5" in extras:
             extras["c5"]["cpu_baseline"] = cpu_baseline_mix()
+        if args.extras and "evaluate" in extras:
+            extras["evaluate"]["cpu_baseline"] = cpu_baseline_evaluate()
         print("# cpu baseline done", file=sys.stderr, flush=True)
 
     line = {

@@ -27,7 +27,8 @@ ERR_HIP = 5
 ENGINES = {"squares3": 0, "squares4": 1, "philox": 2, "philox_long": 3,
            "threefry": 4, "threefry_long": 5}
 FP64, MIXED = 0, 1
-F_ONE, F_BW_SUM, F_MASS2_01 = 0, 1, 2
+F_ONE, F_BW_SUM, F_MASS2_01, F_MASS2, F_COS_HELICITY = 0, 1, 2, 3, 4
+MAX_EVAL_FUNCTORS = 8
 
 
 class Decay(ctypes.Structure):
@@ -112,6 +113,10 @@ _SIGS = {
                              _I, _P, _SZ, _P, _P]),
     "hpsp_average_on_mothers": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _U64, _P,
                                      _U64, _U64, _I, _P, _SZ, _P, _P]),
+    "hpsp_evaluate": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _I, _U64, _U64, _U64,
+                           _P, _U64, _P]),
+    "hpsp_evaluate_mothers": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _I, _U64, _P,
+                                   _U64, _U64, _U64, _P, _U64, _P]),
     "hpsp_malloc": (_I, [ctypes.POINTER(_P), _SZ]),
     "hpsp_free": (_I, [_P]),
     "hpsp_memcpy_to_device": (_I, [_P, _P, _SZ]),
@@ -178,13 +183,27 @@ def make_decay(mother_mass, masses, engine="squares3", mother4=None):
 
 
 def make_functor(kind=F_ONE, terms=()):
+    """F_BW_SUM: terms = [(M, G, cr, ci, a, b), ...]; F_MASS2 and
+    F_COS_HELICITY: terms = (a, b) (for the helicity cosine, a is the particle
+    whose angle is measured and b its partner)."""
     f = Functor()
     f.kind = kind
+    if kind in (F_MASS2, F_COS_HELICITY):
+        f.a[0], f.b[0] = int(terms[0]), int(terms[1])
+        return f
     f.nterms = len(terms)
     for k, (M, G, cr, ci, a, b) in enumerate(terms):
         f.m[k], f.g[k], f.cr[k], f.ci[k] = M, G, cr, ci
         f.a[k], f.b[k] = int(a), int(b)
     return f
+
+
+def functor_array(functors):
+    """A ctypes array of Functor from make_functor results."""
+    arr = (Functor * len(functors))()
+    for j, f in enumerate(functors):
+        arr[j] = f
+    return arr
 
 
 def exported_symbols():

@@ -162,6 +162,52 @@ int hpsp_generate_mothers(const hpsp_decay* d, uint64_t seed, uint64_t first, ui
                          stream);
 }
 
+namespace {
+
+int evaluate_common(const hpsp_decay* d, const hpsp_functor* fs, int nf, uint64_t seed,
+                    const double* mothers, uint64_t mld, uint64_t first, uint64_t n,
+                    double* out, uint64_t ld, void* stream) {
+  int rc = check_decay(d);
+  if (rc) return rc;
+  if (nf < 1 || nf > HPSP_MAX_EVAL_FUNCTORS || !fs) return HPSP_ERR_INVALID_ARGUMENT;
+  if (n == 0) return HPSP_OK;
+  if (!out || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
+  if (mothers && mld < n) return HPSP_ERR_INVALID_ARGUMENT;
+  EvalArgs a;
+  memset(&a, 0, sizeof(a));
+  // Evaluate(mother): the PhaseSpace seed keys the engine as in Generate;
+  // Evaluate(mothers): it is paired with each event index on the device
+  a.d = make_decayk(d, seed);
+  const int nvec = d->n + (mothers ? 1 : 0);
+  for (int j = 0; j < nf; ++j) {
+    rc = make_functork(&fs[j], nvec, &a.f[j], mothers ? 1 : 0);
+    if (rc) return rc;
+  }
+  a.nf = nf;
+  a.seed = seed;
+  a.first = first;
+  a.n = n;
+  a.mothers = mothers;
+  a.mld = mld;
+  a.out = out;
+  a.ld = ld;
+  return hip_status(launch_evaluate_f64(d->n, d->engine, mothers != nullptr, a, as_stream(stream)));
+}
+
+}  // namespace
+
+int hpsp_evaluate(const hpsp_decay* d, const hpsp_functor* fs, int nf, uint64_t seed,
+                  uint64_t first, uint64_t n, double* out, uint64_t ld, void* stream) {
+  return evaluate_common(d, fs, nf, seed, nullptr, 0, first, n, out, ld, stream);
+}
+
+int hpsp_evaluate_mothers(const hpsp_decay* d, const hpsp_functor* fs, int nf, uint64_t seed,
+                          const double* mothers, uint64_t mld, uint64_t first, uint64_t n,
+                          double* out, uint64_t ld, void* stream) {
+  if (n > 0 && !mothers) return HPSP_ERR_INVALID_ARGUMENT;
+  return evaluate_common(d, fs, nf, seed, mothers, mld, first, n, out, ld, stream);
+}
+
 int hpsp_event_weight(const hpsp_decay* d, const hpsp_functor* f, const double* cols,
                       uint64_t ld, uint64_t n, double* w_out, void* stream) {
   int rc = check_decay(d);
@@ -276,7 +322,7 @@ static int average_common(const hpsp_decay* d, const hpsp_functor* f, uint64_t s
   AvgArgs a;
   memset(&a, 0, sizeof(a));
   a.d = make_decayk(d, seed);
-  rc = make_functork(f, d->n + (mothers ? 1 : 0), &a.f);
+  rc = make_functork(f, d->n + (mothers ? 1 : 0), &a.f, mothers ? 1 : 0);
   if (rc) return rc;
   a.first = first;
   a.n = n;

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "hpsp.h"
 
 namespace hpsp {
@@ -236,6 +238,35 @@ __device__ __forceinline__ uint32_t squares_word(uint64_t key, uint64_t ctr) {
     return squares4(key, ctr);
 }
 
+// SplitMix64 step (splitmix.h:54-61).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// Key words an engine's constructor derives from a seed: squares3/4 one
+// splitmix64 draw (squares3.h:81-84), Random123 one draw per key word
+// (EngineR123.h:97-112).
+template <int ENGINE>
+__device__ __forceinline__ void engine_keys_dev(uint64_t seed, uint64_t (&key)[4]) {
+  constexpr int nkey = (ENGINE == HPSP_PHILOX_LONG || ENGINE == HPSP_THREEFRY) ? 2
+                       : (ENGINE == HPSP_THREEFRY_LONG)                       ? 4
+                                                                              : 1;
+  uint64_t s = seed;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) key[i] = (i < nkey) ? splitmix64(s) : 0;
+}
+
+// Szudzik pairing of (event index, seed) that seeds EvalMothers' per-event
+// engines (EvalMothers.h:131-147); the index passes through an int there.
+__device__ __forceinline__ uint64_t pair_hash(uint64_t evt, uint64_t seed) {
+  const uint64_t a = (uint64_t)(int64_t)(int32_t)(uint32_t)evt;
+  const uint64_t A = 2 * a, B = 2 * seed;
+  return (A >= B ? A * A + A + B : A + B * B) / 2;
+}
+
 // Word j of the stream after GRND eng(seed); eng.discard(d):
 // squares: counter d + j; Random123: block counter d + 1 + j / arity
 // (pre-incremented), word j % arity.  Words of one event are produced
@@ -460,6 +491,32 @@ __device__ __forceinline__ double event_weight_f64(const double (&p)[N][4], cons
 // divisions correctly rounded (Unweight(functor) and PhaseSpaceReweight must
 // reproduce the reference's bits); otherwise the Breit-Wigner divisions use
 // a reciprocal (AverageOn, compared within a tolerance).
+// Four-vector p[i] for a runtime (wave-uniform) index i, selected through
+// the unrolled register array with bit masks: a chain of value selects is
+// folded by the optimizer into an indexed load, which moves the whole array
+// to scratch memory.
+template <typename Real, int P>
+__device__ __forceinline__ void pick4(const Real (&p)[P][4], int i, Real (&v)[4]) {
+  using U = std::conditional_t<sizeof(Real) == 8, uint64_t, uint32_t>;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    U x = __builtin_bit_cast(U, p[0][c]);
+#pragma unroll
+    for (int j = 1; j < P; ++j) {
+      const U m = (U)0 - (U)(i == j);
+      x = (x & ~m) | (__builtin_bit_cast(U, p[j][c]) & m);
+    }
+    v[c] = __builtin_bit_cast(Real, x);
+  }
+}
+
+// Vector4R product and mass2 (Vector4R.h:171-176, Vector4R.inl:56-60).
+template <typename Real>
+__device__ __forceinline__ Real dot4(const Real (&a)[4], const Real (&b)[4]) {
+#pragma clang fp contract(off)
+  return a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+}
+
 template <typename Real, bool Exact = true, int P>
 __device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p)[P][4]) {
 #pragma clang fp contract(off)
@@ -469,21 +526,56 @@ __device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p
     const Real s2 = p[0][2] + p[1][2], s3 = p[0][3] + p[1][3];
     return (double)(s0 * s0 - s1 * s1 - s2 * s2 - s3 * s3);
   }
+  if (f.kind == HPSP_F_MASS2) {
+    Real u[4], v[4], s[4];
+    pick4(p, f.a[0], u);
+    pick4(p, f.b[0], v);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s[c] = u[c] + v[c];
+    return (double)dot4(s, s);
+  }
+  if (f.kind == HPSP_F_COS_HELICITY) {
+    // P = p[first] + ... + p[first + count - 1] left to right (first, count
+    // in a[1], b[1]); q = d + p[b0]; d = p[a0]
+    const int first = f.a[1], last = f.a[1] + f.b[1];
+    Real Pv[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const Real sum = Pv[c] + p[i][c];
+        Pv[c] = (i == first) ? p[i][c] : ((i > first && i < last) ? sum : Pv[c]);
+      }
+    }
+    Real d[4], e[4], q[4];
+    pick4(p, f.a[0], d);
+    pick4(p, f.b[0], e);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) q[c] = d[c] + e[c];
+    const Real pd = dot4(Pv, d);
+    const Real pq = dot4(Pv, q);
+    const Real qd = dot4(q, d);
+    const Real mp2 = dot4(Pv, Pv);
+    const Real mq2 = dot4(q, q);
+    const Real md2 = dot4(d, d);
+    const Real num = pd * mq2 - pq * qd;
+    const Real den = (pq * pq - mq2 * mp2) * (qd * qd - mq2 * md2);
+    if constexpr (Exact) {
+      return (double)(num / sqrt_cr(den));
+    } else {
+      if constexpr (sizeof(Real) == 4)
+        return (double)(num * __builtin_amdgcn_rsqf(den));
+      else
+        return num * rcp_fast(sqrt_fast(den));
+    }
+  }
   Real Ar = 0, Ai = 0;
   for (int k = 0; k < f.nterms; ++k) {
-    const int a = f.a[k], b = f.b[k];
-    Real v[4];
+    Real u[4], w[4], v[4];
+    pick4(p, f.a[k], u);
+    pick4(p, f.b[k], w);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      // runtime particle index: select through the unrolled register array
-      Real pa = p[0][c], pb = p[0][c];
-#pragma unroll
-      for (int i = 1; i < P; ++i) {
-        pa = (a == i) ? p[i][c] : pa;
-        pb = (b == i) ? p[i][c] : pb;
-      }
-      v[c] = pa + pb;
-    }
+    for (int c = 0; c < 4; ++c) v[c] = u[c] + w[c];
     const Real s = v[0] * v[0] - v[1] * v[1] - v[2] * v[2] - v[3] * v[3];
     const Real M = (Real)f.m[k], G = (Real)f.g[k];
     const Real dr = M * M - s;

@@ -70,14 +70,27 @@ inline DecayK make_decayk(const hpsp_decay* d, uint64_t seed) {
   return k;
 }
 
-inline int make_functork(const hpsp_functor* f, int nvec, FunctorK* out) {
+// nvec: four-vectors the functor sees; first_daughter: index of the first
+// decay product among them (0, or 1 when the mother comes first).
+inline int make_functork(const hpsp_functor* f, int nvec, FunctorK* out,
+                         int first_daughter = 0) {
   memset(out, 0, sizeof(*out));
   if (!f) {
     out->kind = HPSP_F_ONE;
     return HPSP_OK;
   }
-  if (f->kind < HPSP_F_ONE || f->kind > HPSP_F_MASS2_01) return HPSP_ERR_INVALID_ARGUMENT;
+  if (f->kind < HPSP_F_ONE || f->kind > HPSP_F_COS_HELICITY) return HPSP_ERR_INVALID_ARGUMENT;
   out->kind = f->kind;
+  if (f->kind == HPSP_F_MASS2 || f->kind == HPSP_F_COS_HELICITY) {
+    if (f->a[0] < 0 || f->a[0] >= nvec || f->b[0] < 0 || f->b[0] >= nvec)
+      return HPSP_ERR_INVALID_ARGUMENT;
+    out->a[0] = f->a[0];
+    out->b[0] = f->b[0];
+    if (f->kind == HPSP_F_COS_HELICITY) {  // parent: the sum of the daughters
+      out->a[1] = first_daughter;
+      out->b[1] = nvec - first_daughter;
+    }
+  }
   if (f->kind == HPSP_F_BW_SUM) {
     if (f->nterms < 0 || f->nterms > HPSP_MAX_TERMS) return HPSP_ERR_INVALID_ARGUMENT;
     out->nterms = f->nterms;

@@ -34,6 +34,18 @@ struct AvgArgs {
   Stats* partials;  // one per kAvgChunk events
 };
 
+struct EvalArgs {
+  DecayK d;
+  FunctorK f[HPSP_MAX_EVAL_FUNCTORS];
+  int nf;
+  uint64_t seed;            // Evaluate(mothers): the seed paired with each event index
+  uint64_t first, n;
+  const double* mothers;    // Evaluate(mothers): SoA mother columns
+  uint64_t mld;
+  double* out;              // 1 + nf columns of ld doubles
+  uint64_t ld;
+};
+
 struct WeightArgs {
   DecayK d;
   FunctorK f;
@@ -69,6 +81,7 @@ int launch_generate_f64(int N, int engine, const GenArgs& a, hipStream_t s);
 int launch_generate_mixed(int N, int engine, const GenArgs& a, hipStream_t s);
 int launch_average_f64(int N, int engine, const AvgArgs& a, hipStream_t s);
 int launch_average_mixed(int N, int engine, const AvgArgs& a, hipStream_t s);
+int launch_evaluate_f64(int N, int engine, bool mothers, const EvalArgs& a, hipStream_t s);
 int launch_stats_finalize(const Stats* partials, uint64_t m, Stats* out, hipStream_t s);
 int launch_event_weight(int N, const WeightArgs& a, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,

@@ -12,6 +12,7 @@
 #include <functional>
 #include <initializer_list>
 #include <stdexcept>
+#include <type_traits>
 #include <utility>
 
 #include <hydra/Decays.h>
@@ -22,6 +23,7 @@
 #include <hydra/detail/random/engines.h>
 #include <hydra/device/System.h>
 #include <hydra/device/vector.h>
+#include <hydra/multivector.h>
 
 #include "hpsp.h"
 
@@ -175,6 +177,48 @@ class PhaseSpace {
     return make_range(daughters.begin(), daughters.begin() + (std::ptrdiff_t)mothers.size());
   }
 
+  // ---- Evaluate (PhaseSpace.inl:289-348) --------------------------------
+  // result rows: (weight, functors(daughters)...), one row per event.
+  template <typename... T, typename... FUNCTORS>
+  Range<typename multivector<hydra::tuple<T...>, hydra::device::sys_t>::iterator> Evaluate(
+      Vector4R const& mother, multivector<hydra::tuple<T...>, hydra::device::sys_t>& result,
+      FUNCTORS const&... functors) {
+    static_assert(sizeof...(T) == sizeof...(FUNCTORS) + 1,
+                  "result needs one column for the weight and one per functor");
+    check_functors<FUNCTORS...>();
+    const hpsp_decay d = descriptor(mother);
+    const hpsp_functor fs[] = {detail::to_hpsp(functors)...};
+    detail::hpsp_check(hpsp_evaluate(&d, fs, (int)sizeof...(FUNCTORS), fSeed, 0, result.size(),
+                                     result.data(), result.ld(), nullptr),
+                       "PhaseSpace::Evaluate");
+    detail::hpsp_sync("PhaseSpace::Evaluate");
+    return make_range(result.begin(), result.end());
+  }
+
+  // Over a container of mothers (EvalMothers): each event's engine is seeded
+  // with the pairing hash of its index and the seed; functors see the mother
+  // as particle 0.
+  template <typename... T, typename... FUNCTORS>
+  Range<typename multivector<hydra::tuple<T...>, hydra::device::sys_t>::iterator> Evaluate(
+      hydra::device::vector<Vector4R> const& mothers,
+      multivector<hydra::tuple<T...>, hydra::device::sys_t>& result,
+      FUNCTORS const&... functors) {
+    static_assert(sizeof...(T) == sizeof...(FUNCTORS) + 1,
+                  "result needs one column for the weight and one per functor");
+    check_functors<FUNCTORS...>();
+    if (result.size() < mothers.size())
+      throw std::invalid_argument("[hydra::PhaseSpace::Evaluate]: result shorter than mothers");
+    const hpsp_decay d = descriptor(Vector4R(fMotherMass, 0, 0, 0));
+    const hpsp_functor fs[] = {detail::to_hpsp(functors)...};
+    detail::hpsp_check(
+        hpsp_evaluate_mothers(&d, fs, (int)sizeof...(FUNCTORS), fSeed, mothers.data(),
+                              mothers.ld(), 0, mothers.size(), result.data(), result.ld(),
+                              nullptr),
+        "PhaseSpace::Evaluate");
+    detail::hpsp_sync("PhaseSpace::Evaluate");
+    return make_range(result.begin(), result.begin() + (std::ptrdiff_t)mothers.size());
+  }
+
   // ---- seeds and parameters (PhaseSpace.h:281-326) -----------------------
   // GetSeed/SetSeed go through int (GInt_t), truncating 64-bit seeds.
   GInt_t GetSeed() const { return (GInt_t)fSeed; }
@@ -227,6 +271,14 @@ class PhaseSpace {
     detail::hpsp_check(rc, "PhaseSpace");
     fECM = d.ecm;
     fMaxWeight = d.max_weight;
+  }
+
+  template <typename... F>
+  static void check_functors() {
+    static_assert(sizeof...(F) >= 1 && sizeof...(F) <= HPSP_MAX_EVAL_FUNCTORS,
+                  "Evaluate takes 1..HPSP_MAX_EVAL_FUNCTORS functors");
+    static_assert(std::conjunction<is_phase_space_functor<F>...>::value,
+                  "functors must be device-described phase-space functors");
   }
 
   bool same_decay(double M, std::array<double, N> const& m) const {

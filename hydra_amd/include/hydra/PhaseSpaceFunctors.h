@@ -1,13 +1,18 @@
-// Per-event functors for AverageOn and Unweight(functor).
+// Per-event functors for AverageOn, Evaluate and Unweight(functor).
 //
 // The reference compiles any callable of the N daughters into its Thrust
 // kernels (hydra/Lambda.h:133-200).  Here the kernels are prebuilt for gfx950,
 // so a functor is a description the kernels evaluate on the device (the
 // hpsp_functor of include/hpsp.h):
 //   ConstantFunctor          f = 1
-//   InvariantMass2           f = (p0 + p1)^2
+//   InvariantMass2{a, b}     f = (p[a] + p[b])^2 (default the pair (0, 1))
+//   CosHelicity{d, e}        helicity cosine of p[d] in the pair (d, e) within
+//                            the sum of the daughters: the cos(theta_K*) of
+//                            examples/phase_space/phsp_evaluating_functor.inl:160-176
 //   BreitWignerSum           f = |sum_k c_k / (M_k^2 - s_k - i M_k G_k)|^2,
 //                            s_k = (p[a_k] + p[b_k])^2
+// Indices count the four-vectors the functor sees: the daughters, or for the
+// calls over a container of mothers the mother (0) then the daughters.
 // Any other functor type is rejected at compile time.
 #pragma once
 
@@ -23,7 +28,15 @@ namespace hydra {
 
 struct ConstantFunctor {};
 
-struct InvariantMass2 {};
+struct InvariantMass2 {
+  int a = 0;
+  int b = 1;
+};
+
+struct CosHelicity {
+  int d;
+  int e;
+};
 
 // One relativistic Breit-Wigner term c / (M^2 - s - i M G) of the pair (a, b).
 struct BreitWignerTerm {
@@ -48,6 +61,7 @@ template <typename F>
 struct is_phase_space_functor
     : std::integral_constant<bool, std::is_same<F, ConstantFunctor>::value ||
                                        std::is_same<F, InvariantMass2>::value ||
+                                       std::is_same<F, CosHelicity>::value ||
                                        std::is_same<F, BreitWignerSum>::value> {};
 
 namespace detail {
@@ -58,9 +72,19 @@ inline hpsp_functor to_hpsp(const ConstantFunctor&) {
   return f;
 }
 
-inline hpsp_functor to_hpsp(const InvariantMass2&) {
+inline hpsp_functor to_hpsp(const InvariantMass2& m) {
   hpsp_functor f{};
-  f.kind = HPSP_F_MASS2_01;
+  f.kind = HPSP_F_MASS2;
+  f.a[0] = m.a;
+  f.b[0] = m.b;
+  return f;
+}
+
+inline hpsp_functor to_hpsp(const CosHelicity& h) {
+  hpsp_functor f{};
+  f.kind = HPSP_F_COS_HELICITY;
+  f.a[0] = h.d;
+  f.b[0] = h.e;
   return f;
 }
 

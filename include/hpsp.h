@@ -79,8 +79,25 @@ typedef struct {
  *   HPSP_F_ONE      f = 1
  *   HPSP_F_BW_SUM   f = |sum_k c_k / (M_k^2 - s_k - i M_k G_k)|^2 with
  *                   s_k = (p[a_k] + p[b_k])^2 and c_k = cr_k + i ci_k
- *   HPSP_F_MASS2_01 f = (p0 + p1)^2                                          */
-typedef enum { HPSP_F_ONE = 0, HPSP_F_BW_SUM = 1, HPSP_F_MASS2_01 = 2 } hpsp_functor_kind;
+ *   HPSP_F_MASS2_01 f = (p0 + p1)^2
+ *   HPSP_F_MASS2    f = (p[a0] + p[b0])^2
+ *   HPSP_F_COS_HELICITY
+ *                   helicity cosine of d = p[a0] in q = p[a0] + p[b0] within P,
+ *                   the sum of the decay's daughters (left to right):
+ *                   (P.d q^2 - P.q q.d) / sqrt((P.q^2 - q^2 P^2)(q.d^2 - q^2 d^2)),
+ *                   the cos(theta_K*) of examples/phase_space/
+ *                   phsp_evaluating_functor.inl:160-176
+ * Particle indices count the four-vectors the functor sees: the N daughters,
+ * or for the "mothers" entry points the mother (index 0) then the daughters. */
+typedef enum {
+  HPSP_F_ONE = 0,
+  HPSP_F_BW_SUM = 1,
+  HPSP_F_MASS2_01 = 2,
+  HPSP_F_MASS2 = 3,
+  HPSP_F_COS_HELICITY = 4
+} hpsp_functor_kind;
+
+#define HPSP_MAX_EVAL_FUNCTORS 8
 
 typedef struct {
   int32_t kind;
@@ -201,6 +218,26 @@ int hpsp_average_on(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed, u
 int hpsp_average_on_mothers(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed,
                             const double* mothers, uint64_t mld, uint64_t n, int prec,
                             void* ws, size_t ws_bytes, hpsp_stats* out_dev, void* stream);
+
+/* PhaseSpace<N>::Evaluate(mother, result, functors...) (PhaseSpace.inl:289-297,
+ * 316-331; EvalMother.h:283-306): per event the generation weight and the
+ * value of each functor, no momenta stored.  out holds 1 + nf columns of ld
+ * doubles: column 0 the weight (DecayMother::process's return value), column
+ * 1 + j functor j.  nf is 1..HPSP_MAX_EVAL_FUNCTORS.  Events first..first+n-1
+ * (the reference's are 0..n-1).                                             */
+int hpsp_evaluate(const hpsp_decay* d, const hpsp_functor* fs, int nf, uint64_t seed,
+                  uint64_t first, uint64_t n, double* out, uint64_t ld, void* stream);
+
+/* PhaseSpace<N>::Evaluate(mothers, result, functors...) (PhaseSpace.inl:333-348;
+ * EvalMothers.h:145-273): event k decays the mother in row k of the SoA
+ * mother columns (E, px, py, pz at mothers + c*mld).  Unlike Generate(mothers),
+ * each event's engine is seeded with the pairing hash of (event index, seed)
+ * and not discarded (EvalMothers.h:131-147), fECM is that mother's mass minus
+ * the daughters' masses and the weight is normalised by that mother's maximum
+ * weight.  Functors see the mother as particle 0 and the daughters as 1..N.   */
+int hpsp_evaluate_mothers(const hpsp_decay* d, const hpsp_functor* fs, int nf, uint64_t seed,
+                          const double* mothers, uint64_t mld, uint64_t first, uint64_t n,
+                          double* out, uint64_t ld, void* stream);
 
 /* ---- serving: continuous batching of generation and integration requests --
  * No counterpart in the reference (its calls are synchronous, one at a time).

@@ -40,14 +40,30 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+FPAR = 32  # stride of one functor's parameters in the evaluate calls
+
+
 def functor_params(fid, terms=None):
-    """Pack functor parameters: terms = [(M, G, cr, ci, a, b), ...]."""
-    if fid != 1:
-        return np.zeros(1)
-    flat = [float(len(terms))]
-    for t in terms:
-        flat.extend(float(x) for x in t)
-    return np.array(flat, dtype=np.float64)
+    """Pack functor parameters.  fid 1: terms = [(M, G, cr, ci, a, b), ...];
+    fid 3: terms = (a, b); fid 4: terms = (d, e, first, count)."""
+    if fid == 1:
+        flat = [float(len(terms))]
+        for t in terms:
+            flat.extend(float(x) for x in t)
+        return np.array(flat, dtype=np.float64)
+    if fid in (3, 4):
+        return np.array([float(x) for x in terms], dtype=np.float64)
+    return np.zeros(1)
+
+
+def pack_functors(functors):
+    """[(fid, terms), ...] -> (int32 ids, float64 parameters at stride FPAR)."""
+    ids = np.array([f[0] for f in functors], dtype=np.int32)
+    par = np.zeros(FPAR * max(1, len(functors)))
+    for j, (fid, terms) in enumerate(functors):
+        fp = functor_params(fid, terms)
+        par[FPAR * j:FPAR * j + len(fp)] = fp
+    return ids, par
 
 
 class Oracle:
@@ -80,6 +96,10 @@ class Oracle:
         L.po_average_on.argtypes = [_I, _I, _P, _D, _D, _P, _U64, _U64, _U64, _I, _P, _I, _P]
         L.po_average_on_mothers.argtypes = [_I, _I, _P, _D, _D, _U64, _U64, _P, _I, _P, _I, _P]
         L.po_stats_merge.argtypes = [_P, _P, _P]
+        L.po_evaluate.argtypes = [_I, _I, _P, _D, _D, _P, _U64, _U64, _U64, _I, _P, _P, _P, _I]
+        L.po_evaluate_mothers.argtypes = [_I, _I, _P, _U64, _U64, _U64, _P, _I, _P, _P, _P, _I]
+        L.po_pair_hash.argtypes = [_U64, _U64]
+        L.po_pair_hash.restype = _U64
 
     # -- engines -----------------------------------------------------------
     def splitmix64(self, x):
@@ -211,6 +231,36 @@ class Oracle:
                                      mc.shape[1], _ptr(mc), fid, _ptr(fp), intervals, _ptr(out))
         return out
 
+    def evaluate(self, M, masses, seed, n, functors, first=0, engine=0, mother4=None,
+                 nthreads=0):
+        """PhaseSpace::Evaluate(mother, result, functors): (1+nf, n) columns."""
+        m = _f64(masses)
+        N = len(m)
+        rc, ecm, mw = self.params(M, m)
+        assert rc == 0
+        beta = self.mother_beta(mother4 if mother4 is not None else [M, 0, 0, 0])
+        ids, par = pack_functors(functors)
+        out = np.zeros((1 + len(ids), n))
+        self.L.po_evaluate(N, ENGINES.get(engine, engine), _ptr(m), ecm, mw, _ptr(beta), seed,
+                           first, n, len(ids), _ptr(ids), _ptr(par), _ptr(out), nthreads)
+        return out
+
+    def evaluate_mothers(self, masses, seed, mothers, functors, first=0, engine=0, nthreads=0):
+        """PhaseSpace::Evaluate(mothers, result, functors); mothers (4, n)."""
+        m = _f64(masses)
+        N = len(m)
+        mc = _f64(mothers)
+        n = mc.shape[1]
+        ids, par = pack_functors(functors)
+        out = np.zeros((1 + len(ids), n))
+        self.L.po_evaluate_mothers(N, ENGINES.get(engine, engine), _ptr(m), seed, first, n,
+                                   _ptr(mc), len(ids), _ptr(ids), _ptr(par), _ptr(out),
+                                   nthreads)
+        return out
+
+    def pair_hash(self, evt, seed):
+        return self.L.po_pair_hash(evt, seed)
+
     def stats_merge(self, x, y):
         x = _f64(x)
         y = _f64(y)
@@ -237,6 +287,8 @@ class Reference:
                                   _P, _P, _P, _P]
         L.hr_average_on.argtypes = [_I, _D, _P, _P, _I, ctypes.c_int32, _U64, _I, _P, _P, _P,
                                     _P]
+        L.hr_evaluate.argtypes = [_I, _D, _P, _P, _I, ctypes.c_int32, _U64, _I, _P, _P, _P, _P]
+        L.hr_evaluate_mothers.argtypes = [_I, _D, _P, ctypes.c_int32, _U64, _P, _I, _P, _P, _P]
 
     def set_threads(self, n):
         return self.L.hr_set_threads(n)
@@ -314,6 +366,42 @@ class Reference:
         self.L.hr_average_on(N, M, _ptr(m4), _ptr(m), use, 0 if seed is None else seed, n, fid,
                              _ptr(fp), ctypes.byref(mean), ctypes.byref(sig), ctypes.byref(sec))
         return mean.value, sig.value, sec.value
+
+
+def _ref_evaluate(L, M, masses, seed, n, functors, mother4=None):
+    m = _f64(masses)
+    N = len(m)
+    m4 = _f64(mother4 if mother4 is not None else [M, 0, 0, 0])
+    ids, par = pack_functors(functors)
+    out = np.zeros((n, 1 + len(ids)))
+    sec = _D()
+    use = 0 if seed is None else 1
+    rc = L.hr_evaluate(N, M, _ptr(m4), _ptr(m), use, 0 if seed is None else seed, n, len(ids),
+                       _ptr(ids), _ptr(par), _ptr(out), ctypes.byref(sec))
+    if rc:
+        raise RuntimeError(f"hr_evaluate failed: {rc}")
+    return np.ascontiguousarray(out.T), sec.value
+
+
+def _ref_evaluate_mothers(L, M, masses, seed, mothers, functors):
+    m = _f64(masses)
+    N = len(m)
+    mr = np.ascontiguousarray(np.asarray(mothers, dtype=np.float64).T)  # (n,4)
+    n = mr.shape[0]
+    ids, par = pack_functors(functors)
+    out = np.zeros((n, 1 + len(ids)))
+    rc = L.hr_evaluate_mothers(N, M, _ptr(m), seed, n, _ptr(mr), len(ids), _ptr(ids),
+                               _ptr(par), _ptr(out))
+    if rc:
+        raise RuntimeError(f"hr_evaluate_mothers failed: {rc}")
+    return np.ascontiguousarray(out.T)
+
+
+Reference.evaluate = lambda self, M, masses, seed, n, functors, mother4=None: \
+    _ref_evaluate(self.L, M, masses, seed, n, functors, mother4)
+Reference.evaluate.__doc__ = "PhaseSpace::Evaluate(mother, result, functors) -> ((1+nf, n), s)"
+Reference.evaluate_mothers = lambda self, M, masses, seed, mothers, functors: \
+    _ref_evaluate_mothers(self.L, M, masses, seed, mothers, functors)
 
 
 def rows_to_cols(p):

@@ -353,12 +353,12 @@ void po_mother_beta(const double* m4, double* b) {
  * DecayMothers::process (DecayMothers.h:131-242).  final_mode 0 boosts by
  * beta3 (single mother); 1 boosts by the per-event mother four-vector
  * (applyBoostTo(Vector4R), Vector4R.inl:237-247).                           */
-static double decay_one(int N, int engine, const double* m, double ecm, double maxw,
-                        int final_mode, const double* fin, uint64_t seed, uint64_t evt,
-                        double (*p)[4]) {
+static double decay_core(int N, int engine, const double* m, double ecm, double maxw,
+                         int final_mode, const double* fin, uint64_t seed, uint64_t discard,
+                         double (*p)[4]) {
   po_engine g;
   engine_init(&g, engine, seed);
-  engine_discard(&g, evt + 3 * (uint64_t)N);
+  engine_discard(&g, discard);
 
   double rno[PO_MAXN];
   rno[0] = 0.0;
@@ -435,6 +435,14 @@ static double decay_one(int N, int engine, const double* m, double ecm, double m
   return wt;
 }
 
+/* DecayMother / DecayMothers / AverageMother(s) seed their engine with the
+ * PhaseSpace seed and discard evt + 3N (DecayMother.h:166-167).             */
+static double decay_one(int N, int engine, const double* m, double ecm, double maxw,
+                        int final_mode, const double* fin, uint64_t seed, uint64_t evt,
+                        double (*p)[4]) {
+  return decay_core(N, engine, m, ecm, maxw, final_mode, fin, seed, evt + 3 * (uint64_t)N, p);
+}
+
 void po_decay_events(int N, int engine, const double* masses, double ecm, double max_weight,
                      const double* beta3, uint64_t seed, const uint64_t* evts, uint64_t nevt,
                      double* out_p, double* out_wt) {
@@ -495,6 +503,11 @@ void po_generate_mothers(int N, int engine, const double* masses, double ecm,
 }
 
 /* Functors (arithmetic order fixed; see header).                            */
+/* Vector4R product (Vector4R.h:171-176) */
+static inline double dot4(const double* a, const double* b) {
+  return a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+}
+
 double po_functor(int fid, const double* fpar, int N, const double* p) {
   (void)N;
   if (fid == 0) return 1.0;
@@ -502,6 +515,31 @@ double po_functor(int fid, const double* fpar, int N, const double* p) {
     double s[4];
     for (int c = 0; c < 4; ++c) s[c] = p[c] + p[4 + c];
     return mass2(s);
+  }
+  if (fid == 3) {
+    int a = (int)fpar[0], b = (int)fpar[1];
+    double s[4];
+    for (int c = 0; c < 4; ++c) s[c] = p[4 * a + c] + p[4 * b + c];
+    return mass2(s);
+  }
+  if (fid == 4) {
+    /* helicity cosine of particle d in the pair (d, e) within the sum of
+     * particles first..first+count-1, as phsp_evaluating_functor.inl:160-176
+     * writes it for B0 -> J/psi K pi (d = K, e = pi, the sum of all three) */
+    int d = (int)fpar[0], e = (int)fpar[1], first = (int)fpar[2], count = (int)fpar[3];
+    double P[4], Q[4];
+    for (int c = 0; c < 4; ++c) P[c] = p[4 * first + c];
+    for (int i = first + 1; i < first + count; ++i)
+      for (int c = 0; c < 4; ++c) P[c] = P[c] + p[4 * i + c];
+    for (int c = 0; c < 4; ++c) Q[c] = p[4 * d + c] + p[4 * e + c];
+    const double* D = p + 4 * d;
+    double pd = dot4(P, D);
+    double pq = dot4(P, Q);
+    double qd = dot4(Q, D);
+    double mp2 = mass2(P);
+    double mq2 = mass2(Q);
+    double md2 = mass2(D);
+    return (pd * mq2 - pq * qd) / sqrt((pq * pq - mq2 * mp2) * (qd * qd - mq2 * md2));
   }
   int K = (int)fpar[0];
   double Ar = 0.0, Ai = 0.0;
@@ -699,4 +737,71 @@ void po_average_on_mothers(int N, int engine, const double* masses, double ecm,
                            int intervals, double* out3) {
   average_core(N, engine, masses, ecm, max_weight, 1, NULL, mother_cols, seed, 0, n, fid, fpar,
                intervals, out3);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Evaluate                                                                   */
+
+/* PhaseSpace::Evaluate(mother, result, functors...) (PhaseSpace.inl:316-331;
+ * EvalMother.h:283-306, launch_decayers.inl:66-72): event k (index first+k)
+ * is DecayMother::process's event; out column 0 holds its weight, column
+ * 1+j functor j of the N daughters.  fpar: functor j's parameters at
+ * fpar + PO_FPAR * j.                                                        */
+void po_evaluate(int N, int engine, const double* masses, double ecm, double max_weight,
+                 const double* beta3, uint64_t seed, uint64_t first, uint64_t n, int nf,
+                 const int* fids, const double* fpar, double* out, int nthreads) {
+  set_threads(nthreads);
+  long long nn = (long long)n;
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < nn; ++k) {
+    double p[PO_MAXN][4];
+    double w = decay_one(N, engine, masses, ecm, max_weight, 0, beta3, seed,
+                         first + (uint64_t)k, p);
+    out[(uint64_t)k] = w;
+    for (int j = 0; j < nf; ++j)
+      out[(uint64_t)(1 + j) * n + (uint64_t)k] = po_functor(fids[j], fpar + PO_FPAR * j, N, &p[0][0]);
+  }
+}
+
+/* Szudzik pairing of (event index, seed) that keys EvalMothers' engines
+ * (EvalMothers.h:131-143, applied at :147); the index arrives as an int
+ * (process(const GInt_t evt, ...)), the seed as size_t.                    */
+uint64_t po_pair_hash(uint64_t evt, uint64_t seed) {
+  uint64_t a = (uint64_t)(int64_t)(int32_t)(uint32_t)evt;
+  uint64_t A = 2 * a, B = 2 * seed;
+  return (A >= B ? A * A + A + B : A + B * B) / 2;
+}
+
+/* PhaseSpace::Evaluate(mothers, result, functors...) (PhaseSpace.inl:333-348;
+ * EvalMothers.h:145-273, launch_decayers.inl:75-89): event k decays mother
+ * row k with an engine seeded by po_pair_hash(first+k, seed) and not
+ * discarded; fECM is the mother's mass minus the daughters' and the weight is
+ * normalised by that mother's maximum; the final boost is by the mother.
+ * Functors see the mother as particle 0 followed by the N daughters.        */
+void po_evaluate_mothers(int N, int engine, const double* masses, uint64_t seed,
+                         uint64_t first, uint64_t n, const double* mother_cols, int nf,
+                         const int* fids, const double* fpar, double* out, int nthreads) {
+  set_threads(nthreads);
+  long long nn = (long long)n;
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < nn; ++k) {
+    double m4[4];
+    for (int c = 0; c < 4; ++c) m4[c] = mother_cols[(uint64_t)c * n + (uint64_t)k];
+    double te = mass(m4);
+    for (int i = 0; i < N; ++i) te -= masses[i];
+    double emmax = te + masses[0], emmin = 0.0, wtmax = 1.0;
+    for (int i = 1; i < N; ++i) {
+      emmin += masses[i - 1];
+      emmax += masses[i];
+      wtmax *= pdk(emmax, emmin, masses[i]);
+    }
+    double p[PO_MAXN + 1][4];
+    double w = decay_core(N, engine, masses, te, 1.0 / wtmax, 1, m4,
+                          po_pair_hash(first + (uint64_t)k, seed), 0, &p[1]);
+    for (int c = 0; c < 4; ++c) p[0][c] = m4[c];
+    out[(uint64_t)k] = w;
+    for (int j = 0; j < nf; ++j)
+      out[(uint64_t)(1 + j) * n + (uint64_t)k] =
+          po_functor(fids[j], fpar + PO_FPAR * j, N + 1, &p[0][0]);
+  }
 }

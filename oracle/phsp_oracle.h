@@ -17,12 +17,19 @@
  *               id 1: f = |sum_k c_k / (M_k^2 - s_k - i M_k G_k)|^2,
  *                     s_k = (p_a + p_b)^2, par = {K, (M,G,cr,ci,a,b) x K}
  *               id 2: f = (p0 + p1)^2
+ *               id 3: f = (p_a + p_b)^2, par = {a, b}
+ *               id 4: helicity cosine of particle d in the pair (d, e) within
+ *                     P = p_first + ... + p_(first+count-1),
+ *                     par = {d, e, first, count}
+ *   - evaluate outputs: column 0 the weight, column 1+j functor j (ld = n);
+ *     functor j's parameters at fpar + PO_FPAR*j
  */
 #ifndef PHSP_ORACLE_H
 #define PHSP_ORACLE_H
 #include <stdint.h>
 
 #define PO_MAXN 16
+#define PO_FPAR 32
 
 #ifdef __cplusplus
 extern "C" {
@@ -68,6 +75,14 @@ void po_average_on_mothers(int N, int engine, const double* masses, double ecm,
                            const double* mother_cols, int fid, const double* fpar,
                            int intervals, double* out3);
 void po_stats_merge(const double* x3, const double* y3, double* out3);
+
+void po_evaluate(int N, int engine, const double* masses, double ecm, double max_weight,
+                 const double* beta3, uint64_t seed, uint64_t first, uint64_t n, int nf,
+                 const int* fids, const double* fpar, double* out, int nthreads);
+uint64_t po_pair_hash(uint64_t evt, uint64_t seed);
+void po_evaluate_mothers(int N, int engine, const double* masses, uint64_t seed,
+                         uint64_t first, uint64_t n, const double* mother_cols, int nf,
+                         const int* fids, const double* fpar, double* out, int nthreads);
 
 #ifdef __cplusplus
 }

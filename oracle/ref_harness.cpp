@@ -20,6 +20,7 @@
 #include <hydra/host/System.h>
 #include <hydra/device/System.h>
 #include <hydra/Decays.h>
+#include <hydra/multivector.h>
 #include <hydra/detail/Hash.h>
 
 #include <omp.h>
@@ -88,6 +89,10 @@ inline void row_to_array(Row const& row, double* out, std::index_sequence<I...>)
 //   id 1: f = |sum_k c_k / (M_k^2 - s_k - i M_k G_k)|^2,  s_k = (p_a + p_b)^2
 //         par = {K, (M, G, cr, ci, a, b) x K}
 //   id 2: f = (p0 + p1)^2
+//   id 3: f = (p_a + p_b)^2                             par = {a, b}
+//   id 4: helicity cosine of particle d in the pair (d, e) within
+//         P = p_first + ... + p_(first+count-1), written as
+//         phsp_evaluating_functor.inl:160-176 writes it  par = {d, e, first, count}
 struct Functor {
   int id;
   int K;
@@ -100,6 +105,23 @@ struct Functor {
     if (id == 2) {
       hydra::Vector4R s = p[0] + p[1];
       return s.mass2();
+    }
+    if (id == 3) {
+      hydra::Vector4R s = p[a[0]] + p[b[0]];
+      return s.mass2();
+    }
+    if (id == 4) {
+      hydra::Vector4R P = p[a[1]];
+      for (int i = a[1] + 1; i < a[1] + b[1]; ++i) P = P + p[i];
+      hydra::Vector4R q = p[a[0]] + p[b[0]];
+      hydra::Vector4R d = p[a[0]];
+      double pd = P * d;
+      double pq = P * q;
+      double qd = q * d;
+      double mp2 = P.mass2();
+      double mq2 = q.mass2();
+      double md2 = d.mass2();
+      return (pd * mq2 - pq * qd) / sqrt((pq * pq - mq2 * mp2) * (qd * qd - mq2 * md2));
     }
     double Ar = 0.0, Ai = 0.0;
     for (int k = 0; k < K; ++k) {
@@ -122,6 +144,16 @@ struct Functor {
 Functor make_functor(int id, const double* par) {
   Functor f{};
   f.id = id;
+  if (id == 3) {
+    f.a[0] = (int)par[0];
+    f.b[0] = (int)par[1];
+  }
+  if (id == 4) {
+    f.a[0] = (int)par[0];
+    f.b[0] = (int)par[1];
+    f.a[1] = (int)par[2];
+    f.b[1] = (int)par[3];
+  }
   if (id == 1) {
     f.K = (int)par[0];
     for (int k = 0; k < f.K; ++k) {
@@ -140,6 +172,7 @@ Functor make_functor(int id, const double* par) {
 // Callable handed to PhaseSpace::AverageOn: receives a tuple of N Vector4R.
 template <size_t N>
 struct TupleFunctor {
+  typedef double return_type;  // PhaseSpace::Evaluate reads FUNCTOR::return_type
   Functor f;
   template <typename Tuple, size_t... I>
   double call(Tuple const& t, std::index_sequence<I...>) const {
@@ -323,6 +356,76 @@ void average_on(double M, const double* mother4, const double* masses, int use_s
   *sigma = r.second;
 }
 
+constexpr int kFpar = 32;  // stride of a functor's parameters
+
+template <size_t N>
+void evaluate(double M, const double* mother4, const double* masses, int use_seed, int32_t seed,
+              uint64_t n, int nf, const int* fids, const double* fpar, double* out,
+              double* seconds) {
+  double m[N];
+  for (size_t i = 0; i < N; ++i) m[i] = masses[i];
+  hydra::PhaseSpace<N> phsp(M, m);
+  if (use_seed) phsp.SetSeed(seed);
+  hydra::Vector4R mother(mother4[0], mother4[1], mother4[2], mother4[3]);
+  TupleFunctor<N> f0{make_functor(fids[0], fpar)};
+  TupleFunctor<N> f1{make_functor(nf > 1 ? fids[1] : 0, fpar + kFpar)};
+  double t0 = now_s();
+  if (nf == 1) {
+    hydra::multivector<hydra::tuple<double, double>, hydra::device::sys_t> r(n);
+    phsp.Evaluate(mother, r, f0);
+    if (seconds) *seconds = now_s() - t0;
+    for (uint64_t i = 0; i < n; ++i) {
+      auto row = r[i];
+      out[i * 2] = hydra::get<0>(row);
+      out[i * 2 + 1] = hydra::get<1>(row);
+    }
+  } else {
+    hydra::multivector<hydra::tuple<double, double, double>, hydra::device::sys_t> r(n);
+    phsp.Evaluate(mother, r, f0, f1);
+    if (seconds) *seconds = now_s() - t0;
+    for (uint64_t i = 0; i < n; ++i) {
+      auto row = r[i];
+      out[i * 3] = hydra::get<0>(row);
+      out[i * 3 + 1] = hydra::get<1>(row);
+      out[i * 3 + 2] = hydra::get<2>(row);
+    }
+  }
+}
+
+template <size_t N>
+void evaluate_mothers(double M, const double* masses, int32_t seed, uint64_t n,
+                      const double* mothers /* n x 4 */, int nf, const int* fids,
+                      const double* fpar, double* out) {
+  double m[N];
+  for (size_t i = 0; i < N; ++i) m[i] = masses[i];
+  hydra::PhaseSpace<N> phsp(M, m);
+  phsp.SetSeed(seed);
+  hydra::device::vector<hydra::Vector4R> mv(n);
+  for (uint64_t i = 0; i < n; ++i)
+    mv[i] = hydra::Vector4R(mothers[4 * i], mothers[4 * i + 1], mothers[4 * i + 2],
+                            mothers[4 * i + 3]);
+  TupleFunctor<N + 1> f0{make_functor(fids[0], fpar)};
+  TupleFunctor<N + 1> f1{make_functor(nf > 1 ? fids[1] : 0, fpar + kFpar)};
+  if (nf == 1) {
+    hydra::multivector<hydra::tuple<double, double>, hydra::device::sys_t> r(n);
+    phsp.Evaluate(mv, r, f0);
+    for (uint64_t i = 0; i < n; ++i) {
+      auto row = r[i];
+      out[i * 2] = hydra::get<0>(row);
+      out[i * 2 + 1] = hydra::get<1>(row);
+    }
+  } else {
+    hydra::multivector<hydra::tuple<double, double, double>, hydra::device::sys_t> r(n);
+    phsp.Evaluate(mv, r, f0, f1);
+    for (uint64_t i = 0; i < n; ++i) {
+      auto row = r[i];
+      out[i * 3] = hydra::get<0>(row);
+      out[i * 3 + 1] = hydra::get<1>(row);
+      out[i * 3 + 2] = hydra::get<2>(row);
+    }
+  }
+}
+
 // Runtime N dispatch (2..12).
 template <template <size_t> class Op, typename... Args>
 int dispatch(int N, Args&&... args) {
@@ -353,6 +456,12 @@ template <size_t N> struct AvgOp {
 };
 template <size_t N> struct UwOp {
   template <typename... A> static void run(int* rc, A... a) { *rc = unweight<N>(a...); }
+};
+template <size_t N> struct EvalOp {
+  template <typename... A> static void run(A... a) { evaluate<N>(a...); }
+};
+template <size_t N> struct EvalMothersOp {
+  template <typename... A> static void run(A... a) { evaluate_mothers<N>(a...); }
 };
 template <size_t N> struct ParamsOp {
   static void run(double M, const double* masses, double* ecm, double* maxw,
@@ -454,6 +563,26 @@ int hr_average_on(int N, double M, const double* mother4, const double* masses, 
                   double* sigma, double* seconds) {
   return dispatch<AvgOp>(N, M, mother4, masses, use_seed, seed, n, fid, fpar, mean, sigma,
                          seconds);
+}
+
+// PhaseSpace<N>::Evaluate(mother, result, f0[, f1]) with result a
+// multivector<tuple<double, double[, double]>>: rows of (weight, f0[, f1]).
+// nf is 1 or 2; functor j's parameters at fpar + 32*j.
+int hr_evaluate(int N, double M, const double* mother4, const double* masses, int use_seed,
+                int32_t seed, uint64_t n, int nf, const int* fids, const double* fpar,
+                double* out, double* seconds) {
+  if (nf < 1 || nf > 2) return -3;
+  return dispatch<EvalOp>(N, M, mother4, masses, use_seed, seed, n, nf, fids, fpar, out,
+                          seconds);
+}
+
+// PhaseSpace<N>::Evaluate(mothers, result, f0[, f1]); the functors receive the
+// mother followed by the N daughters.
+int hr_evaluate_mothers(int N, double M, const double* masses, int32_t seed, uint64_t n,
+                        const double* mothers, int nf, const int* fids, const double* fpar,
+                        double* out) {
+  if (nf < 1 || nf > 2) return -3;
+  return dispatch<EvalMothersOp>(N, M, masses, seed, n, mothers, nf, fids, fpar, out);
 }
 
 }  // extern "C"

@@ -18,10 +18,16 @@
 #include <hydra/PhaseSpace.h>
 #include <hydra/Vector4R.h>
 #include <hydra/device/System.h>
+#include <hydra/multivector.h>
 
 declarg(A, hydra::Vector4R)
 declarg(B, hydra::Vector4R)
 declarg(C, hydra::Vector4R)
+// the columns of examples/phase_space/phsp_evaluating_functor.inl
+declarg(Weight, double)
+declarg(M13Sq, double)
+declarg(M23Sq, double)
+declarg(CosTheta, double)
 
 using namespace hydra::arguments;
 
@@ -156,6 +162,33 @@ int main(int argc, char** argv) {
       for (int q = 0; q < 4; ++q) mrows.push_back(c.get(q));
     }
     emit_array("mothers_rows_C", mrows);
+
+    // Evaluate(mother, dataset, M13^2, M23^2, cos(theta_K*)) as the example does
+    hydra::multivector<hydra::tuple<Weight, M13Sq, M23Sq, CosTheta>, hydra::device::sys_t>
+        dataset(4096);
+    phsp.Evaluate(Parent, dataset, hydra::InvariantMass2{0, 2}, hydra::InvariantMass2{1, 2},
+                  hydra::CosHelicity{1, 2});
+    hydra::multivector<hydra::tuple<Weight, M13Sq, M23Sq, CosTheta>, hydra::host::sys_t> data(
+        dataset);
+    std::vector<double> erows;
+    for (size_t i : {0u, 1u, 2u, 4095u}) {
+      auto row = data[i];
+      erows.push_back(std::get<Weight>(row));
+      erows.push_back(std::get<M13Sq>(row));
+      erows.push_back(std::get<M23Sq>(row));
+      erows.push_back(std::get<CosTheta>(row));
+    }
+    emit_array("evaluate_rows", erows);
+    // over the 256 mothers: functors see (mother, A, B, C)
+    hydra::multivector<hydra::tuple<Weight, M13Sq>, hydra::device::sys_t> dm(256);
+    phsp.Evaluate(mothers, dm, hydra::InvariantMass2{1, 3});
+    std::vector<double> emrows;
+    for (size_t i : {0u, 100u, 255u}) {
+      auto row = dm[i];
+      emrows.push_back(std::get<0>(row).Value());
+      emrows.push_back(std::get<1>(row).Value());
+    }
+    emit_array("evaluate_mothers_rows", emrows);
   }
   std::printf("\"ok\": 1\n}\n");
   return 0;

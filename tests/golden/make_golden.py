@@ -18,6 +18,8 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 from oracle import ENGINES, Reference  # noqa: E402
 
+# evaluate outputs are (1 + nf, n): row 0 the weight, row 1 + j functor j
+
 # Decays of the workloads (3-body B -> J/psi K pi; 10-body flat; near-threshold
 # 10-body; 2-body; 12-body with unequal masses).
 DECAYS = {
@@ -125,6 +127,19 @@ def main():
     out["avg_thr_m2"] = np.array([mean, sig])
     out["avg_procs"] = np.array([procs])
     out["bw3_par"] = bw_par(BW3)
+
+    # PhaseSpace::Evaluate(mother, result, M13^2, M23^2) and with the helicity
+    # cosine of phsp_evaluating_functor.inl (fid 3: (p_a + p_b)^2, fid 4:
+    # helicity cosine, par = {d, e, first, count}); 1024 events
+    ev_m2 = [(3, (0, 2)), (3, (1, 2))]
+    ev_cos = [(4, (1, 2, 0, 3)), (1, BW3)]
+    out["eval_c1_m2"], _ = R.evaluate(M, m, SEED, 1024, ev_m2)
+    out["eval_c1_cos_bw"], _ = R.evaluate(M, m, SEED, 1024, ev_cos)
+    # Evaluate(mothers, result, ...): functors see (mother, daughters...)
+    out["evalm_c1_m2"] = R.evaluate_mothers(M, m, SEED, mothers, [(3, (1, 3)), (3, (0, 1))])
+    out["evalm_c1_cos"] = R.evaluate_mothers(M, m, SEED, mothers, [(4, (2, 3, 1, 3))])
+    p5 = DECAYS["n12"]
+    out["eval_n12_m2"], _ = R.evaluate(p5[0], p5[1], SEED, 256, [(3, (4, 11)), (4, (3, 7, 0, 12))])
 
     np.savez_compressed(os.path.join(HERE, "phsp_golden.npz"), **out)
     print("wrote", os.path.join(HERE, "phsp_golden.npz"), len(out), "arrays")

@@ -101,3 +101,20 @@ def momentum_error(a, b, N):
 
 __all__ = ["DEV", "FP64", "MIXED", "generate", "event_weight", "unweight", "average_on",
            "rng_words", "bw_functor", "momentum_error", "make_decay", "make_functor"]
+
+
+def evaluate(dec, functors, seed, n, first=0, mothers=None, ld=None):
+    """hpsp_evaluate / hpsp_evaluate_mothers -> (1 + nf, ld) float64 tensor."""
+    ld = ld or max(n, 1)
+    out = torch.empty((1 + len(functors), ld), dtype=torch.float64, device=DEV)
+    fs = capi.functor_array(functors)
+    if mothers is None:
+        rc = lib().hpsp_evaluate(ctypes.byref(dec), fs, len(functors), seed, first, n, ptr(out),
+                                 ld, stream_handle())
+    else:
+        mt = torch.as_tensor(np.ascontiguousarray(mothers), dtype=torch.float64, device=DEV)
+        rc = lib().hpsp_evaluate_mothers(ctypes.byref(dec), fs, len(functors), seed, ptr(mt),
+                                         mt.shape[1], first, n, ptr(out), ld, stream_handle())
+    check(rc, "hpsp_evaluate")
+    torch.cuda.synchronize()
+    return out

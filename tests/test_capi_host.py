@@ -88,3 +88,30 @@ def test_calls_validate_before_touching_the_device():
                              None) == capi.ERR_WORKSPACE
     # n = 0 is a no-op
     assert L.hpsp_generate(ctypes.byref(d), 1, 0, 0, 0, None, 0, 0, None, None, None) == 0
+
+
+def test_evaluate_validates_before_touching_the_device():
+    L = capi.lib()
+    d = capi.make_decay(5.279, [3.0969, 0.493677, 0.13957])
+    out = ctypes.c_void_p(64)
+    one = capi.functor_array([capi.make_functor(capi.F_ONE)])
+    nine = capi.functor_array([capi.make_functor(capi.F_ONE)] * 9)
+    assert L.hpsp_evaluate(ctypes.byref(d), one, 0, 1, 0, 10, out, 10, None) == 1
+    assert L.hpsp_evaluate(ctypes.byref(d), nine, 9, 1, 0, 10, out, 10, None) == 1
+    assert L.hpsp_evaluate(ctypes.byref(d), None, 1, 1, 0, 10, out, 10, None) == 1
+    assert L.hpsp_evaluate(ctypes.byref(d), one, 1, 1, 0, 10, None, 10, None) == 1
+    assert L.hpsp_evaluate(ctypes.byref(d), one, 1, 1, 0, 10, out, 5, None) == 1
+    # particle indices beyond what the functor sees
+    bad = capi.functor_array([capi.make_functor(capi.F_MASS2, (0, 3))])
+    assert L.hpsp_evaluate(ctypes.byref(d), bad, 1, 1, 0, 10, out, 10, None) == 1
+    bad = capi.functor_array([capi.make_functor(capi.F_COS_HELICITY, (-1, 2))])
+    assert L.hpsp_evaluate(ctypes.byref(d), bad, 1, 1, 0, 10, out, 10, None) == 1
+    # with mothers the mother is particle 0, so index N is valid ...
+    ok = capi.functor_array([capi.make_functor(capi.F_MASS2, (0, 3))])
+    assert L.hpsp_evaluate_mothers(ctypes.byref(d), ok, 1, 1, None, 10, 0, 10, out, 10,
+                                   None) == 1  # (null mothers)
+    assert L.hpsp_evaluate_mothers(ctypes.byref(d), ok, 1, 1, out, 5, 0, 10, out, 10,
+                                   None) == 1  # (mld < n)
+    # ... and n = 0 is a no-op
+    assert L.hpsp_evaluate(ctypes.byref(d), one, 1, 1, 0, 0, None, 0, None) == 0
+    assert L.hpsp_evaluate_mothers(ctypes.byref(d), ok, 1, 1, None, 0, 0, 0, None, 0, None) == 0

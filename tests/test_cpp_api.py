@@ -79,3 +79,17 @@ def test_gpu_generate_unweight_average(golden, oracle):
     exp = cols.reshape(3, 4, 256)[2][:, [0, 100, 255]].T
     got = np.array(r["mothers_rows_C"]).reshape(3, 4)
     assert np.max(np.abs(got - exp) / exp[:, 0:1]) <= 1e-12
+    # Evaluate(mother, dataset, M13^2, M23^2, cos(theta_K*)) against the oracle,
+    # which agrees bit for bit with the reference's PhaseSpace::Evaluate
+    ev = oracle.evaluate(M, MASSES, 159753, 4096, [(3, (0, 2)), (3, (1, 2)), (4, (1, 2, 0, 3))])
+    got = np.array(r["evaluate_rows"]).reshape(4, 4).T
+    exp = ev[:, [0, 1, 2, 4095]]
+    assert np.array_equal(got[0], exp[0])  # generation weights: bit-exact
+    assert np.max(np.abs(got[1:3] - exp[1:3]) / (M * M)) <= 1e-12
+    assert np.max(np.abs(got[3] - exp[3])) <= 1e-9
+    # Evaluate over the 256 mothers: functors see (mother, A, B, C)
+    em = oracle.evaluate_mothers(MASSES, 159753, mothers, [(3, (1, 3))])
+    got = np.array(r["evaluate_mothers_rows"]).reshape(3, 2).T
+    exp = em[:, [0, 100, 255]]
+    assert np.array_equal(got[0], exp[0])
+    assert np.max(np.abs(got[1] - exp[1]) / (M * M)) <= 1e-12

@@ -278,3 +278,90 @@ def test_oracle_equals_reference_random_decays(oracle, reference, N):
     nacc, pu, idx, _ = reference.unweight(M, m, 31, 700, seed)
     n2, out, idx2, _ = oracle.unweight(M, m, cols, seed)
     assert nacc == n2 and np.array_equal(idx, idx2)
+
+
+# ---------------------------------------------------------------- Evaluate
+BW3 = [(0.892, 0.05, 0.4967141530, -0.1382643012, 1, 2),
+       (1.43, 0.11, 0.6476885381, 1.5230298564, 1, 2),
+       (0.775, 0.149, -0.2341533747, -0.2341368981, 0, 2)]
+
+
+def test_evaluate_golden(oracle, golden):
+    M, m = DECAYS["c1_3body"]
+    got = oracle.evaluate(M, m, SEED, 1024, [(3, (0, 2)), (3, (1, 2))])
+    assert np.array_equal(bits(got), bits(golden["eval_c1_m2"]))
+    got = oracle.evaluate(M, m, SEED, 1024, [(4, (1, 2, 0, 3)), (1, BW3)])
+    assert np.array_equal(bits(got), bits(golden["eval_c1_cos_bw"]))
+    M12, m12 = DECAYS["n12"]
+    got = oracle.evaluate(M12, m12, SEED, 256, [(3, (4, 11)), (4, (3, 7, 0, 12))])
+    assert np.array_equal(bits(got), bits(golden["eval_n12_m2"]))
+    mothers = golden["genm_mothers"]
+    got = oracle.evaluate_mothers(m, SEED, mothers, [(3, (1, 3)), (3, (0, 1))])
+    assert np.array_equal(bits(got), bits(golden["evalm_c1_m2"]))
+    got = oracle.evaluate_mothers(m, SEED, mothers, [(4, (2, 3, 1, 3))])
+    assert np.array_equal(bits(got), bits(golden["evalm_c1_cos"]))
+
+
+def test_evaluate_consistent_with_generate(oracle):
+    """Evaluate's weight is Generate's generation weight; its functor columns
+    are the functors of Generate's momenta, computed independently here."""
+    M, m = DECAYS["c1_3body"]
+    n = 2000
+    ev = oracle.evaluate(M, m, 77, n, [(3, (0, 2)), (4, (1, 2, 0, 3))], first=5000)
+    cols, wt = oracle.generate(M, m, 77, n, first=5000)
+    p = cols_to_rows(cols, 3)
+    assert np.array_equal(bits(ev[0]), bits(wt))
+    s = p[:, 0] + p[:, 2]
+    m13 = s[:, 0] ** 2 - (s[:, 1:] ** 2).sum(axis=1)
+    assert np.max(np.abs(ev[1] - m13)) <= 1e-12 * M * M
+    # helicity cosine from the K direction in the (K pi) rest frame against
+    # the (K pi) direction in the B frame: Lorentz-invariant closed form
+    P = p.sum(axis=1)
+    q = p[:, 1] + p[:, 2]
+    d = p[:, 1]
+
+    def dot(a, b):
+        return a[:, 0] * b[:, 0] - (a[:, 1:] * b[:, 1:]).sum(axis=1)
+    num = dot(P, d) * dot(q, q) - dot(P, q) * dot(q, d)
+    den = np.sqrt((dot(P, q) ** 2 - dot(q, q) * dot(P, P)) * (dot(q, d) ** 2 - dot(q, q) * dot(d, d)))
+    assert np.max(np.abs(ev[2] - num / den)) <= 1e-9
+    assert np.all(np.abs(ev[2]) <= 1.0 + 1e-12)
+    # decay in the (K pi) rest frame is isotropic under flat phase space:
+    # the weighted mean of cos(theta) vanishes
+    assert abs(np.sum(ev[0] * ev[2]) / np.sum(ev[0])) < 0.05
+
+
+def test_evaluate_mothers_closed_forms(oracle):
+    """Mothers at rest with the generator's mass: the weight normalisation and
+    fECM are the constructor's (same maximum), momenta sum to the mother, and
+    the engine of event k is keyed by the pairing hash of (k, seed)."""
+    M, m = DECAYS["c1_3body"]
+    n = 500
+    mothers = np.zeros((4, n))
+    mothers[0] = M
+    ev = oracle.evaluate_mothers(m, 99, mothers, [(3, (0, 1)), (1, BW3)])
+    assert np.all(ev[0] > 0) and np.all(ev[0] <= 1.0)
+    # f = (mother + p1)^2 = M^2 + m1^2 + 2 M E1 with E1 <= M: within bounds
+    assert np.all(ev[1] >= (M + m[0]) ** 2 - 1e-9)
+    # pairing hash (Szudzik): a >= b -> a^2 + a + b else a + b^2, over 2x values
+    for a, b in ((0, 0), (3, 7), (7, 3), (2**31 - 1, 5), (2**31, 5)):
+        A = (a if a < 2**31 else a - 2**32) % 2**64 * 2 % 2**64
+        B = 2 * b
+        want = ((A * A + A + B) if A >= B else (A + B * B)) % 2**64 // 2
+        assert oracle.pair_hash(a, b) == want
+
+
+def test_evaluate_equals_reference(oracle, reference):
+    for N in (2, 3, 6, 12):
+        rng = np.random.default_rng(40 + N)
+        m = list(rng.uniform(0.1, 1.0, N))
+        M = sum(m) * rng.uniform(1.05, 3.0)
+        F = [(3, (0, N - 1)), (4, (0, N - 1, 0, N))]
+        ref, _ = reference.evaluate(M, m, 4321, 300, F)
+        assert np.array_equal(bits(oracle.evaluate(M, m, 4321, 300, F)), bits(ref))
+        mt = np.zeros((4, 300))
+        mt[1:] = rng.normal(0, 2.0, (3, 300))
+        mt[0] = np.sqrt((M * rng.uniform(1.0, 1.2, 300)) ** 2 + (mt[1:] ** 2).sum(axis=0))
+        Fm = [(3, (1, N)), (4, (1, N, 1, N))]
+        ref = reference.evaluate_mothers(M, m, 4321, mt, Fm)
+        assert np.array_equal(bits(oracle.evaluate_mothers(m, 4321, mt, Fm)), bits(ref))
