@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--extras", type=int, default=1, help="also time the other workloads")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--only", default="",
-                    help="run one workload: c2, c3, c4, evaluate, c5 (serving), or copy (a plain 1 GiB device copy "
+                    help="run one workload: c2, c3, c4, evaluate, histogram, c5 (serving), or copy (a plain 1 GiB device copy "
                          "that calibrates the profiler's byte counters)")
     return ap.parse_args()
 
@@ -294,6 +294,91 @@ def run_evaluate(torch, capi, n=1 << 27, reps=5):
     return res
 
 
+def run_histogram(torch, capi, n=1 << 27, reps=5):
+    """The Dalitz plot of phsp_basic.inl:196-214: a 100 x 100 DenseHistogram
+    of (m^2_AB, m^2_BC) weighted by PhaseSpaceWeight, over the headline's
+    2^27 generated 3-body events.  Fused: axes and weights computed from the
+    event columns in the fill (reads 96 B/event); columns: from two value
+    columns and the weight column written beforehand (reads 24 B/event)."""
+    lib = capi.lib()
+    dec = capi.make_decay(M3, MASSES3)
+    sh = capi.stream_handle()
+    cols = torch.empty((12, n), dtype=torch.float64, device="cuda:0")
+    evw = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    capi.check(lib.hpsp_generate(ctypes.byref(dec), SEED, 0, n, capi.FP64, capi.ptr(cols), n, 0,
+                                 None, capi.ptr(evw), sh), "generate")
+    m = MASSES3
+    lo = [(m[0] + m[1]) ** 2, (m[1] + m[2]) ** 2]
+    hi = [(M3 - m[2]) ** 2, (M3 - m[0]) ** 2]
+    h = capi.make_histogram([100, 100], lo, hi)
+    out = torch.empty(10002, dtype=torch.float64, device="cuda:0")
+    axes = capi.functor_array([capi.make_functor(capi.F_MASS2, (0, 1)),
+                               capi.make_functor(capi.F_MASS2, (1, 2))])
+    res = {"events": n, "bins": "100 x 100"}
+    for tag, mode, w, nbytes in (("fused_phsp_weight", capi.HIST_WEIGHT_PHSP, None, 96),
+                                 ("fused_weight_column", capi.HIST_WEIGHT_COLUMN, evw, 104)):
+        def fill():
+            capi.check(lib.hpsp_histogram_fill_events(ctypes.byref(h), ctypes.byref(dec), axes,
+                                                      capi.ptr(cols), n, n, mode, capi.ptr(w),
+                                                      capi.ptr(out), sh), "fill_events")
+        fill()
+        t = Timer(torch)
+        for _ in range(reps):
+            t.start()
+            fill()
+            t.stop()
+        res[f"{tag}_ms"] = t.avg_ms
+        res[f"{tag}_events_per_s"] = n / (t.avg_ms / 1e3)
+        res[f"{tag}_GBs"] = n * nbytes / (t.avg_ms * 1e6)
+    res["sum_of_weights"] = float(out.sum().item())
+    # the same plot from value columns (as Evaluate writes them)
+    x = torch.empty((2, n), dtype=torch.float64, device="cuda:0")
+    c = cols.view(3, 4, n)
+    for j, (a, b) in enumerate(((0, 1), (1, 2))):
+        s = c[a] + c[b]
+        x[j] = s[0] * s[0] - s[1] * s[1] - s[2] * s[2] - s[3] * s[3]
+        del s
+    del cols
+
+    def fillc():
+        capi.check(lib.hpsp_histogram_fill(ctypes.byref(h), capi.ptr(x), n, n, capi.ptr(evw),
+                                           capi.ptr(out), sh), "fill")
+    fillc()
+    t = Timer(torch)
+    for _ in range(reps):
+        t.start()
+        fillc()
+        t.stop()
+    res["columns_ms"] = t.avg_ms
+    res["columns_events_per_s"] = n / (t.avg_ms / 1e3)
+    res["columns_GBs"] = n * 24 / (t.avg_ms * 1e6)
+    return res
+
+
+def cpu_baseline_histogram(n=1 << 22):
+    """The reference's DenseHistogram<double,2>::Fill (OMP, all host threads)
+    on the Dalitz variables and weights of 2^22 events of the same decay."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    lib = orc.REF_NATIVE_LIB if os.path.exists(orc.REF_NATIVE_LIB) else orc.REF_LIB
+    if not os.path.exists(lib):
+        return None
+    R = orc.Reference(lib)
+    cores = len(os.sched_getaffinity(0))
+    R.set_threads(cores)
+    p, w, _ = R.generate(M3, MASSES3, SEED, n)
+    sab, sbc = p[:, 0] + p[:, 1], p[:, 1] + p[:, 2]
+    x = np.vstack([sab[:, 0] ** 2 - (sab[:, 1:] ** 2).sum(1), sbc[:, 0] ** 2 - (sbc[:, 1:] ** 2).sum(1)])
+    m = MASSES3
+    lo = [(m[0] + m[1]) ** 2, (m[1] + m[2]) ** 2]
+    hi = [(M3 - m[2]) ** 2, (M3 - m[0]) ** 2]
+    _, sec = R.histogram_fill([100, 100], lo, hi, x, w)
+    return {"value": n / sec, "unit": "events/s", "cores": cores, "kind": "reference",
+            "sample": f"{n} events: DenseHistogram<double,2>::Fill(dalitz, weights), 100 x 100, "
+                      f"reference OMP backend, {cores} threads (sort_by_key + reduce_by_key)"}
+
+
 BW2 = [(0.892, 0.05, 1.0, 0.3, 1, 2), (1.43, 0.11, -0.4, 0.8, 0, 1)]
 
 
@@ -503,6 +588,10 @@ def main():
         extras["evaluate"] = run_evaluate(torch, capi)
         print(json.dumps(extras), flush=True)
         return
+    if args.only == "histogram":
+        extras["histogram"] = run_histogram(torch, capi)
+        print(json.dumps(extras), flush=True)
+        return
     if args.only == "c5":
         extras["c5"] = run_c5(torch, capi)
         extras["c5_saturated"] = run_c5(torch, capi, rate=50000.0)
@@ -531,6 +620,8 @@ def main():
         print("# c4 done", file=sys.stderr, flush=True)
         extras["evaluate"] = run_evaluate(torch, capi)
         print("# evaluate done", file=sys.stderr, flush=True)
+        extras["histogram"] = run_histogram(torch, capi)
+        print("# histogram done", file=sys.stderr, flush=True)
         extras["c5"] = run_c5(torch, capi)
         print("# c5 done", file=sys.stderr, flush=True)
     extras["c2"] = c2
@@ -555,6 +646,8 @@ def main():
             extras["c5"]["cpu_baseline"] = cpu_baseline_mix()
         if args.extras and "evaluate" in extras:
             extras["evaluate"]["cpu_baseline"] = cpu_baseline_evaluate()
+        if args.extras and "histogram" in extras:
+            extras["histogram"]["cpu_baseline"] = cpu_baseline_histogram()
         print("# cpu baseline done", file=sys.stderr, flush=True)
 
     line = {

@@ -50,6 +50,24 @@ class Stats(ctypes.Structure):
     _fields_ = [("w", ctypes.c_double), ("mean", ctypes.c_double), ("m2", ctypes.c_double)]
 
 
+HIST_MAX_DIM = 4
+HIST_WEIGHT_ONE, HIST_WEIGHT_COLUMN, HIST_WEIGHT_PHSP = 0, 1, 2
+
+
+class Histogram(ctypes.Structure):
+    _fields_ = [("ndim", ctypes.c_int32), ("grid", ctypes.c_uint64 * HIST_MAX_DIM),
+                ("lower", ctypes.c_double * HIST_MAX_DIM),
+                ("upper", ctypes.c_double * HIST_MAX_DIM)]
+
+
+def make_histogram(grid, lower, upper):
+    h = Histogram()
+    h.ndim = len(grid)
+    for i, (g, lo, hi) in enumerate(zip(grid, lower, upper)):
+        h.grid[i], h.lower[i], h.upper[i] = int(g), float(lo), float(hi)
+    return h
+
+
 class Request(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("prec", ctypes.c_int32),
                 ("out_f32", ctypes.c_int32), ("want_weights", ctypes.c_int32),
@@ -117,6 +135,10 @@ _SIGS = {
                            _P, _U64, _P]),
     "hpsp_evaluate_mothers": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _I, _U64, _P,
                                    _U64, _U64, _U64, _P, _U64, _P]),
+    "hpsp_histogram_bins": (_U64, [ctypes.POINTER(Histogram)]),
+    "hpsp_histogram_fill": (_I, [ctypes.POINTER(Histogram), _P, _U64, _U64, _P, _P, _P]),
+    "hpsp_histogram_fill_events": (_I, [ctypes.POINTER(Histogram), ctypes.POINTER(Decay),
+                                        ctypes.POINTER(Functor), _P, _U64, _U64, _I, _P, _P, _P]),
     "hpsp_malloc": (_I, [ctypes.POINTER(_P), _SZ]),
     "hpsp_free": (_I, [_P]),
     "hpsp_memcpy_to_device": (_I, [_P, _P, _SZ]),

@@ -3,6 +3,7 @@
 // dispatch.  Host arithmetic here follows the reference's order
 // (built with -ffp-contract=off).
 #include <math.h>
+#include <algorithm>
 #include <string.h>
 
 #include "hpsp.h"
@@ -206,6 +207,98 @@ int hpsp_evaluate_mothers(const hpsp_decay* d, const hpsp_functor* fs, int nf, u
                           double* out, uint64_t ld, void* stream) {
   if (n > 0 && !mothers) return HPSP_ERR_INVALID_ARGUMENT;
   return evaluate_common(d, fs, nf, seed, mothers, mld, first, n, out, ld, stream);
+}
+
+namespace {
+
+int make_histk(const hpsp_histogram* h, HistK* k) {
+  if (!h || h->ndim < 1 || h->ndim > HPSP_HIST_MAX_DIM) return HPSP_ERR_INVALID_ARGUMENT;
+  memset(k, 0, sizeof(*k));
+  k->ndim = h->ndim;
+  uint64_t nb = 1;
+  for (int i = 0; i < h->ndim; ++i) {
+    if (h->grid[i] == 0 || !(h->upper[i] > h->lower[i])) return HPSP_ERR_INVALID_ARGUMENT;
+    if (nb > (1ULL << 40) / h->grid[i]) return HPSP_ERR_INVALID_ARGUMENT;
+    nb *= h->grid[i];
+    k->lower[i] = h->lower[i];
+    k->delta[i] = h->upper[i] - h->lower[i];  // GetGlobalBin's fDelta
+    k->gridf[i] = (double)h->grid[i];
+  }
+  for (int i = 0; i < h->ndim; ++i) {
+    uint64_t st = 1;
+    for (int j = h->ndim - 1; j > i; --j) st *= h->grid[j];
+    k->stride[i] = st;
+  }
+  k->nbins = nb;
+  return HPSP_OK;
+}
+
+// LDS available to one block, in doubles (the device's per-block limit, at
+// most 128 KiB so that the fill kernel keeps room for its own use).
+uint64_t hist_lds_slots() {
+  static const uint64_t slots = [] {
+    int dev = 0, bytes = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess)
+      bytes = 65536;
+    return (uint64_t)std::min(bytes, 131072) / sizeof(double);
+  }();
+  return slots;
+}
+
+}  // namespace
+
+uint64_t hpsp_histogram_bins(const hpsp_histogram* h) {
+  HistK k;
+  return make_histk(h, &k) == HPSP_OK ? k.nbins : 0;
+}
+
+int hpsp_histogram_fill(const hpsp_histogram* h, const double* x, uint64_t ldx, uint64_t n,
+                        const double* w, double* contents, void* stream) {
+  HistArgs a;
+  memset(&a, 0, sizeof(a));
+  int rc = make_histk(h, &a.h);
+  if (rc) return rc;
+  if (!contents || (n > 0 && (!x || ldx < n))) return HPSP_ERR_INVALID_ARGUMENT;
+  a.x = x;
+  a.ldx = ldx;
+  a.w = w;
+  a.weight_mode = w ? HPSP_HIST_WEIGHT_COLUMN : HPSP_HIST_WEIGHT_ONE;
+  a.n = n;
+  a.contents = contents;
+  a.lds_slots = hist_lds_slots();
+  return hip_status(launch_histogram_fill(0, a, as_stream(stream)));
+}
+
+int hpsp_histogram_fill_events(const hpsp_histogram* h, const hpsp_decay* d,
+                               const hpsp_functor* axes, const double* cols, uint64_t ld,
+                               uint64_t n, int weight_mode, const double* w, double* contents,
+                               void* stream) {
+  HistArgs a;
+  memset(&a, 0, sizeof(a));
+  int rc = make_histk(h, &a.h);
+  if (rc) return rc;
+  rc = check_decay(d);
+  if (rc) return rc;
+  if (!contents || !axes) return HPSP_ERR_INVALID_ARGUMENT;
+  if (weight_mode < HPSP_HIST_WEIGHT_ONE || weight_mode > HPSP_HIST_WEIGHT_PHSP)
+    return HPSP_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (!cols || ld < n)) return HPSP_ERR_INVALID_ARGUMENT;
+  if (n > 0 && weight_mode == HPSP_HIST_WEIGHT_COLUMN && !w) return HPSP_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < h->ndim; ++i) {
+    rc = make_functork(&axes[i], d->n, &a.axes[i]);
+    if (rc) return rc;
+  }
+  a.d = make_decayk(d, 0);
+  a.cols = cols;
+  a.ld = ld;
+  a.weight_mode = weight_mode;
+  a.w = w;
+  a.n = n;
+  a.contents = contents;
+  a.lds_slots = hist_lds_slots();
+  return hip_status(launch_histogram_fill(d->n, a, as_stream(stream)));
 }
 
 int hpsp_event_weight(const hpsp_decay* d, const hpsp_functor* f, const double* cols,

@@ -46,6 +46,33 @@ struct EvalArgs {
   uint64_t ld;
 };
 
+// Dense histogram geometry with the reference's derived quantities:
+// delta = upper - lower, gridf = (double)grid, row-major strides.
+struct HistK {
+  int ndim;
+  double lower[HPSP_HIST_MAX_DIM], delta[HPSP_HIST_MAX_DIM], gridf[HPSP_HIST_MAX_DIM];
+  uint64_t stride[HPSP_HIST_MAX_DIM];
+  uint64_t nbins;
+};
+
+constexpr int kHistBlock = 1024;
+
+struct HistArgs {
+  HistK h;
+  const double* x;          // columns source: ndim value columns
+  uint64_t ldx;
+  DecayK d;                 // events source: masses and fMaxWeight (PhaseSpaceWeight)
+  FunctorK axes[HPSP_HIST_MAX_DIM];
+  const double* cols;       // events source: 4N momentum columns
+  uint64_t ld;
+  int weight_mode;          // hpsp_hist_weight
+  const double* w;
+  uint64_t n;
+  double* contents;         // nbins + 2
+  int use_lds;              // block-private bins in LDS
+  uint64_t lds_slots;       // LDS capacity in doubles (launcher)
+};
+
 struct WeightArgs {
   DecayK d;
   FunctorK f;
@@ -82,6 +109,7 @@ int launch_generate_mixed(int N, int engine, const GenArgs& a, hipStream_t s);
 int launch_average_f64(int N, int engine, const AvgArgs& a, hipStream_t s);
 int launch_average_mixed(int N, int engine, const AvgArgs& a, hipStream_t s);
 int launch_evaluate_f64(int N, int engine, bool mothers, const EvalArgs& a, hipStream_t s);
+int launch_histogram_fill(int N, const HistArgs& a, hipStream_t s);  // N = 0: columns
 int launch_stats_finalize(const Stats* partials, uint64_t m, Stats* out, hipStream_t s);
 int launch_event_weight(int N, const WeightArgs& a, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,

@@ -239,6 +239,53 @@ int hpsp_evaluate_mothers(const hpsp_decay* d, const hpsp_functor* fs, int nf, u
                           const double* mothers, uint64_t mld, uint64_t first, uint64_t n,
                           double* out, uint64_t ld, void* stream);
 
+/* ---- dense histograms ------------------------------------------------------
+ * hydra::DenseHistogram<double, ndim> (hydra/DenseHistogram.h:54-636,
+ * detail/DenseHistogram.inl) as the phase-space examples fill it downstream of
+ * Generate (examples/phase_space/phsp_basic.inl:196-214).  contents holds
+ * nbins + 2 doubles: bin (i_1, ..., i_ndim) at i_1*(g_2*...*g_ndim) + ... + i_ndim,
+ * then the underflow (all axes below range) and overflow (all above) slots.
+ * A fill overwrites the contents, as the reference's does.  Bin keys follow
+ * GetGlobalBin (GetGlobalBin.h:41-209), including its edge cases: a value
+ * exactly at an axis' upper limit has index grid (in one dimension that is the
+ * underflow slot).  Events whose key the reference leaves undefined (an axis
+ * at or below lower - delta/grid, NaN, a key past nbins + 1) are not counted.
+ * Sums inside a bin are accumulated in a different order than the
+ * reference's sorted reduce_by_key: equal to rounding; unit weights exactly. */
+#define HPSP_HIST_MAX_DIM 4
+
+typedef struct {
+  int32_t ndim;                         /* 1..HPSP_HIST_MAX_DIM */
+  uint64_t grid[HPSP_HIST_MAX_DIM];     /* bins per axis */
+  double lower[HPSP_HIST_MAX_DIM];
+  double upper[HPSP_HIST_MAX_DIM];
+} hpsp_histogram;
+
+typedef enum {
+  HPSP_HIST_WEIGHT_ONE = 0,    /* every event counts 1 (Fill(data)) */
+  HPSP_HIST_WEIGHT_COLUMN = 1, /* weights from a column (Fill(data, weights)) */
+  HPSP_HIST_WEIGHT_PHSP = 2    /* the event's PhaseSpaceWeight (Decays.inl:215-231) */
+} hpsp_hist_weight;
+
+/* Number of bins (product of the grid); contents need two more doubles. */
+uint64_t hpsp_histogram_bins(const hpsp_histogram* h);
+
+/* DenseHistogram::Fill(data[, weights]): x holds ndim columns of n values
+ * (column d at x + d*ldx); w is a weight column or NULL for unit weights.   */
+int hpsp_histogram_fill(const hpsp_histogram* h, const double* x, uint64_t ldx, uint64_t n,
+                        const double* w, double* contents, void* stream);
+
+/* The same fill with the axis values computed from the events themselves:
+ * axis d is functor axes[d] of the N daughters of the SoA event columns
+ * (4N columns of ld doubles, as hpsp_generate writes them), weights per
+ * weight_mode (w is the column for HPSP_HIST_WEIGHT_COLUMN).  This is the
+ * reference's `events | dalitz_calculator` range histogrammed with
+ * `events | GetEventWeightFunctor()`, without materialising either.        */
+int hpsp_histogram_fill_events(const hpsp_histogram* h, const hpsp_decay* d,
+                               const hpsp_functor* axes, const double* cols, uint64_t ld,
+                               uint64_t n, int weight_mode, const double* w, double* contents,
+                               void* stream);
+
 /* ---- serving: continuous batching of generation and integration requests --
  * No counterpart in the reference (its calls are synchronous, one at a time).
  * A server owns an HBM arena and a worker thread.  Requests are admitted when

@@ -100,6 +100,9 @@ class Oracle:
         L.po_evaluate_mothers.argtypes = [_I, _I, _P, _U64, _U64, _U64, _P, _I, _P, _P, _P, _I]
         L.po_pair_hash.argtypes = [_U64, _U64]
         L.po_pair_hash.restype = _U64
+        L.po_histogram_fill.argtypes = [_I, _P, _P, _P, _P, _U64, _U64, _P, _P]
+        L.po_hist_key.argtypes = [_I, _P, _P, _P, _P]
+        L.po_hist_key.restype = _U64
 
     # -- engines -----------------------------------------------------------
     def splitmix64(self, x):
@@ -258,6 +261,23 @@ class Oracle:
                                    nthreads)
         return out
 
+    def histogram_fill(self, grid, lower, upper, x, w=None):
+        """DenseHistogram::Fill: x (ndim, n) values, optional weights; returns
+        the nbins + 2 contents (underflow, overflow last)."""
+        g = np.ascontiguousarray(grid, dtype=np.uint64)
+        lo, hi = _f64(lower), _f64(upper)
+        xv = _f64(np.atleast_2d(x))
+        ndim, n = xv.shape
+        out = np.zeros(int(np.prod(g)) + 2)
+        self.L.po_histogram_fill(ndim, _ptr(g), _ptr(lo), _ptr(hi), _ptr(xv), n, n,
+                                 _ptr(None if w is None else _f64(w)), _ptr(out))
+        return out
+
+    def hist_key(self, grid, lower, upper, x):
+        g = np.ascontiguousarray(grid, dtype=np.uint64)
+        return self.L.po_hist_key(len(g), _ptr(g), _ptr(_f64(lower)), _ptr(_f64(upper)),
+                                  _ptr(_f64(x)))
+
     def pair_hash(self, evt, seed):
         return self.L.po_pair_hash(evt, seed)
 
@@ -289,6 +309,7 @@ class Reference:
                                     _P]
         L.hr_evaluate.argtypes = [_I, _D, _P, _P, _I, ctypes.c_int32, _U64, _I, _P, _P, _P, _P]
         L.hr_evaluate_mothers.argtypes = [_I, _D, _P, ctypes.c_int32, _U64, _P, _I, _P, _P, _P]
+        L.hr_histogram_fill.argtypes = [_I, _P, _P, _P, _P, _U64, _P, _P, _P]
 
     def set_threads(self, n):
         return self.L.hr_set_threads(n)
@@ -402,6 +423,23 @@ Reference.evaluate = lambda self, M, masses, seed, n, functors, mother4=None: \
 Reference.evaluate.__doc__ = "PhaseSpace::Evaluate(mother, result, functors) -> ((1+nf, n), s)"
 Reference.evaluate_mothers = lambda self, M, masses, seed, mothers, functors: \
     _ref_evaluate_mothers(self.L, M, masses, seed, mothers, functors)
+
+
+def _ref_histogram_fill(L, grid, lower, upper, x, w=None):
+    g = np.ascontiguousarray(grid, dtype=np.uint64)
+    xv = _f64(np.atleast_2d(x))
+    ndim, n = xv.shape
+    out = np.zeros(int(np.prod(g)) + 2)
+    sec = _D()
+    rc = L.hr_histogram_fill(ndim, _ptr(g), _ptr(_f64(lower)), _ptr(_f64(upper)), _ptr(xv), n,
+                             _ptr(None if w is None else _f64(w)), _ptr(out), ctypes.byref(sec))
+    if rc:
+        raise RuntimeError(f"hr_histogram_fill failed: {rc}")
+    return out, sec.value
+
+
+Reference.histogram_fill = lambda self, grid, lower, upper, x, w=None: \
+    _ref_histogram_fill(self.L, grid, lower, upper, x, w)
 
 
 def rows_to_cols(p):

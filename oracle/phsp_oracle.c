@@ -805,3 +805,81 @@ void po_evaluate_mothers(int N, int engine, const double* masses, uint64_t seed,
           po_functor(fids[j], fpar + PO_FPAR * j, N + 1, &p[0][0]);
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Dense histograms                                                           */
+
+/* GetGlobalBin (GetGlobalBin.h:41-150, one dimension :154-209): per axis
+ * X = (x - lower) * grid / (upper - lower); all axes below -> the underflow
+ * slot nbins, all above grid -> the overflow slot nbins+1, otherwise the
+ * row-major index of size_t(X).  Where the reference's conversion or index is
+ * undefined (an axis at or below -1 or NaN; an index past nbins+1) the event
+ * is dropped: returns UINT64_MAX.                                           */
+uint64_t po_hist_key(int ndim, const uint64_t* grid, const double* lower, const double* upper,
+                     const double* x) {
+  uint64_t nbins = 1;
+  for (int i = 0; i < ndim; ++i) nbins *= grid[i];
+  double X[8];
+  int under = 1, over = 1;
+  for (int i = 0; i < ndim; ++i) {
+    double delta = upper[i] - lower[i];
+    X[i] = (x[i] - lower[i]) * (double)grid[i] / delta;
+    under = under && (X[i] < 0.0);
+    over = over && (X[i] > (double)grid[i]);
+  }
+  if (under) return nbins;
+  if (over) return nbins + 1;
+  uint64_t key = 0;
+  for (int i = 0; i < ndim; ++i) {
+    if (!(X[i] > -1.0) || X[i] >= 18446744073709551616.0) return UINT64_MAX;
+    uint64_t idx = (uint64_t)X[i];  /* truncation; X in (-1, 0) gives 0 */
+    uint64_t stride = 1;
+    for (int j = ndim - 1; j > i; --j) stride *= grid[j];
+    key += stride * idx;
+  }
+  return key <= nbins + 1 ? key : UINT64_MAX;
+}
+
+typedef struct {
+  uint64_t key;
+  uint64_t idx;
+} po_key_idx;
+
+static int cmp_key_idx(const void* a, const void* b) {
+  const po_key_idx* x = (const po_key_idx*)a;
+  const po_key_idx* y = (const po_key_idx*)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+/* DenseHistogram::Fill (DenseHistogram.inl:49-105, 174-222): keys, a stable
+ * sort by key, reduce_by_key of the weights (1.0 each without weights) in
+ * that order, scattered into nbins + 2 zeroed contents.  x: ndim columns of
+ * n values (column d at x + d*ldx).                                          */
+void po_histogram_fill(int ndim, const uint64_t* grid, const double* lower, const double* upper,
+                       const double* x, uint64_t ldx, uint64_t n, const double* w,
+                       double* contents) {
+  uint64_t nbins = 1;
+  for (int i = 0; i < ndim; ++i) nbins *= grid[i];
+  for (uint64_t b = 0; b < nbins + 2; ++b) contents[b] = 0.0;
+  po_key_idx* ki = (po_key_idx*)malloc(sizeof(po_key_idx) * (n ? n : 1));
+  uint64_t m = 0;
+  for (uint64_t k = 0; k < n; ++k) {
+    double v[8];
+    for (int d = 0; d < ndim; ++d) v[d] = x[(uint64_t)d * ldx + k];
+    uint64_t key = po_hist_key(ndim, grid, lower, upper, v);
+    if (key == UINT64_MAX) continue;
+    ki[m].key = key;
+    ki[m].idx = k;
+    ++m;
+  }
+  qsort(ki, m, sizeof(po_key_idx), cmp_key_idx);
+  for (uint64_t a = 0; a < m;) {
+    uint64_t b = a;
+    double s = w ? w[ki[a].idx] : 1.0;
+    for (b = a + 1; b < m && ki[b].key == ki[a].key; ++b) s += w ? w[ki[b].idx] : 1.0;
+    contents[ki[a].key] = s;
+    a = b;
+  }
+  free(ki);
+}

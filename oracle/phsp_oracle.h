@@ -80,6 +80,11 @@ void po_evaluate(int N, int engine, const double* masses, double ecm, double max
                  const double* beta3, uint64_t seed, uint64_t first, uint64_t n, int nf,
                  const int* fids, const double* fpar, double* out, int nthreads);
 uint64_t po_pair_hash(uint64_t evt, uint64_t seed);
+uint64_t po_hist_key(int ndim, const uint64_t* grid, const double* lower, const double* upper,
+                     const double* x);
+void po_histogram_fill(int ndim, const uint64_t* grid, const double* lower, const double* upper,
+                       const double* x, uint64_t ldx, uint64_t n, const double* w,
+                       double* contents);
 void po_evaluate_mothers(int N, int engine, const double* masses, uint64_t seed,
                          uint64_t first, uint64_t n, const double* mother_cols, int nf,
                          const int* fids, const double* fpar, double* out, int nthreads);

@@ -21,6 +21,7 @@
 #include <hydra/device/System.h>
 #include <hydra/Decays.h>
 #include <hydra/multivector.h>
+#include <hydra/DenseHistogram.h>
 #include <hydra/detail/Hash.h>
 
 #include <omp.h>
@@ -583,6 +584,47 @@ int hr_evaluate_mothers(int N, double M, const double* masses, int32_t seed, uin
                         double* out) {
   if (nf < 1 || nf > 2) return -3;
   return dispatch<EvalMothersOp>(N, M, masses, seed, n, mothers, nf, fids, fpar, out);
+}
+
+// DenseHistogram<double, ndim>::Fill(data[, weights]) with ndim 1 or 2: data
+// x holds ndim columns of n values; contents receives GetBinContent(b) for
+// b = 0 .. nbins + 1 (the last two are the underflow and overflow slots).
+int hr_histogram_fill(int ndim, const uint64_t* grid, const double* lower, const double* upper,
+                      const double* x, uint64_t n, const double* w, double* contents,
+                      double* seconds) {
+  if (ndim == 1) {
+    hydra::DenseHistogram<double, 1, hydra::device::sys_t> H((size_t)grid[0], lower[0], upper[0]);
+    hydra::device::vector<double> data(x, x + n);
+    double t0 = now_s();
+    if (w) {
+      hydra::device::vector<double> wv(w, w + n);
+      H.Fill(data.begin(), data.end(), wv.begin());
+    } else {
+      H.Fill(data.begin(), data.end());
+    }
+    if (seconds) *seconds = now_s() - t0;
+    for (uint64_t b = 0; b < grid[0] + 2; ++b) contents[b] = H.GetBinContent((size_t)b);
+    return 0;
+  }
+  if (ndim == 2) {
+    std::array<size_t, 2> g{(size_t)grid[0], (size_t)grid[1]};
+    std::array<double, 2> lo{lower[0], lower[1]}, hi{upper[0], upper[1]};
+    hydra::DenseHistogram<double, 2, hydra::device::sys_t> H(g, lo, hi);
+    hydra::multivector<hydra::tuple<double, double>, hydra::device::sys_t> data(n);
+    for (uint64_t i = 0; i < n; ++i) data[i] = hydra::make_tuple(x[i], x[n + i]);
+    double t0 = now_s();
+    if (w) {
+      hydra::device::vector<double> wv(w, w + n);
+      H.Fill(data.begin(), data.end(), wv.begin());
+    } else {
+      H.Fill(data.begin(), data.end());
+    }
+    if (seconds) *seconds = now_s() - t0;
+    const uint64_t nb = grid[0] * grid[1];
+    for (uint64_t b = 0; b < nb + 2; ++b) contents[b] = H.GetBinContent((size_t)b);
+    return 0;
+  }
+  return -1;
 }
 
 }  // extern "C"

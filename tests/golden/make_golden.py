@@ -141,6 +141,26 @@ def main():
     p5 = DECAYS["n12"]
     out["eval_n12_m2"], _ = R.evaluate(p5[0], p5[1], SEED, 256, [(3, (4, 11)), (4, (3, 7, 0, 12))])
 
+    # DenseHistogram fills of the Dalitz variables (m^2_AB, m^2_BC) of the 4096
+    # generated events, weighted by PhaseSpaceWeight and unweighted, and a
+    # one-dimensional fill with the edge cases (upper limit, just below the
+    # lower limit, far outside)
+    pg = out["gen_c1_p"]
+    sab, sbc = pg[:, 0] + pg[:, 1], pg[:, 1] + pg[:, 2]
+
+    def m2(v):
+        return v[:, 0] * v[:, 0] - v[:, 1] * v[:, 1] - v[:, 2] * v[:, 2] - v[:, 3] * v[:, 3]
+    dal = np.vstack([m2(sab), m2(sbc)])
+    lo = [(m[0] + m[1]) ** 2, (m[1] + m[2]) ** 2]
+    hi = [(M - m[2]) ** 2, (M - m[0]) ** 2]
+    out["hist_dalitz_x"] = dal
+    out["hist_dalitz_limits"] = np.array([lo, hi])
+    out["hist_dalitz_w"], _ = R.histogram_fill([20, 20], lo, hi, dal, out["gen_c1_w"])
+    out["hist_dalitz_n"], _ = R.histogram_fill([20, 20], lo, hi, dal)
+    x1 = np.concatenate([dal[0, :1000], [hi[0], lo[0] - 1e-9, lo[0] - 100.0, hi[0] + 5.0]])
+    out["hist_1d_x"] = x1
+    out["hist_1d_n"], _ = R.histogram_fill([25], [lo[0]], [hi[0]], x1[None, :])
+
     np.savez_compressed(os.path.join(HERE, "phsp_golden.npz"), **out)
     print("wrote", os.path.join(HERE, "phsp_golden.npz"), len(out), "arrays")
 

@@ -115,3 +115,34 @@ def test_evaluate_validates_before_touching_the_device():
     # ... and n = 0 is a no-op
     assert L.hpsp_evaluate(ctypes.byref(d), one, 1, 1, 0, 0, None, 0, None) == 0
     assert L.hpsp_evaluate_mothers(ctypes.byref(d), ok, 1, 1, None, 0, 0, 0, None, 0, None) == 0
+
+
+def test_histogram_validates_before_touching_the_device():
+    L = capi.lib()
+    out = ctypes.c_void_p(64)
+    good = capi.make_histogram([10, 10], [0, 0], [1, 1])
+    assert L.hpsp_histogram_bins(good) == 100
+    for bad in (capi.make_histogram([], [], []),
+                capi.make_histogram([10, 0], [0, 0], [1, 1]),
+                capi.make_histogram([10], [1.0], [1.0]),
+                capi.make_histogram([10], [2.0], [1.0]),
+                capi.make_histogram([2**21, 2**21], [0, 0], [1, 1])):
+        assert L.hpsp_histogram_bins(bad) == 0
+        assert L.hpsp_histogram_fill(bad, out, 10, 10, None, out, None) == 1
+    h5 = capi.make_histogram([2] * 4, [0] * 4, [1] * 4)
+    h5.ndim = 5
+    assert L.hpsp_histogram_fill(h5, out, 10, 10, None, out, None) == 1
+    assert L.hpsp_histogram_fill(good, None, 10, 10, None, out, None) == 1   # null data
+    assert L.hpsp_histogram_fill(good, out, 5, 10, None, out, None) == 1     # ldx < n
+    assert L.hpsp_histogram_fill(good, out, 10, 10, None, None, None) == 1   # null contents
+    d = capi.make_decay(5.279, [3.0969, 0.493677, 0.13957])
+    axes = capi.functor_array([capi.make_functor(capi.F_MASS2, (0, 1)),
+                               capi.make_functor(capi.F_MASS2, (1, 3))])  # index 3 > N-1
+    assert L.hpsp_histogram_fill_events(good, ctypes.byref(d), axes, out, 10, 10, 0, None, out,
+                                        None) == 1
+    axes = capi.functor_array([capi.make_functor(capi.F_MASS2, (0, 1)),
+                               capi.make_functor(capi.F_MASS2, (1, 2))])
+    assert L.hpsp_histogram_fill_events(good, ctypes.byref(d), axes, out, 10, 10, 3, None, out,
+                                        None) == 1  # weight mode
+    assert L.hpsp_histogram_fill_events(good, ctypes.byref(d), axes, out, 10, 10,
+                                        capi.HIST_WEIGHT_COLUMN, None, out, None) == 1

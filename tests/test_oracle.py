@@ -365,3 +365,57 @@ def test_evaluate_equals_reference(oracle, reference):
         Fm = [(3, (1, N)), (4, (1, N, 1, N))]
         ref = reference.evaluate_mothers(M, m, 4321, mt, Fm)
         assert np.array_equal(bits(oracle.evaluate_mothers(m, 4321, mt, Fm)), bits(ref))
+
+
+# ---------------------------------------------------------------- DenseHistogram
+def test_histogram_golden(oracle, golden):
+    x = golden["hist_dalitz_x"]
+    lo, hi = golden["hist_dalitz_limits"]
+    got = oracle.histogram_fill([20, 20], lo, hi, x, golden["gen_c1_w"])
+    assert np.array_equal(bits(got), bits(golden["hist_dalitz_w"]))
+    got = oracle.histogram_fill([20, 20], lo, hi, x)
+    assert np.array_equal(got, golden["hist_dalitz_n"])
+    got = oracle.histogram_fill([25], [lo[0]], [hi[0]], golden["hist_1d_x"][None, :])
+    assert np.array_equal(got, golden["hist_1d_n"])
+
+
+def test_histogram_keys_closed_form(oracle):
+    """GetGlobalBin by hand: row-major keys, the all-below / all-above slots,
+    the upper edge's index grid, truncation of (-1, 0) to 0, undefined cases
+    dropped."""
+    g, lo, hi = [4, 5], [0.0, 10.0], [2.0, 20.0]
+    nb = 20
+    assert oracle.hist_key(g, lo, hi, [0.1, 10.1]) == 0
+    assert oracle.hist_key(g, lo, hi, [1.9, 19.9]) == 3 * 5 + 4
+    assert oracle.hist_key(g, lo, hi, [-1.0, 5.0]) == nb          # all below
+    assert oracle.hist_key(g, lo, hi, [3.0, 30.0]) == nb + 1      # all above
+    assert oracle.hist_key(g, lo, hi, [-0.2, 15.0]) == 2          # X0 in (-1, 0) -> 0
+    assert oracle.hist_key(g, lo, hi, [2.0, 10.0]) == 4 * 5       # X0 == grid -> slot nb
+    assert oracle.hist_key(g, lo, hi, [-0.6, 15.0]) == 2**64 - 1  # X0 = -1.2: undefined
+    assert oracle.hist_key(g, lo, hi, [float("nan"), 15.0]) == 2**64 - 1
+    assert oracle.hist_key(g, lo, hi, [1.0, 40.0]) == 2**64 - 1   # key past nb + 1
+    # 1-D: the upper limit itself lands in the underflow slot (index == grid)
+    assert oracle.hist_key([10], [0.0], [1.0], [1.0]) == 10
+    # sums: unit weights count the events; weights sum per bin
+    rng = np.random.default_rng(0)
+    x = rng.uniform([0, 10], [2, 20], (5000, 2)).T
+    w = rng.uniform(0, 1, 5000)
+    h = oracle.histogram_fill(g, lo, hi, x, w)
+    assert abs(h.sum() - w.sum()) < 1e-9 and h[nb] == 0 and h[nb + 1] == 0
+    assert oracle.histogram_fill(g, lo, hi, x).sum() == 5000
+
+
+def test_histogram_equals_reference(oracle, reference):
+    rng = np.random.default_rng(5)
+    for grid in ([30], [17, 9]):
+        nd = len(grid)
+        lo = rng.uniform(-1, 0, nd)
+        hi = lo + rng.uniform(1, 2, nd)
+        x = rng.uniform(lo - 0.01 * (hi - lo), hi + 0.2 * (hi - lo), (20000, nd)).T
+        w = rng.exponential(1.0, 20000)
+        # keep to events whose key the reference defines
+        ok = np.array([oracle.hist_key(grid, lo, hi, x[:, k]) != 2**64 - 1 for k in range(20000)])
+        x, w = np.ascontiguousarray(x[:, ok]), w[ok]
+        for ww in (w, None):
+            ref, _ = reference.histogram_fill(grid, lo, hi, x, ww)
+            assert np.array_equal(bits(oracle.histogram_fill(grid, lo, hi, x, ww)), bits(ref))
