@@ -14,6 +14,7 @@
 #include <vector>
 
 #include <hydra/Decays.h>
+#include <hydra/DenseHistogram.h>
 #include <hydra/Function.h>
 #include <hydra/PhaseSpace.h>
 #include <hydra/Vector4R.h>
@@ -189,6 +190,26 @@ int main(int argc, char** argv) {
       emrows.push_back(std::get<1>(row).Value());
     }
     emit_array("evaluate_mothers_rows", emrows);
+
+    // Dalitz plot of the 4096 events (phsp_basic.inl:196-214): the fused fill
+    // with the container's weight column, and with PhaseSpaceWeight computed
+    // inside the fill, and from Evaluate's value columns
+    const std::array<double, 2> lo{(A_mass + B_mass) * (A_mass + B_mass),
+                                   (B_mass + C_mass) * (B_mass + C_mass)};
+    const std::array<double, 2> hi{(P_mass - C_mass) * (P_mass - C_mass),
+                                   (P_mass - A_mass) * (P_mass - A_mass)};
+    auto dalitz_variables =
+        Events | hydra::make_axes(hydra::InvariantMass2{0, 1}, hydra::InvariantMass2{1, 2});
+    auto dalitz_weights = Events | Events.GetEventWeightFunctor();
+    auto Hist_Dalitz = hydra::make_dense_histogram<double, 2>(
+        hydra::device::sys, {20, 20}, lo, hi, dalitz_variables, dalitz_weights);
+    emit_array("hist_dalitz_w", Hist_Dalitz.GetBinsContents());
+    hydra::DenseHistogram<double, 2, hydra::device::sys_t> H2({20, 20}, lo, hi);
+    H2.Fill(dalitz_variables, Events.GetEventWeightFunctor());
+    emit_array("hist_dalitz_phsp", H2.GetBinsContents());
+    H2.Fill(dalitz_variables);
+    emit_array("hist_dalitz_n", H2.GetBinsContents());
+    std::printf("\"hist_bin_3_7\": %.17g,\n", H2.GetBinContent(std::array<size_t, 2>{3, 7}));
   }
   std::printf("\"ok\": 1\n}\n");
   return 0;

@@ -93,3 +93,12 @@ def test_gpu_generate_unweight_average(golden, oracle):
     exp = em[:, [0, 100, 255]]
     assert np.array_equal(got[0], exp[0])
     assert np.max(np.abs(got[1] - exp[1]) / (M * M)) <= 1e-12
+    # Dalitz DenseHistogram (phsp_basic.inl:196-214) of the same 4096 events:
+    # bins as the reference's (events sit far from bin edges compared with
+    # the 1e-12 momentum tolerance), weights as gen_weights above
+    assert np.array_equal(np.array(r["hist_dalitz_n"]), golden["hist_dalitz_n"])
+    hw = golden["hist_dalitz_w"]
+    for key in ("hist_dalitz_w", "hist_dalitz_phsp"):
+        got = np.array(r[key])
+        assert np.all(np.abs(got - hw) <= 1e-10 * hw), key
+    assert r["hist_bin_3_7"] == golden["hist_dalitz_n"][3 * 20 + 7]
