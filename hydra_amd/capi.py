@@ -50,6 +50,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("w", ctypes.c_double), ("mean", ctypes.c_double), ("m2", ctypes.c_double)]
 
 
+ORDER_STABLE, ORDER_SEQUENTIAL = 0, 1
+UNWEIGHT_SAMPLE_SEED = 0x8EC74D321E6B5A27
 HIST_MAX_DIM = 4
 HIST_WEIGHT_ONE, HIST_WEIGHT_COLUMN, HIST_WEIGHT_PHSP = 0, 1, 2
 
@@ -135,6 +137,9 @@ _SIGS = {
                            _P, _U64, _P]),
     "hpsp_evaluate_mothers": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _I, _U64, _P,
                                    _U64, _U64, _U64, _P, _U64, _P]),
+    "hpsp_unweight_sample_workspace_size": (_SZ, [_U64]),
+    "hpsp_unweight_sample": (_I, [_I, _P, _U64, _U64, _P, _D, _U64, _U64, _I, _P, _U64, _P, _P,
+                                  _P, _SZ, _P, _P]),
     "hpsp_histogram_bins": (_U64, [ctypes.POINTER(Histogram)]),
     "hpsp_histogram_fill": (_I, [ctypes.POINTER(Histogram), _P, _U64, _U64, _P, _P, _P]),
     "hpsp_histogram_fill_events": (_I, [ctypes.POINTER(Histogram), ctypes.POINTER(Decay),

@@ -397,6 +397,53 @@ int hpsp_unweight(const hpsp_decay* d, const hpsp_functor* f, double max_value, 
   return hip_status(launch_unweight(d->n, a, s));
 }
 
+size_t hpsp_unweight_sample_workspace_size(uint64_t n) {
+  return uw_layout(n).total + align_up((n ? n : 1) * sizeof(uint64_t));
+}
+
+int hpsp_unweight_sample(int ncols, const double* cols_in, uint64_t ld_in, uint64_t n,
+                         const double* w, double max_pdf, uint64_t seed, uint64_t jump,
+                         int order, double* cols_out, uint64_t ld_out, uint64_t* idx_out,
+                         double* w_out, void* ws, size_t ws_bytes, uint64_t* n_acc_dev,
+                         void* stream) {
+  if (ncols < 0 || !n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (order != HPSP_ORDER_STABLE && order != HPSP_ORDER_SEQUENTIAL)
+    return HPSP_ERR_INVALID_ARGUMENT;
+  if (n > 0) {
+    if (!w) return HPSP_ERR_INVALID_ARGUMENT;
+    if (cols_out && (ncols == 0 || !cols_in || ld_in < n || ld_out < n))
+      return HPSP_ERR_INVALID_ARGUMENT;
+  }
+  const UwLayout L = uw_layout(n);
+  if (!ws || ws_bytes < hpsp_unweight_sample_workspace_size(n)) return HPSP_ERR_WORKSPACE;
+  char* base = static_cast<char*>(ws);
+  UnweightArgs a;
+  memset(&a, 0, sizeof(a));
+  a.rule = 1;
+  a.order = order;
+  a.tidx = reinterpret_cast<uint64_t*>(base + L.total);
+  a.jump = jump;
+  a.maxv = max_pdf;
+  uint64_t s0 = seed;
+  a.uw_key = splitmix64_host(&s0);
+  a.cols_in = cols_in;
+  a.ld_in = ld_in;
+  a.n = n;
+  a.w_in = w;
+  a.cols_out = cols_out;
+  a.ld_out = ld_out;
+  a.idx_out = idx_out;
+  a.w_out = w_out;
+  a.flags = reinterpret_cast<uint64_t*>(base + L.flags);
+  a.tile_count = reinterpret_cast<uint32_t*>(base + L.tile_count);
+  a.tile_off = reinterpret_cast<uint64_t*>(base + L.tile_off);
+  a.run_sum = reinterpret_cast<uint64_t*>(base + L.run_sum);
+  a.n_acc = n_acc_dev;
+  double* mv = max_pdf > 0.0 ? nullptr : reinterpret_cast<double*>(base + L.maxv);
+  double* bm = reinterpret_cast<double*>(base + L.block_max);
+  return hip_status(launch_unweight_sample(ncols, a, bm, grid_for(n), mv, as_stream(stream)));
+}
+
 size_t hpsp_average_on_workspace_size(uint64_t n) {
   const uint64_t m = (n + kAvgChunk - 1) / kAvgChunk;
   return align_up((m ? m : 1) * sizeof(Stats));

@@ -101,6 +101,10 @@ struct UnweightArgs {
   uint64_t* tile_off;   // exclusive prefix of tile_count
   uint64_t* run_sum;    // per-1024-tile sums, then their exclusive prefix
   uint64_t* n_acc;      // total accepted (device)
+  int rule;             // 0: w / max > u (Decays::Unweight); 1: u * max <= w (RndFlag)
+  uint64_t jump;        // counter offset of the uniforms (rule 1)
+  int order;            // 0: both parts stable; 1: sequential swap partition
+  uint64_t* tidx;       // order 1: original index of the r-th kept sample
 };
 
 // Kernel launchers; each returns a hipError_t value.
@@ -115,6 +119,9 @@ int launch_event_weight(int N, const WeightArgs& a, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s);
+// hydra::unweight over ncols value columns and a weight column (rule 1).
+int launch_unweight_sample(int ncols, const UnweightArgs& a, double* block_max,
+                           uint64_t nblocks, double* max_dev, hipStream_t s);
 int launch_rng_words(int engine, const uint64_t* key, uint64_t discard, uint64_t n,
                      uint64_t* out, hipStream_t s);
 

@@ -70,8 +70,10 @@ __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
         w = a.w_in ? a.w_in[k] : event_weight_f64<N>(p, a.d.m, a.d.maxw);
         if (a.use_f) w = w * functor_eval<double>(a.f, p);
       }
-      const double u = (double)squares3(a.uw_key, k) * 0x1p-32;
-      acc = (w / maxv) > u;
+      const double u = (double)squares3(a.uw_key, a.jump + k) * 0x1p-32;
+      // Decays::Unweight (FlagDaugthers, Decays.inl:75-89): w / max > u;
+      // hydra::unweight (RndFlag, RandomUtils.h:254-262): u * max <= w
+      acc = a.rule == 0 ? (w / maxv) > u : (u * maxv) <= w;
     }
     const uint64_t ball = __ballot(acc);
     if (lane == 0) *(glob(a.flags) + g) = ball;
@@ -243,6 +245,177 @@ __global__ __launch_bounds__(kBlock) void k_uw_scatter(const UnweightArgs a) {
   }
 }
 
+// Destinations of one group of 64 samples in the stable partition: kept lanes
+// to goff + (kept lanes below), the others to nacc + (k0 - goff) + (rejected
+// lanes below).
+__device__ __forceinline__ uint64_t stable_dst(uint64_t fw, uint64_t below, int lane,
+                                               uint64_t goff, uint64_t nacc, uint64_t k0,
+                                               bool* kept) {
+  const uint32_t acc_below = (uint32_t)__popcll(fw & below);
+  *kept = (fw >> lane) & 1ULL;
+  return *kept ? goff + acc_below : nacc + k0 - goff + ((uint32_t)lane - acc_below);
+}
+
+// Group offsets of a tile in LDS (lanes 0..15 of wave 0 scan the popcounts).
+__device__ __forceinline__ void tile_group_offsets(const UnweightArgs& a, uint64_t tile,
+                                                   uint64_t* goff) {
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  const uint64_t ngroups = kUwTile / kWave;
+  if (wave == 0) {
+    const uint64_t gi = tile * ngroups + (uint64_t)lane;
+    const uint32_t own =
+        (lane < (int)ngroups && gi * kWave < a.n) ? (uint32_t)__popcll(a.flags[gi]) : 0u;
+    uint32_t c = own;
+#pragma unroll
+    for (int off = 1; off < (int)ngroups; off <<= 1) {
+      const uint32_t o = __shfl_up(c, off, kWave);
+      if (lane >= off) c += o;
+    }
+    if (lane < (int)ngroups) goff[lane] = a.tile_off[tile] + (c - own);
+  }
+  __syncthreads();
+}
+
+// Order 1: the original index of every kept sample by its rank.
+__global__ __launch_bounds__(kBlock) void k_uw_kept_index(const UnweightArgs a) {
+  __shared__ uint64_t goff[kUwTile / kWave];
+  const uint64_t tile = blockIdx.x;
+  tile_group_offsets(a, tile, goff);
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+  const uint64_t nacc = *a.n_acc;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint64_t gl = (uint64_t)j * (kBlock / kWave) + wave;
+    const uint64_t g = tile * (kUwTile / kWave) + gl;
+    const uint64_t k = g * kWave + (uint64_t)lane;
+    if (k >= a.n) continue;
+    bool kept;
+    const uint64_t dst = stable_dst(a.flags[g], below, lane, goff[gl], nacc, g * kWave, &kept);
+    if (kept) a.tidx[dst] = k;
+  }
+}
+
+// Order 1 destination of a rejected sample e: in place from nacc on; below
+// nacc it follows the swaps of the sequential partition: to the position of
+// the kept sample of rank e, then of rank that position, while below nacc
+// (each step strictly increases the position).
+__device__ __forceinline__ uint64_t swap_chain_dst(const UnweightArgs& a, uint64_t e,
+                                                   uint64_t nacc) {
+  if (e >= nacc) return e;
+  uint64_t p = a.tidx[e];
+  // at most nacc steps; every index read is a kept rank below nacc
+  for (uint64_t it = 0; p < nacc && it < nacc; ++it) p = a.tidx[p];
+  return p;
+}
+
+// hydra::unweight's scatter of ncols value columns (any count): the same
+// destinations as k_uw_scatter (or the sequential-partition ones), columns
+// moved in batches of eight with all four groups' loads of a batch in flight.
+__global__ __launch_bounds__(kBlock) void k_uw_scatter_cols(const UnweightArgs a, int ncols) {
+  __shared__ uint64_t goff[kUwTile / kWave];
+  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
+  const uint64_t per_xcd = (ntiles + kXcds - 1) / kXcds;
+  const uint64_t tile = (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (tile >= ntiles) return;
+  const int lane = __lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t ngroups = kUwTile / kWave;
+  const uint64_t tile_base = a.tile_off[tile];
+  const uint64_t nacc = *a.n_acc;
+  if (wave == 0) {
+    const uint64_t gi = tile * ngroups + (uint64_t)lane;
+    const uint32_t own =
+        (lane < (int)ngroups && gi * kWave < a.n) ? (uint32_t)__popcll(a.flags[gi]) : 0u;
+    uint32_t c = own;
+#pragma unroll
+    for (int off = 1; off < (int)ngroups; off <<= 1) {
+      const uint32_t o = __shfl_up(c, off, kWave);
+      if (lane >= off) c += o;
+    }
+    if (lane < (int)ngroups) goff[lane] = tile_base + (c - own);
+  }
+  __syncthreads();
+  const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+  uint64_t dst[kItems], src[kItems], k0s[kItems];
+  bool live[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint64_t gl = (uint64_t)j * (kBlock / kWave) + wave;
+    const uint64_t g = tile * ngroups + gl;
+    const uint64_t k0 = g * kWave;
+    k0s[j] = k0;
+    src[j] = k0 + (uint64_t)lane;
+    live[j] = src[j] < a.n;
+    dst[j] = 0;
+    if (k0 < a.n) {
+      bool kept;
+      dst[j] = stable_dst(a.flags[g], below, lane, goff[gl], nacc, k0, &kept);
+      if (a.order == 1 && !kept && live[j]) dst[j] = swap_chain_dst(a, src[j], nacc);
+    }
+  }
+  constexpr int B = 8;
+  for (int c0 = 0; c0 < ncols; c0 += B) {
+    double v[kItems][B];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+#pragma unroll
+      for (int c = 0; c < B; ++c)
+        if (live[j] && c0 + c < ncols)
+          v[j][c] = *(glob_u(a.cols_in + (uint64_t)(c0 + c) * a.ld_in + k0s[j]) + lane);
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+#pragma unroll
+      for (int c = 0; c < B; ++c)
+        if (live[j] && c0 + c < ncols)
+          *(glob_u(a.cols_out + (uint64_t)(c0 + c) * a.ld_out) + dst[j]) = v[j][c];
+  }
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    if (live[j]) {
+      if (a.idx_out) *(glob(a.idx_out) + dst[j]) = src[j];
+      if (a.w_out) *(glob(a.w_out) + dst[j]) = a.w_in[src[j]];
+    }
+  }
+}
+
+// Largest value of a column (max_element's value), per block.
+__global__ __launch_bounds__(kBlock) void k_max_column(const double* w, uint64_t n,
+                                                       double* block_max) {
+  __shared__ double sh[kBlock];
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  double mx = -__builtin_inf();
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += stride)
+    mx = (mx < w[k]) ? w[k] : mx;
+  sh[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      sh[threadIdx.x] = (sh[threadIdx.x] < sh[threadIdx.x + off]) ? sh[threadIdx.x + off]
+                                                                  : sh[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) block_max[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_max_of_blocks(const double* block_max, uint64_t m,
+                                                          double* out) {
+  __shared__ double sh[kBlock];
+  double mx = -__builtin_inf();
+  for (uint64_t i = threadIdx.x; i < m; i += kBlock) mx = (mx < block_max[i]) ? block_max[i] : mx;
+  sh[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      sh[threadIdx.x] = (sh[threadIdx.x] < sh[threadIdx.x + off]) ? sh[threadIdx.x + off]
+                                                                  : sh[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
 __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
 
 constexpr int kNumN = HPSP_MAX_N - 1;
@@ -268,6 +441,37 @@ static constexpr auto uw_table(std::integer_sequence<int, I...>) {
   return std::array<void (*)(const UnweightArgs&, hipStream_t), sizeof...(I)>{&uw_one<2 + I>...};
 }
 static const auto kUw = uw_table(std::make_integer_sequence<int, kNumN>{});
+
+int launch_unweight_sample(int ncols, const UnweightArgs& in, double* block_max,
+                           uint64_t nblocks, double* max_dev, hipStream_t s) {
+  if (in.n == 0) {
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, in.n_acc, (uint64_t)0);
+    return (int)hipGetLastError();
+  }
+  UnweightArgs a = in;
+  if (max_dev) {  // max_element of the weights
+    hipLaunchKernelGGL(k_max_column, dim3((unsigned)nblocks), dim3(kBlock), 0, s, a.w_in, a.n,
+                       block_max);
+    hipLaunchKernelGGL(k_max_of_blocks, dim3(1), dim3(kBlock), 0, s, block_max, nblocks, max_dev);
+    a.maxv_dev = max_dev;
+  }
+  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
+  hipLaunchKernelGGL(k_uw_flags<2>, dim3((unsigned)ntiles), dim3(kBlock), 0, s, a);
+  const uint64_t nruns = (ntiles + kScanRun - 1) / kScanRun;
+  hipLaunchKernelGGL(k_uw_scan_blocks, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
+                     ntiles, a.run_sum);
+  hipLaunchKernelGGL(k_uw_scan_top, dim3(1), dim3(kBlock), 0, s, a.run_sum, nruns, a.n_acc);
+  hipLaunchKernelGGL(k_uw_scan_tiles, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
+                     ntiles, a.run_sum, a.tile_off);
+  if (a.cols_out || a.idx_out || a.w_out) {
+    if (a.order == 1)
+      hipLaunchKernelGGL(k_uw_kept_index, dim3((unsigned)ntiles), dim3(kBlock), 0, s, a);
+    const uint64_t grid = (ntiles + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL(k_uw_scatter_cols, dim3((unsigned)grid), dim3(kBlock), 0, s, a,
+                       a.cols_out ? ncols : 0);
+  }
+  return (int)hipGetLastError();
+}
 
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s) {
   if (a.n == 0) {

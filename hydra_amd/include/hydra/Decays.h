@@ -308,6 +308,46 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     fCap = cap;
   }
 
+  // hydra::unweight(device::sys, events, functor, ...) (Random.inl:134-230):
+  // u * max <= w with w the functor's values, kept events first; the
+  // rejected ones in the order of the reference's sequential partition.
+  // Permutes the container out of place.
+ public:
+  Range<iterator> UnweightBy(EventWeights const& w, double max_pdf, size_t rng_seed,
+                             size_t rng_jump) {
+    const size_t n = fSize;
+    if (w.size() < n) throw std::invalid_argument("[hydra::unweight]: fewer weights than events");
+    detail::device_buffer<double> out(4 * N * (n ? n : 1));
+    detail::device_buffer<double> wout(n ? n : 1);
+    const size_t wsb = hpsp_unweight_sample_workspace_size(n);
+    detail::device_buffer<unsigned char> ws(wsb);
+    detail::device_buffer<uint64_t> nacc(1);
+    const double* wc = weights_cache();
+    // the container's own PhaseSpaceWeight column moves with the events
+    detail::hpsp_check(hpsp_unweight_sample(4 * (int)N, fCols.data(), fCap, n, w.data(), max_pdf,
+                                            (uint64_t)rng_seed, (uint64_t)rng_jump,
+                                            HPSP_ORDER_SEQUENTIAL, out.data(), n ? n : 1,
+                                            nullptr, nullptr, ws.data(), wsb, nacc.data(),
+                                            nullptr),
+                       "hydra::unweight");
+    if (wc) {
+      // reorder the cached weights with the same permutation (as a one-column sample)
+      detail::hpsp_check(hpsp_unweight_sample(1, wc, n, n, w.data(), max_pdf, (uint64_t)rng_seed,
+                                              (uint64_t)rng_jump, HPSP_ORDER_SEQUENTIAL,
+                                              wout.data(), n ? n : 1, nullptr, nullptr,
+                                              ws.data(), wsb, nacc.data(), nullptr),
+                         "hydra::unweight");
+    }
+    uint64_t acc = 0;
+    detail::hpsp_check(hpsp_memcpy_to_host(&acc, nacc.data(), sizeof(acc)), "hydra::unweight");
+    fCols.swap(out);
+    fW.swap(wout);
+    fWValid = fWValid && wc != nullptr;
+    fCap = n ? n : 1;
+    return Range<iterator>(begin(), begin() + (std::ptrdiff_t)acc);
+  }
+
+ private:
   Range<iterator> unweight(const hpsp_decay& d, const hpsp_functor* f, double max_value,
                            size_t seed) {
     const size_t n = fSize;
@@ -366,6 +406,19 @@ EventWeights operator|(const Decays<hydra::tuple<Particles...>, hydra::device::s
     detail::hpsp_sync("PhaseSpaceReweight");
   }
   return out;
+}
+
+// hydra::unweight(device::sys, events, functor[, max_pdf, rng_seed, rng_jump])
+// (hydra/Random.h:133-330, detail/Random.inl:134-230), as
+// examples/phase_space/phsp_unweighting_functor.inl:209 calls it with
+// events.GetEventWeightFunctor(f): the functor's values weight the events.
+template <typename... Particles, typename W>
+auto unweight(hydra::device::sys_t const&,
+              Decays<hydra::tuple<Particles...>, hydra::device::sys_t>& events, W const& functor,
+              double max_pdf = -1.0, size_t rng_seed = 0x8ec74d321e6b5a27, size_t rng_jump = 0)
+    -> decltype(events.UnweightBy(events | functor, max_pdf, rng_seed, rng_jump)) {
+  EventWeights w = events | functor;
+  return events.UnweightBy(w, max_pdf, rng_seed, rng_jump);
 }
 
 }  // namespace hydra

@@ -199,6 +199,27 @@ int hpsp_unweight(const hpsp_decay* d, const hpsp_functor* f, double max_value, 
 /* ---- integration -------------------------------------------------------- */
 
 /* Workspace bytes needed by hpsp_average_on for n events. */
+/* hydra::unweight(policy, data, functor[, max_pdf, rng_seed, rng_jump])
+ * (hydra/detail/Random.inl:134-230, defaults in hydra/Random.h:133-330) with
+ * the functor's values precomputed in the column w: keeps sample k iff
+ * u_k * max <= w_k (RndFlag, RandomUtils.h:254-262) with u_k the uniform of
+ * word jump + k of squares3(seed) and max = max_pdf if > 0, else the largest
+ * weight.  Kept samples come first, in their original order.  The rejected
+ * ones follow in their original order (HPSP_ORDER_STABLE) or in the order the
+ * reference's thrust::partition leaves them on its OMP backend, the
+ * sequential swap partition (HPSP_ORDER_SEQUENTIAL).  ncols value columns
+ * move with their samples (cols_out may be NULL); idx_out / w_out optional.
+ * The reference's defaults: seed 0x8ec74d321e6b5a27, jump 0.               */
+typedef enum { HPSP_ORDER_STABLE = 0, HPSP_ORDER_SEQUENTIAL = 1 } hpsp_partition_order;
+
+size_t hpsp_unweight_sample_workspace_size(uint64_t n);
+
+int hpsp_unweight_sample(int ncols, const double* cols_in, uint64_t ld_in, uint64_t n,
+                         const double* w, double max_pdf, uint64_t seed, uint64_t jump,
+                         int order, double* cols_out, uint64_t ld_out, uint64_t* idx_out,
+                         double* w_out, void* ws, size_t ws_bytes, uint64_t* n_acc_dev,
+                         void* stream);
+
 size_t hpsp_average_on_workspace_size(uint64_t n);
 
 /* PhaseSpace<N,GRND>::AverageOn(policy, mother, f, n) over events

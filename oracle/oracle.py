@@ -100,6 +100,8 @@ class Oracle:
         L.po_evaluate_mothers.argtypes = [_I, _I, _P, _U64, _U64, _U64, _P, _I, _P, _P, _P, _I]
         L.po_pair_hash.argtypes = [_U64, _U64]
         L.po_pair_hash.restype = _U64
+        L.po_unweight_sample.argtypes = [_I, _P, _U64, _P, _D, _U64, _U64, _I, _P, _P, _P]
+        L.po_unweight_sample.restype = _U64
         L.po_histogram_fill.argtypes = [_I, _P, _P, _P, _P, _U64, _U64, _P, _P]
         L.po_hist_key.argtypes = [_I, _P, _P, _P, _P]
         L.po_hist_key.restype = _U64
@@ -278,6 +280,18 @@ class Oracle:
         return self.L.po_hist_key(len(g), _ptr(g), _ptr(_f64(lower)), _ptr(_f64(upper)),
                                   _ptr(_f64(x)))
 
+    def unweight_sample(self, cols, w, max_pdf=-1.0, seed=0x8EC74D321E6B5A27, jump=0, order=0):
+        """hydra::unweight on value columns (ncols, n) and weights; returns
+        (kept, reordered columns, original index of each output row).
+        order 0: both parts stable; 1: the reference OMP build's order."""
+        c = _f64(np.atleast_2d(cols))
+        ncols, n = c.shape
+        out = np.zeros_like(c)
+        idx = np.zeros(n, dtype=np.uint64)
+        kept = self.L.po_unweight_sample(ncols, _ptr(c), n, _ptr(_f64(w)), max_pdf, seed, jump,
+                                         order, _ptr(out), _ptr(idx), None)
+        return kept, out, idx
+
     def pair_hash(self, evt, seed):
         return self.L.po_pair_hash(evt, seed)
 
@@ -310,6 +324,7 @@ class Reference:
         L.hr_evaluate.argtypes = [_I, _D, _P, _P, _I, ctypes.c_int32, _U64, _I, _P, _P, _P, _P]
         L.hr_evaluate_mothers.argtypes = [_I, _D, _P, ctypes.c_int32, _U64, _P, _I, _P, _P, _P]
         L.hr_histogram_fill.argtypes = [_I, _P, _P, _P, _P, _U64, _P, _P, _P]
+        L.hr_unweight_sample.argtypes = [_P, _U64, _D, _D, _D, _U64, _U64, _P, _P, _P]
 
     def set_threads(self, n):
         return self.L.hr_set_threads(n)
@@ -440,6 +455,26 @@ def _ref_histogram_fill(L, grid, lower, upper, x, w=None):
 
 Reference.histogram_fill = lambda self, grid, lower, upper, x, w=None: \
     _ref_histogram_fill(self.L, grid, lower, upper, x, w)
+
+
+def _ref_unweight_sample(L, x, mean, sigma, max_pdf, seed, jump):
+    xv = _f64(x)
+    n = len(xv)
+    xo = np.zeros(n)
+    w = np.zeros(n)
+    kept = _U64()
+    rc = L.hr_unweight_sample(_ptr(xv), n, mean, sigma, max_pdf, seed, jump, _ptr(xo), _ptr(w),
+                              ctypes.byref(kept))
+    if rc:
+        raise RuntimeError(f"hr_unweight_sample failed: {rc}")
+    return kept.value, xo, w
+
+
+Reference.unweight_sample = lambda self, x, mean, sigma, max_pdf=-1.0, \
+    seed=0x8EC74D321E6B5A27, jump=0: _ref_unweight_sample(self.L, x, mean, sigma, max_pdf, seed,
+                                                          jump)
+Reference.unweight_sample.__doc__ = ("hydra::unweight(device::sys, x, Gaussian(mean, sigma), max, "
+                                     "seed, jump) -> (kept, reordered x, gauss(x))")
 
 
 def rows_to_cols(p):

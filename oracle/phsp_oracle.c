@@ -883,3 +883,66 @@ void po_histogram_fill(int ndim, const uint64_t* grid, const double* lower, cons
   }
   free(ki);
 }
+
+/* hydra::unweight(policy, data, functor, max_pdf, rng_seed, rng_jump)
+ * (Random.inl:134-177) with w_k = functor(data_k) precomputed: max = max_pdf
+ * if > 0 else the largest weight (max_element); sample k is kept iff
+ * u * max <= w_k with u the uniform of squares3(rng_seed) after
+ * discard(rng_jump + k) (RndFlag, RandomUtils.h:254-262:
+ * uniform_real_distribution(0, max) gives u * (max - 0) + 0).  The order:
+ *   order 0: kept first, both parts in their original order;
+ *   order 1: the reference's OMP build, whose thrust::partition runs the
+ *            sequential swap algorithm: the kept part in order, each
+ *            rejected sample before position nacc moved to the position of
+ *            the kept sample it is swapped with (followed while that is
+ *            below nacc), the others in place.                              */
+uint64_t po_unweight_sample(int ncols, const double* cols_in, uint64_t n, const double* w,
+                            double max_pdf, uint64_t seed, uint64_t jump, int order,
+                            double* cols_out, uint64_t* idx_out, uint8_t* flags_out) {
+  double mx = max_pdf;
+  if (!(max_pdf > 0.0) && n > 0) {
+    mx = w[0];
+    for (uint64_t k = 1; k < n; ++k)
+      if (mx < w[k]) mx = w[k];
+  }
+  uint64_t key = seed;
+  key = po_splitmix64(&key);
+  uint8_t* f = flags_out ? flags_out : (uint8_t*)malloc(n ? n : 1);
+  uint64_t nacc = 0;
+  for (uint64_t k = 0; k < n; ++k) {
+    double u = po_uniform(0, po_squares3(key, jump + k));
+    double urnd = u * (mx - 0.0) + 0.0;
+    f[k] = (urnd <= w[k]) ? 1 : 0;
+    nacc += f[k];
+  }
+  uint64_t* pos = (uint64_t*)malloc(sizeof(uint64_t) * (n ? n : 1)); /* output row of k */
+  if (order == 0) {
+    uint64_t a = 0, r = nacc;
+    for (uint64_t k = 0; k < n; ++k) pos[k] = f[k] ? a++ : r++;
+  } else {
+    /* forward swap partition: first = trues seen; a true at next swaps with
+     * position first.  Simulated on a position -> sample map.              */
+    uint64_t* at = (uint64_t*)malloc(sizeof(uint64_t) * (n ? n : 1));
+    for (uint64_t k = 0; k < n; ++k) at[k] = k;
+    uint64_t first = 0;
+    while (first < n && f[first]) ++first;
+    for (uint64_t next = first + 1; next < n; ++next) {
+      if (f[next]) {
+        uint64_t t = at[first];
+        at[first] = at[next];
+        at[next] = t;
+        ++first;
+      }
+    }
+    for (uint64_t p = 0; p < n; ++p) pos[at[p]] = p;
+    free(at);
+  }
+  for (uint64_t k = 0; k < n; ++k) {
+    if (idx_out) idx_out[pos[k]] = k;
+    if (cols_out)
+      for (int c = 0; c < ncols; ++c) cols_out[(uint64_t)c * n + pos[k]] = cols_in[(uint64_t)c * n + k];
+  }
+  free(pos);
+  if (!flags_out) free(f);
+  return nacc;
+}

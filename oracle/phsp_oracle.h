@@ -80,6 +80,9 @@ void po_evaluate(int N, int engine, const double* masses, double ecm, double max
                  const double* beta3, uint64_t seed, uint64_t first, uint64_t n, int nf,
                  const int* fids, const double* fpar, double* out, int nthreads);
 uint64_t po_pair_hash(uint64_t evt, uint64_t seed);
+uint64_t po_unweight_sample(int ncols, const double* cols_in, uint64_t n, const double* w,
+                            double max_pdf, uint64_t seed, uint64_t jump, int order,
+                            double* cols_out, uint64_t* idx_out, uint8_t* flags_out);
 uint64_t po_hist_key(int ndim, const uint64_t* grid, const double* lower, const double* upper,
                      const double* x);
 void po_histogram_fill(int ndim, const uint64_t* grid, const double* lower, const double* upper,

@@ -22,6 +22,9 @@
 #include <hydra/Decays.h>
 #include <hydra/multivector.h>
 #include <hydra/DenseHistogram.h>
+#include <hydra/Random.h>
+#include <hydra/Parameter.h>
+#include <hydra/functions/Gaussian.h>
 #include <hydra/detail/Hash.h>
 
 #include <omp.h>
@@ -625,6 +628,25 @@ int hr_histogram_fill(int ndim, const uint64_t* grid, const double* lower, const
     return 0;
   }
   return -1;
+}
+
+// hydra::unweight(device::sys, data, gauss, max_pdf, rng_seed, rng_jump) with
+// gauss = hydra::Gaussian<double>(mean, sigma), as examples/random/
+// unweight_sample.inl:118-158 runs it (the data-plus-weights overload does not
+// compile in the reference: Random.inl:68).  w_out receives gauss(x_k) as the
+// same functor computes it; x_out the reordered data.
+int hr_unweight_sample(const double* x, uint64_t n, double mean, double sigma, double max_pdf,
+                       uint64_t seed, uint64_t jump, double* x_out, double* w_out,
+                       uint64_t* n_kept) {
+  auto Mean = hydra::Parameter::Create("mean").Value(mean);
+  auto Sigma = hydra::Parameter::Create("sigma").Value(sigma);
+  auto gauss = hydra::Gaussian<double>(Mean, Sigma);
+  for (uint64_t k = 0; k < n; ++k) w_out[k] = gauss(x[k]);
+  hydra::device::vector<double> data(x, x + n);
+  auto r = hydra::unweight(hydra::device::sys, data, gauss, max_pdf, (size_t)seed, (size_t)jump);
+  *n_kept = (uint64_t)r.size();
+  for (uint64_t i = 0; i < n; ++i) x_out[i] = data[i];
+  return 0;
 }
 
 }  // extern "C"

@@ -191,6 +191,19 @@ int main(int argc, char** argv) {
     }
     emit_array("evaluate_mothers_rows", emrows);
 
+    // hydra::unweight(device::sys, events, events.GetEventWeightFunctor(bw))
+    // (phsp_unweighting_functor.inl:209) on a copy
+    auto E4 = Events;
+    auto uw_free = hydra::unweight(hydra::device::sys, E4, E4.GetEventWeightFunctor(bw));
+    std::printf("\"free_unweight_nacc\": %zu,\n", uw_free.size());
+    std::vector<double> fu_rows;
+    for (size_t i : {0ul, 1ul, (size_t)uw_free.size(), (size_t)4095}) {
+      auto ev = E4[i];
+      const hydra::Vector4R c = std::get<2>(ev);
+      for (int q = 0; q < 4; ++q) fu_rows.push_back(c.get(q));
+    }
+    emit_array("free_unweight_rows_C", fu_rows);
+
     // Dalitz plot of the 4096 events (phsp_basic.inl:196-214): the fused fill
     // with the container's weight column, and with PhaseSpaceWeight computed
     // inside the fill, and from Evaluate's value columns

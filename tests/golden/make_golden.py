@@ -161,6 +161,17 @@ def main():
     out["hist_1d_x"] = x1
     out["hist_1d_n"], _ = R.histogram_fill([25], [lo[0]], [hi[0]], x1[None, :])
 
+    # hydra::unweight(device::sys, flat, gauss, max, seed, jump) as
+    # examples/random/unweight_sample.inl:118-158 runs it (flat sample in
+    # [0, 20), Gaussian(10, 1)): the reordered sample and the weights
+    xs = np.random.default_rng(11).uniform(0.0, 20.0, 5000)
+    out["uws_x"] = xs
+    for tag, mp, jump in (("auto", -1.0, 0), ("max", 0.5, 10000)):
+        kept, xo, wg = R.unweight_sample(xs, 10.0, 1.0, mp, 0x8EC74D321E6B5A27, jump)
+        out[f"uws_{tag}_kept"] = np.array([kept], dtype=np.uint64)
+        out[f"uws_{tag}_x"] = xo
+        out["uws_w"] = wg
+
     np.savez_compressed(os.path.join(HERE, "phsp_golden.npz"), **out)
     print("wrote", os.path.join(HERE, "phsp_golden.npz"), len(out), "arrays")
 

@@ -118,3 +118,26 @@ def evaluate(dec, functors, seed, n, first=0, mothers=None, ld=None):
     check(rc, "hpsp_evaluate")
     torch.cuda.synchronize()
     return out
+
+
+def unweight_sample(cols, w, max_pdf=-1.0, seed=capi.UNWEIGHT_SAMPLE_SEED, jump=0,
+                    order=capi.ORDER_STABLE, want_cols=True, want_idx=True, want_w=False):
+    """hpsp_unweight_sample on (ncols, n) value columns -> (kept, cols, idx, w)."""
+    c = torch.as_tensor(np.ascontiguousarray(np.atleast_2d(cols)), dtype=torch.float64,
+                        device=DEV)
+    ncols, n = c.shape
+    wt = torch.as_tensor(np.ascontiguousarray(w), dtype=torch.float64, device=DEV)
+    ws_bytes = lib().hpsp_unweight_sample_workspace_size(n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
+    out = torch.empty_like(c) if want_cols else None
+    idx = torch.empty(max(n, 1), dtype=torch.int64, device=DEV) if want_idx else None
+    w_out = torch.empty(max(n, 1), dtype=torch.float64, device=DEV) if want_w else None
+    nacc = torch.zeros(1, dtype=torch.int64, device=DEV)
+    rc = lib().hpsp_unweight_sample(ncols, ptr(c), n, n, ptr(wt), max_pdf, seed, jump, order,
+                                    ptr(out), n, ptr(idx), ptr(w_out), ptr(ws), ws_bytes,
+                                    ptr(nacc), stream_handle())
+    check(rc, "hpsp_unweight_sample")
+    torch.cuda.synchronize()
+    return (int(nacc.item()), None if out is None else out.cpu().numpy(),
+            None if idx is None else idx.cpu().numpy()[:n],
+            None if w_out is None else w_out.cpu().numpy()[:n])

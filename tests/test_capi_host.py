@@ -146,3 +146,29 @@ def test_histogram_validates_before_touching_the_device():
                                         None) == 1  # weight mode
     assert L.hpsp_histogram_fill_events(good, ctypes.byref(d), axes, out, 10, 10,
                                         capi.HIST_WEIGHT_COLUMN, None, out, None) == 1
+
+
+def test_unweight_sample_validates_before_touching_the_device():
+    L = capi.lib()
+    p = ctypes.c_void_p(64)
+    n = 100
+    wsb = L.hpsp_unweight_sample_workspace_size(n)
+    assert wsb > L.hpsp_unweight_workspace_size(n)
+    args = dict(ncols=1, cols=p, ld=n, n=n, w=p, mp=-1.0, seed=1, jump=0, order=0, out=p,
+                ldo=n, idx=None, wo=None, ws=p, wsb=wsb, nacc=p)
+
+    def call(**kw):
+        a = dict(args)
+        a.update(kw)
+        return L.hpsp_unweight_sample(a["ncols"], a["cols"], a["ld"], a["n"], a["w"], a["mp"],
+                                      a["seed"], a["jump"], a["order"], a["out"], a["ldo"],
+                                      a["idx"], a["wo"], a["ws"], a["wsb"], a["nacc"], None)
+    assert call(order=2) == 1
+    assert call(ncols=-1) == 1
+    assert call(w=None) == 1
+    assert call(nacc=None) == 1
+    assert call(ld=50) == 1
+    assert call(ldo=50) == 1
+    assert call(ncols=0) == 1          # columns out with no columns
+    assert call(wsb=wsb - 8) == capi.ERR_WORKSPACE
+    assert call(ws=None) == capi.ERR_WORKSPACE

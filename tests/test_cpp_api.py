@@ -102,3 +102,13 @@ def test_gpu_generate_unweight_average(golden, oracle):
         got = np.array(r[key])
         assert np.all(np.abs(got - hw) <= 1e-10 * hw), key
     assert r["hist_bin_3_7"] == golden["hist_dalitz_n"][3 * 20 + 7]
+    # hydra::unweight(device::sys, events, events.GetEventWeightFunctor(bw)):
+    # the oracle (the reference's order bit for bit) on the reference's events
+    from oracle import rows_to_cols
+    gcols = rows_to_cols(golden["gen_c1_p"])
+    wf = oracle.event_weight(M, MASSES, gcols, fid=1, fpar=golden["bw3_par"])
+    k, out, _ = oracle.unweight_sample(gcols, wf, -1.0, 0x8EC74D321E6B5A27, 0, order=1)
+    assert r["free_unweight_nacc"] == k
+    got = np.array(r["free_unweight_rows_C"]).reshape(4, 4)
+    exp = out.reshape(3, 4, 4096)[2][:, [0, 1, k, 4095]].T
+    assert np.max(np.abs(got - exp) / exp[:, 0:1]) <= 1e-12

@@ -419,3 +419,50 @@ def test_histogram_equals_reference(oracle, reference):
         for ww in (w, None):
             ref, _ = reference.histogram_fill(grid, lo, hi, x, ww)
             assert np.array_equal(bits(oracle.histogram_fill(grid, lo, hi, x, ww)), bits(ref))
+
+
+# ---------------------------------------------------------------- hydra::unweight
+def test_unweight_sample_equals_reference(oracle, reference):
+    """hydra::unweight(device::sys, x, Gaussian, max, seed, jump): the reference
+    OMP build's order (sequential swap partition) bit for bit; the stable
+    order keeps the same prefix and the same rejected set."""
+    rng = np.random.default_rng(1)
+    for n, mp, jump in ((100000, -1.0, 0), (50000, 0.3, 200000), (7, -1.0, 5),
+                        (12, -1.0, 0), (1, -1.0, 0)):
+        x = rng.uniform(0, 20, n)
+        kept, xo, w = reference.unweight_sample(x, 10.0, 1.0, mp, 0x8EC74D321E6B5A27, jump)
+        k1, o1, _ = oracle.unweight_sample(x[None, :], w, mp, 0x8EC74D321E6B5A27, jump, order=1)
+        assert k1 == kept and np.array_equal(bits(o1[0]), bits(xo))
+        k0, o0, _ = oracle.unweight_sample(x[None, :], w, mp, 0x8EC74D321E6B5A27, jump, order=0)
+        assert np.array_equal(o0[0][:k0], xo[:kept])
+        assert np.array_equal(np.sort(o0[0][k0:]), np.sort(xo[kept:]))
+
+
+def test_unweight_sample_closed_form(oracle):
+    """Acceptance u * max <= w by hand; the sequential-partition order by a
+    direct simulation of the swap loop."""
+    rng = np.random.default_rng(3)
+    n = 3000
+    w = rng.uniform(0, 2, n)
+    words = oracle.rng_words("squares3", 0x8EC74D321E6B5A27, 17, n).astype(np.float64)
+    keep = (words / 2**32) * w.max() <= w
+    k, out, idx = oracle.unweight_sample(np.arange(n, dtype=float)[None, :], w, -1.0,
+                                         0x8EC74D321E6B5A27, 17, order=1)
+    assert k == keep.sum()
+    a = list(range(n))
+    first = 0
+    while first < n and keep[first]:
+        first += 1
+    for nxt in range(first + 1, n):
+        if keep[nxt]:
+            a[first], a[nxt] = a[nxt], a[first]
+            first += 1
+    assert np.array_equal(idx, np.array(a, dtype=np.uint64))
+
+
+def test_unweight_sample_golden(oracle, golden):
+    x, w = golden["uws_x"], golden["uws_w"]
+    for tag, mp, jump in (("auto", -1.0, 0), ("max", 0.5, 10000)):
+        k, out, _ = oracle.unweight_sample(x[None, :], w, mp, 0x8EC74D321E6B5A27, jump, order=1)
+        assert k == int(golden[f"uws_{tag}_kept"][0])
+        assert np.array_equal(bits(out[0]), bits(golden[f"uws_{tag}_x"]))
