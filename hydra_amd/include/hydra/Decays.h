@@ -26,6 +26,7 @@
 #include <hydra/Vector4R.h>
 #include <hydra/detail/Runtime.h>
 #include <hydra/device/System.h>
+#include <hydra/host/System.h>
 
 #include "hpsp.h"
 
@@ -190,6 +191,10 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   Decays(Decays&& other) noexcept = default;
   Decays& operator=(const Decays& other) = default;
   Decays& operator=(Decays&& other) noexcept = default;
+
+  // trans-backend copy from host storage (Decays.h:180-193, 606-618)
+  explicit Decays(Decays<hydra::tuple<Particles...>, hydra::host::sys_t> const& other);
+  Decays& operator=(Decays<hydra::tuple<Particles...>, hydra::host::sys_t> const& other);
 
   // ---- STL-like interface (Decays.h:298-394)
   size_t size() const { return fSize; }
@@ -380,6 +385,91 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   detail::device_buffer<double> fW;
   bool fWValid = false;
 };
+
+// Decays in host memory (the reference's Decays<tuple<P...>, host::sys_t>):
+// the same structure-of-arrays columns, copied to and from the device in one
+// strided transfer.  Rows read as tuples of the particle types.
+template <typename... Particles>
+class Decays<hydra::tuple<Particles...>, hydra::host::sys_t> {
+ public:
+  static constexpr size_t N = sizeof...(Particles);
+  enum { nparticles = N };
+  typedef hydra::tuple<Particles...> value_type;
+  typedef hydra::tuple<Particles...> tuple_type;
+  typedef hydra::host::sys_t system_type;
+
+  Decays(double motherMass, std::array<double, N> const& daughtersMasses, size_t nentries = 0)
+      : fMotherMass(motherMass), fMasses(daughtersMasses), fSize(nentries),
+        fCols(4 * N * nentries) {
+    fMaxWeight = detail::make_decay<N>(fMotherMass, fMasses, "Decays").max_weight;
+  }
+  // trans-backend copy from the device (Decays.h:180-193)
+  explicit Decays(Decays<hydra::tuple<Particles...>, hydra::device::sys_t> const& other)
+      : fMotherMass(other.GetMotherMass()), fMaxWeight(other.GetMaxWeight()),
+        fMasses(other.GetMasses()), fSize(other.size()), fCols(4 * N * other.size()) {
+    if (fSize)
+      detail::hpsp_check(hpsp_memcpy2d_to_host(fCols.data(), fSize * sizeof(double), other.data(),
+                                               other.ld() * sizeof(double),
+                                               fSize * sizeof(double), 4 * N),
+                         "Decays(host <- device)");
+  }
+
+  size_t size() const { return fSize; }
+  size_t ld() const { return fSize; }
+  double GetMotherMass() const { return fMotherMass; }
+  double GetMaxWeight() const { return fMaxWeight; }
+  std::array<double, N> GetMasses() const { return fMasses; }
+  const double* data() const { return fCols.data(); }
+  double* data() { return fCols.data(); }
+  const double* column(size_t particle, int component) const {
+    return fCols.data() + (4 * particle + component) * fSize;
+  }
+  value_type operator[](size_t k) const {
+    return make_row(k, std::make_index_sequence<N>{});
+  }
+
+ private:
+  template <size_t... I>
+  value_type make_row(size_t k, std::index_sequence<I...>) const {
+    return value_type(Vector4R(column(I, 0)[k], column(I, 1)[k], column(I, 2)[k],
+                               column(I, 3)[k])...);
+  }
+
+  double fMotherMass;
+  double fMaxWeight;
+  std::array<double, N> fMasses;
+  size_t fSize;
+  std::vector<double> fCols;
+};
+
+template <typename... Particles>
+Decays<hydra::tuple<Particles...>, hydra::device::sys_t>::Decays(
+    Decays<hydra::tuple<Particles...>, hydra::host::sys_t> const& other)
+    : fMotherMass(other.GetMotherMass()), fMasses(other.GetMasses()) {
+  init(other.size());
+  fMaxWeight = other.GetMaxWeight();
+  if (fSize)
+    detail::hpsp_check(hpsp_memcpy2d_to_device(fCols.data(), fCap * sizeof(double), other.data(),
+                                               other.ld() * sizeof(double),
+                                               fSize * sizeof(double), 4 * N),
+                       "Decays(device <- host)");
+}
+
+// assignment keeps this container's masses, as the reference's operator=
+// copies rows only (Decays.h:606-618)
+template <typename... Particles>
+Decays<hydra::tuple<Particles...>, hydra::device::sys_t>&
+Decays<hydra::tuple<Particles...>, hydra::device::sys_t>::operator=(
+    Decays<hydra::tuple<Particles...>, hydra::host::sys_t> const& other) {
+  resize(other.size());
+  fWValid = false;
+  if (fSize)
+    detail::hpsp_check(hpsp_memcpy2d_to_device(fCols.data(), fCap * sizeof(double), other.data(),
+                                               other.ld() * sizeof(double),
+                                               fSize * sizeof(double), 4 * N),
+                       "Decays(device <- host)");
+  return *this;
+}
 
 // events | weight functor  (Range2.inl:213-238): weights computed on the device.
 template <typename... Particles>

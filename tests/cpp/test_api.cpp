@@ -204,6 +204,23 @@ int main(int argc, char** argv) {
     }
     emit_array("free_unweight_rows_C", fu_rows);
 
+    // trans-backend copies: device -> host -> device
+    hydra::Decays<hydra::tuple<A, B, C>, hydra::host::sys_t> HostEvents(Events);
+    hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> Back(HostEvents);
+    int same = (int)(HostEvents.size() == Events.size() && Back.size() == Events.size());
+    for (size_t i : {0ul, 17ul, 4095ul}) {
+      auto h = HostEvents[i];
+      auto d = Events[i];
+      auto b = Back[i];
+      for (int q = 0; q < 4; ++q) {
+        const hydra::Vector4R hc = std::get<2>(h), dc = std::get<2>(d), bc = std::get<2>(b);
+        same &= (int)(hc.get(q) == dc.get(q) && bc.get(q) == dc.get(q));
+      }
+    }
+    std::printf("\"host_copy_same\": %d,\n", same);
+    auto wback = Back | Back.GetEventWeightFunctor();
+    emit_array("host_copy_weights3", {wback[0], wback[17], wback[4095]});
+
     // Dalitz plot of the 4096 events (phsp_basic.inl:196-214): the fused fill
     // with the container's weight column, and with PhaseSpaceWeight computed
     // inside the fill, and from Evaluate's value columns

@@ -112,3 +112,7 @@ def test_gpu_generate_unweight_average(golden, oracle):
     got = np.array(r["free_unweight_rows_C"]).reshape(4, 4)
     exp = out.reshape(3, 4, 4096)[2][:, [0, 1, k, 4095]].T
     assert np.max(np.abs(got - exp) / exp[:, 0:1]) <= 1e-12
+    # device -> host -> device copies keep every value
+    assert r["host_copy_same"] == 1
+    w = np.array(r["gen_weights"])
+    assert r["host_copy_weights3"] == [w[0], w[17], w[4095]]
