@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--extras", type=int, default=1, help="also time the other workloads")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--only", default="",
-                    help="run one workload: c2, c3, c4, evaluate, histogram, c5 (serving), or copy (a plain 1 GiB device copy "
+                    help="run one workload: c2, c3, c4, evaluate, histogram, range, c5 (serving), or copy (a plain 1 GiB device copy "
                          "that calibrates the profiler's byte counters)")
     return ap.parse_args()
 
@@ -379,6 +379,74 @@ def cpu_baseline_histogram(n=1 << 22):
                       f"reference OMP backend, {cores} threads (sort_by_key + reduce_by_key)"}
 
 
+MK, MPI = 0.493677, 0.13957061
+
+
+def run_range(torch, capi, n=1 << 27, reps=5):
+    """phase_space_range(K+ at rest, 3 pi, seed 321) of
+    examples/misc/range_semantics.inl:154-167 over 2^27 elements: written out
+    as columns (weight + 12 momentum doubles, 104 B/element), and histogrammed
+    as the example does (m(pi pi) of daughters 0 and 1, 10 bins) with the
+    elements generated inside the fill (no memory traffic but the bins)."""
+    lib = capi.lib()
+    dec = capi.make_range([MK, 0.0, 0.0, 0.0], [MPI] * 3)
+    sh = capi.stream_handle()
+    res = {"elements": n, "decay": "K+ (at rest) -> 3 pi, seed 321"}
+    cols = torch.empty((12, n), dtype=torch.float64, device="cuda:0")
+    wt = torch.empty(n, dtype=torch.float64, device="cuda:0")
+
+    def gen():
+        capi.check(lib.hpsp_range_generate(ctypes.byref(dec), 321, 0, n, capi.ptr(cols), n,
+                                           capi.ptr(wt), sh), "range_generate")
+    gen()
+    t = Timer(torch)
+    for _ in range(reps):
+        t.start()
+        gen()
+        t.stop()
+    res["generate_ms"] = t.avg_ms
+    res["generate_elements_per_s"] = n / (t.avg_ms / 1e3)
+    res["generate_GBs"] = n * 104 / (t.avg_ms * 1e6)
+    del cols, wt
+    h = capi.make_histogram([10], [2 * MPI], [MK - MPI])
+    out = torch.empty(12, dtype=torch.float64, device="cuda:0")
+    axes = capi.functor_array([capi.make_functor(capi.F_MASS, (0, 1))])
+    for tag, mode in (("hist", capi.HIST_WEIGHT_ONE),
+                      ("hist_weighted", capi.HIST_WEIGHT_GENERATION)):
+        def fill():
+            capi.check(lib.hpsp_histogram_fill_range(ctypes.byref(h), ctypes.byref(dec), 321,
+                                                     axes, 0, n, mode, capi.ptr(out), sh),
+                       "fill_range")
+        fill()
+        t = Timer(torch)
+        for _ in range(reps):
+            t.start()
+            fill()
+            t.stop()
+        res[f"{tag}_ms"] = t.avg_ms
+        res[f"{tag}_elements_per_s"] = n / (t.avg_ms / 1e3)
+        if mode == capi.HIST_WEIGHT_ONE:
+            res["hist_counts"] = float(out[:10].sum().item())
+    return res
+
+
+def cpu_baseline_range(n=1 << 24):
+    """The reference's range histogram (DenseHistogram<double,1>::Fill over
+    phase_space_range | invariant mass, OMP, all host threads), 2^24 elements."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    lib = orc.REF_NATIVE_LIB if os.path.exists(orc.REF_NATIVE_LIB) else orc.REF_LIB
+    if not os.path.exists(lib):
+        return None
+    R = orc.Reference(lib)
+    cores = len(os.sched_getaffinity(0))
+    R.set_threads(cores)
+    _, sec = R.range_histogram3([MK, 0, 0, 0], [MPI] * 3, 321, n, 0, 1, 10, 2 * MPI, MK - MPI)
+    return {"value": n / sec, "unit": "elements/s", "cores": cores, "kind": "reference",
+            "sample": f"{n} elements: DenseHistogram<double,1>::Fill(phase_space_range | "
+                      f"(A+B).mass()), 10 bins, reference OMP backend, {cores} threads"}
+
+
 BW2 = [(0.892, 0.05, 1.0, 0.3, 1, 2), (1.43, 0.11, -0.4, 0.8, 0, 1)]
 
 
@@ -592,6 +660,12 @@ def main():
         extras["histogram"] = run_histogram(torch, capi)
         print(json.dumps(extras), flush=True)
         return
+    if args.only == "range":
+        extras["range"] = run_range(torch, capi)
+        if args.cpu_baseline:
+            extras["range"]["cpu_baseline"] = cpu_baseline_range()
+        print(json.dumps(extras), flush=True)
+        return
     if args.only == "c5":
         extras["c5"] = run_c5(torch, capi)
         extras["c5_saturated"] = run_c5(torch, capi, rate=50000.0)
@@ -622,6 +696,8 @@ def main():
         print("# evaluate done", file=sys.stderr, flush=True)
         extras["histogram"] = run_histogram(torch, capi)
         print("# histogram done", file=sys.stderr, flush=True)
+        extras["range"] = run_range(torch, capi)
+        print("# range done", file=sys.stderr, flush=True)
         extras["c5"] = run_c5(torch, capi)
         print("# c5 done", file=sys.stderr, flush=True)
     extras["c2"] = c2
@@ -648,6 +724,8 @@ def main():
             extras["evaluate"]["cpu_baseline"] = cpu_baseline_evaluate()
         if args.extras and "histogram" in extras:
             extras["histogram"]["cpu_baseline"] = cpu_baseline_histogram()
+        if args.extras and "range" in extras:
+            extras["range"]["cpu_baseline"] = cpu_baseline_range()
         print("# cpu baseline done", file=sys.stderr, flush=True)
 
     line = {

@@ -26,8 +26,9 @@ ERR_HIP = 5
 
 ENGINES = {"squares3": 0, "squares4": 1, "philox": 2, "philox_long": 3,
            "threefry": 4, "threefry_long": 5}
+MINSTD_RAND = 6  # phase_space_range's engine (hpsp_range_* only)
 FP64, MIXED = 0, 1
-F_ONE, F_BW_SUM, F_MASS2_01, F_MASS2, F_COS_HELICITY = 0, 1, 2, 3, 4
+F_ONE, F_BW_SUM, F_MASS2_01, F_MASS2, F_COS_HELICITY, F_MASS = 0, 1, 2, 3, 4, 5
 MAX_EVAL_FUNCTORS = 8
 
 
@@ -53,7 +54,7 @@ class Stats(ctypes.Structure):
 ORDER_STABLE, ORDER_SEQUENTIAL = 0, 1
 UNWEIGHT_SAMPLE_SEED = 0x8EC74D321E6B5A27
 HIST_MAX_DIM = 4
-HIST_WEIGHT_ONE, HIST_WEIGHT_COLUMN, HIST_WEIGHT_PHSP = 0, 1, 2
+HIST_WEIGHT_ONE, HIST_WEIGHT_COLUMN, HIST_WEIGHT_PHSP, HIST_WEIGHT_GENERATION = 0, 1, 2, 3
 
 
 class Histogram(ctypes.Structure):
@@ -144,6 +145,10 @@ _SIGS = {
     "hpsp_histogram_fill": (_I, [ctypes.POINTER(Histogram), _P, _U64, _U64, _P, _P, _P]),
     "hpsp_histogram_fill_events": (_I, [ctypes.POINTER(Histogram), ctypes.POINTER(Decay),
                                         ctypes.POINTER(Functor), _P, _U64, _U64, _I, _P, _P, _P]),
+    "hpsp_range_init": (_I, [ctypes.POINTER(Decay), _P, _I, _P]),
+    "hpsp_range_generate": (_I, [ctypes.POINTER(Decay), _U64, _U64, _U64, _P, _U64, _P, _P]),
+    "hpsp_histogram_fill_range": (_I, [ctypes.POINTER(Histogram), ctypes.POINTER(Decay), _U64,
+                                       ctypes.POINTER(Functor), _U64, _U64, _I, _P, _P]),
     "hpsp_malloc": (_I, [ctypes.POINTER(_P), _SZ]),
     "hpsp_free": (_I, [_P]),
     "hpsp_memcpy_to_device": (_I, [_P, _P, _SZ]),
@@ -209,13 +214,22 @@ def make_decay(mother_mass, masses, engine="squares3", mother4=None):
     return d
 
 
+def make_range(mother4, masses):
+    """Decay description of phase_space_range(mother, masses, seed, length)."""
+    d = Decay()
+    m = (ctypes.c_double * len(masses))(*[float(x) for x in masses])
+    m4 = (ctypes.c_double * 4)(*[float(x) for x in mother4])
+    check(lib().hpsp_range_init(ctypes.byref(d), m4, len(masses), m), "hpsp_range_init")
+    return d
+
+
 def make_functor(kind=F_ONE, terms=()):
-    """F_BW_SUM: terms = [(M, G, cr, ci, a, b), ...]; F_MASS2 and
+    """F_BW_SUM: terms = [(M, G, cr, ci, a, b), ...]; F_MASS2, F_MASS and
     F_COS_HELICITY: terms = (a, b) (for the helicity cosine, a is the particle
     whose angle is measured and b its partner)."""
     f = Functor()
     f.kind = kind
-    if kind in (F_MASS2, F_COS_HELICITY):
+    if kind in (F_MASS2, F_COS_HELICITY, F_MASS):
         f.a[0], f.b[0] = int(terms[0]), int(terms[1])
         return f
     f.nterms = len(terms)

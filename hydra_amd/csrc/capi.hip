@@ -279,7 +279,7 @@ int hpsp_histogram_fill_events(const hpsp_histogram* h, const hpsp_decay* d,
   memset(&a, 0, sizeof(a));
   int rc = make_histk(h, &a.h);
   if (rc) return rc;
-  rc = check_decay(d);
+  rc = check_decay_masses(d);
   if (rc) return rc;
   if (!contents || !axes) return HPSP_ERR_INVALID_ARGUMENT;
   if (weight_mode < HPSP_HIST_WEIGHT_ONE || weight_mode > HPSP_HIST_WEIGHT_PHSP)
@@ -301,9 +301,81 @@ int hpsp_histogram_fill_events(const hpsp_histogram* h, const hpsp_decay* d,
   return hip_status(launch_histogram_fill(d->n, a, as_stream(stream)));
 }
 
+// ---- phase_space_range ----------------------------------------------------
+
+int hpsp_range_init(hpsp_decay* d, const double mother4[4], int n, const double* masses) {
+  if (!d || !mother4 || !masses || n < 2 || n > HPSP_MAX_N) return HPSP_ERR_INVALID_ARGUMENT;
+  // mother.mass() (Vector4R.inl:196-208)
+  const double m2 = mother4[0] * mother4[0] - mother4[1] * mother4[1] -
+                    mother4[2] * mother4[2] - mother4[3] * mother4[3];
+  const double M = m2 > 0.0 ? sqrt(m2) : 0.0;
+  // fTeCmTm and fWtMax follow the PhaseSpace constructor's arithmetic
+  // (GenerateDecay.h:64-86)
+  int rc = hpsp_decay_init(d, n, M, masses, HPSP_SQUARES3);
+  if (rc) return rc;
+  d->engine = HPSP_MINSTD_RAND;
+  return hpsp_decay_set_mother(d, mother4);
+}
+
+namespace {
+
+int range_args(const hpsp_decay* d, uint64_t seed, uint64_t first, uint64_t n, RangeArgs* a) {
+  if (!d || d->engine != HPSP_MINSTD_RAND || d->n < 2 || d->n > HPSP_MAX_N)
+    return HPSP_ERR_INVALID_ARGUMENT;
+  const uint64_t skip = 3 * (uint64_t)d->n;
+  if (first > UINT64_MAX - skip || n > UINT64_MAX - skip - first) return HPSP_ERR_INVALID_ARGUMENT;
+  memset(a, 0, sizeof(*a));
+  hpsp_decay dd = *d;
+  dd.engine = HPSP_SQUARES3;  // no key words: the state below is the whole engine
+  a->d = make_decayk(&dd, 0);
+  // GRND eng((int)seed); eng.discard(first + 3N); then one step for word 0
+  // (GenerateDecay.h:208-209)
+  const uint32_t x0 = minstd_seed((uint32_t)seed);
+  a->x_start = minstd_mul(minstd_mul(x0, minstd_pow(first + skip)), kMinstdA);
+  a->n = n;
+  return HPSP_OK;
+}
+
+}  // namespace
+
+int hpsp_range_generate(const hpsp_decay* d, uint64_t seed, uint64_t first, uint64_t n,
+                        double* cols, uint64_t ld, double* wt, void* stream) {
+  RangeArgs a;
+  int rc = range_args(d, seed, first, n, &a);
+  if (rc) return rc;
+  if (n == 0) return HPSP_OK;
+  if (!cols || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
+  a.cols = cols;
+  a.ld = ld;
+  a.wt = wt;
+  return hip_status(launch_range_generate(d->n, a, as_stream(stream)));
+}
+
+int hpsp_histogram_fill_range(const hpsp_histogram* h, const hpsp_decay* d, uint64_t seed,
+                              const hpsp_functor* axes, uint64_t first, uint64_t n,
+                              int weight_mode, double* contents, void* stream) {
+  RangeArgs a;
+  int rc = range_args(d, seed, first, n, &a);
+  if (rc) return rc;
+  rc = make_histk(h, &a.h);
+  if (rc) return rc;
+  if (!contents || !axes) return HPSP_ERR_INVALID_ARGUMENT;
+  if (weight_mode != HPSP_HIST_WEIGHT_ONE && weight_mode != HPSP_HIST_WEIGHT_GENERATION &&
+      weight_mode != HPSP_HIST_WEIGHT_PHSP)
+    return HPSP_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < h->ndim; ++i) {
+    rc = make_functork(&axes[i], d->n, &a.axes[i]);
+    if (rc) return rc;
+  }
+  a.weight_mode = weight_mode;
+  a.contents = contents;
+  a.lds_slots = hist_lds_slots();
+  return hip_status(launch_range_histogram(d->n, a, as_stream(stream)));
+}
+
 int hpsp_event_weight(const hpsp_decay* d, const hpsp_functor* f, const double* cols,
                       uint64_t ld, uint64_t n, double* w_out, void* stream) {
-  int rc = check_decay(d);
+  int rc = check_decay_masses(d);
   if (rc) return rc;
   if (n == 0) return HPSP_OK;
   if (!cols || !w_out || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
@@ -325,7 +397,7 @@ size_t hpsp_unweight_workspace_size(uint64_t n) { return uw_layout(n).total; }
 int hpsp_max_weight(const hpsp_decay* d, const hpsp_functor* f, const double* cols,
                     uint64_t ld, uint64_t n, void* ws, size_t ws_bytes, double* max_dev,
                     void* stream) {
-  int rc = check_decay(d);
+  int rc = check_decay_masses(d);
   if (rc) return rc;
   if (n == 0 || !cols || !max_dev || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
   const UwLayout L = uw_layout(n);
@@ -348,7 +420,7 @@ int hpsp_unweight(const hpsp_decay* d, const hpsp_functor* f, double max_value, 
                   const double* cols_in, uint64_t ld_in, uint64_t n, const double* w_in,
                   double* cols_out, uint64_t ld_out, uint64_t* idx_out, double* w_out,
                   void* ws, size_t ws_bytes, uint64_t* n_acc_dev, void* stream) {
-  int rc = check_decay(d);
+  int rc = check_decay_masses(d);
   if (rc) return rc;
   if (!n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
   if (n > 0) {

@@ -24,7 +24,7 @@ __device__ __forceinline__ double genbod_f64(const DecayK& d, const WordFn& word
   rno[0] = 0.0;
   rno[N - 1] = 1.0;
   if constexpr (N > 2) {
-    using WT = std::conditional_t<is_squares<ENGINE>(), uint32_t, uint64_t>;
+    using WT = std::conditional_t<narrow_words<ENGINE>(), uint32_t, uint64_t>;
     WT v[N - 2];
 #pragma unroll
     for (int k = 0; k < N - 2; ++k) v[k] = (WT)word(k);

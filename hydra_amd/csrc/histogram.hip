@@ -21,42 +21,9 @@
 #include <array>
 #include <utility>
 
-#include "hpsp_launch.h"
+#include "hist_key.h"
 
 namespace hpsp {
-
-constexpr uint64_t kDrop = ~0ULL;
-
-// GetGlobalBin: X = (x - lower) * grid / delta per axis; all axes below ->
-// underflow slot nbins, all above grid -> overflow slot nbins + 1, else the
-// row-major index of the truncated X.  Where the reference's conversion or
-// index is undefined (an axis at or below -1, NaN, an index past nbins + 1)
-// the event is dropped.
-__device__ __forceinline__ uint64_t hist_key(const HistK& h, const double (&x)[HPSP_HIST_MAX_DIM]) {
-#pragma clang fp contract(off)
-  double X[HPSP_HIST_MAX_DIM];
-  bool under = true, over = true;
-#pragma unroll
-  for (int d = 0; d < HPSP_HIST_MAX_DIM; ++d) {
-    if (d < h.ndim) {
-      X[d] = (x[d] - h.lower[d]) * h.gridf[d] / h.delta[d];
-      under = under && (X[d] < 0.0);
-      over = over && (X[d] > h.gridf[d]);
-    }
-  }
-  if (under) return h.nbins;
-  if (over) return h.nbins + 1;
-  uint64_t key = 0;
-#pragma unroll
-  for (int d = 0; d < HPSP_HIST_MAX_DIM; ++d) {
-    if (d < h.ndim) {
-      if (!(X[d] > -1.0) || X[d] >= 0x1p64) return kDrop;
-      const uint64_t idx = X[d] < 0.0 ? 0 : (uint64_t)X[d];
-      key += h.stride[d] * idx;
-    }
-  }
-  return key <= h.nbins + 1 ? key : kDrop;
-}
 
 template <int N>
 __device__ __forceinline__ void event_axes(const HistArgs& a, uint64_t k,
@@ -120,6 +87,11 @@ __global__ void k_zero(double* p, uint64_t n) {
     p[i] = 0.0;
 }
 
+void launch_zero(double* p, uint64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<uint64_t>((n + 255) / 256, 1024)),
+                     dim3(256), 0, s, p, n);
+}
+
 template <int N>
 static void hist_one(const HistArgs& a, unsigned grid, size_t lds, hipStream_t s) {
   hipLaunchKernelGGL(k_hist_fill<N>, dim3(grid), dim3(kHistBlock), lds, s, a);
@@ -136,8 +108,7 @@ static const auto kHist = hist_table(std::make_integer_sequence<int, HPSP_MAX_N>
 int launch_histogram_fill(int N, const HistArgs& in, hipStream_t s) {
   HistArgs a = in;
   const uint64_t nslots = a.h.nbins + 2;
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<uint64_t>((nslots + 255) / 256, 1024)),
-                     dim3(256), 0, s, a.contents, nslots);
+  launch_zero(a.contents, nslots, s);
   if (a.n == 0) return (int)hipGetLastError();
   a.use_lds = nslots <= a.lds_slots ? 1 : 0;
   const size_t lds = a.use_lds ? nslots * sizeof(double) : 0;

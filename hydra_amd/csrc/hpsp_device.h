@@ -288,11 +288,55 @@ __device__ __forceinline__ void r123_words(const uint64_t* key, uint64_t d, uint
   }
 }
 
-// word -> double in [0,1) (uniform_real_distribution.inl:62-80).
+// minstd_rand: x -> 48271 x mod (2^31 - 1) (linear_congruential_engine.inl:
+// 53-61, linear_congruential_engine.h:138-142).  With m a Mersenne prime,
+// p mod m folds as (p & m) + (p >> 31): for x, y < m the sum is at most
+// 2m - 1, so one conditional subtraction completes it.
+constexpr uint32_t kMinstdA = 48271u;
+constexpr uint32_t kMinstdM = 0x7fffffffu;
+
+__host__ __device__ __forceinline__ uint32_t minstd_mul(uint32_t x, uint32_t y) {
+  const uint64_t p = (uint64_t)x * y;
+  const uint32_t r = (uint32_t)(p & kMinstdM) + (uint32_t)(p >> 31);
+  return r >= kMinstdM ? r - kMinstdM : r;
+}
+
+// 48271^e mod m by square-and-multiply (the reference's discard,
+// linear_congruential_engine_discard.h:49-80, forms the same power).
+__host__ __device__ __forceinline__ uint32_t minstd_pow(uint64_t e) {
+  uint32_t r = 1, b = kMinstdA;
+  while (e) {
+    if (e & 1) r = minstd_mul(r, b);
+    b = minstd_mul(b, b);
+    e >>= 1;
+  }
+  return r;
+}
+
+// The engine's state after seed(s): s mod m, or 1 where that is 0
+// (linear_congruential_engine.inl:40-50).
+__host__ __device__ __forceinline__ uint32_t minstd_seed(uint32_t s) {
+  const uint32_t x = s % kMinstdM;
+  return x ? x : 1u;
+}
+
+constexpr int kMinstd = HPSP_MINSTD_RAND;
+
+// Engines whose words fit 32 bits.
+template <int ENGINE>
+constexpr bool narrow_words() {
+  return is_squares<ENGINE>() || ENGINE == kMinstd;
+}
+
+// word -> double in [0,1) (uniform_real_distribution.inl:62-80):
+// (word - min) / (1 + (max - min)).  minstd_rand has min 1 and max m - 1, so
+// (x - 1) / 2147483646, a rounded division.
 template <int ENGINE>
 __device__ __forceinline__ double to_unit(uint64_t w) {
 #pragma clang fp contract(off)
-  if constexpr (is_squares<ENGINE>())
+  if constexpr (ENGINE == kMinstd)
+    return (double)((uint32_t)w - 1u) / 2147483646.0;
+  else if constexpr (is_squares<ENGINE>())
     return (double)(uint32_t)w * 0x1p-32;
   else
     return (double)w * 0x1p-64;
@@ -526,13 +570,21 @@ __device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p
     const Real s2 = p[0][2] + p[1][2], s3 = p[0][3] + p[1][3];
     return (double)(s0 * s0 - s1 * s1 - s2 * s2 - s3 * s3);
   }
-  if (f.kind == HPSP_F_MASS2) {
+  if (f.kind == HPSP_F_MASS2 || f.kind == HPSP_F_MASS) {
     Real u[4], v[4], s[4];
     pick4(p, f.a[0], u);
     pick4(p, f.b[0], v);
 #pragma unroll
     for (int c = 0; c < 4; ++c) s[c] = u[c] + v[c];
-    return (double)dot4(s, s);
+    const Real m2 = dot4(s, s);
+    if (f.kind == HPSP_F_MASS2) return (double)m2;
+    // Vector4R::mass: 0 unless the square is positive
+    if constexpr (sizeof(Real) == 4)
+      return m2 > 0.0f ? (double)__builtin_sqrtf(m2) : 0.0;
+    else if constexpr (Exact)
+      return m2 > 0.0 ? sqrt_cr(m2) : 0.0;
+    else
+      return m2 > 0.0 ? sqrt_fast(m2) : 0.0;
   }
   if (f.kind == HPSP_F_COS_HELICITY) {
     // P = p[first] + ... + p[first + count - 1] left to right (first, count

@@ -39,6 +39,16 @@ inline int check_decay(const hpsp_decay* d) {
   return HPSP_OK;
 }
 
+// The entry points that read only the masses and fMaxWeight of a decay
+// (PhaseSpaceWeight, unweighting, fills from stored events) also take a
+// phase_space_range description.
+inline int check_decay_masses(const hpsp_decay* d) {
+  if (!d) return HPSP_ERR_INVALID_ARGUMENT;
+  if (d->n < 2 || d->n > HPSP_MAX_N) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!valid_engine(d->engine) && d->engine != HPSP_MINSTD_RAND) return HPSP_ERR_INVALID_ARGUMENT;
+  return HPSP_OK;
+}
+
 inline DecayK make_decayk(const hpsp_decay* d, uint64_t seed) {
   DecayK k;
   memset(&k, 0, sizeof(k));
@@ -79,9 +89,9 @@ inline int make_functork(const hpsp_functor* f, int nvec, FunctorK* out,
     out->kind = HPSP_F_ONE;
     return HPSP_OK;
   }
-  if (f->kind < HPSP_F_ONE || f->kind > HPSP_F_COS_HELICITY) return HPSP_ERR_INVALID_ARGUMENT;
+  if (f->kind < HPSP_F_ONE || f->kind > HPSP_F_MASS) return HPSP_ERR_INVALID_ARGUMENT;
   out->kind = f->kind;
-  if (f->kind == HPSP_F_MASS2 || f->kind == HPSP_F_COS_HELICITY) {
+  if (f->kind == HPSP_F_MASS2 || f->kind == HPSP_F_COS_HELICITY || f->kind == HPSP_F_MASS) {
     if (f->a[0] < 0 || f->a[0] >= nvec || f->b[0] < 0 || f->b[0] >= nvec)
       return HPSP_ERR_INVALID_ARGUMENT;
     out->a[0] = f->a[0];

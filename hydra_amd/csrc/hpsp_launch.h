@@ -73,6 +73,25 @@ struct HistArgs {
   uint64_t lds_slots;       // LDS capacity in doubles (launcher)
 };
 
+// phase_space_range: the DecayMother events of a minstd_rand stream.  Word j
+// of element first + k is x_start * 48271^(k + j) mod (2^31 - 1), x_start
+// being the engine's state after discard(first + 3N) and one step.
+struct RangeArgs {
+  DecayK d;            // key words unused
+  uint32_t x_start;
+  uint32_t a_stride;   // 48271^(grid * kBlock): the state advance of one grid pass
+  uint64_t n;
+  double* cols;        // generate: 4N momentum columns of ld doubles
+  uint64_t ld;
+  double* wt;          // generate: weights or null
+  HistK h;             // histogram fill
+  FunctorK axes[HPSP_HIST_MAX_DIM];
+  int weight_mode;
+  double* contents;
+  int use_lds;
+  uint64_t lds_slots;
+};
+
 struct WeightArgs {
   DecayK d;
   FunctorK f;
@@ -114,6 +133,9 @@ int launch_average_f64(int N, int engine, const AvgArgs& a, hipStream_t s);
 int launch_average_mixed(int N, int engine, const AvgArgs& a, hipStream_t s);
 int launch_evaluate_f64(int N, int engine, bool mothers, const EvalArgs& a, hipStream_t s);
 int launch_histogram_fill(int N, const HistArgs& a, hipStream_t s);  // N = 0: columns
+void launch_zero(double* p, uint64_t n, hipStream_t s);
+int launch_range_generate(int N, const RangeArgs& a, hipStream_t s);
+int launch_range_histogram(int N, const RangeArgs& a, hipStream_t s);
 int launch_stats_finalize(const Stats* partials, uint64_t m, Stats* out, hipStream_t s);
 int launch_event_weight(int N, const WeightArgs& a, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,

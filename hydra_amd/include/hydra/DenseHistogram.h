@@ -8,6 +8,9 @@
 // reference's does.  Data come from
 //   * value columns: a multivector of N columns (for example Evaluate's
 //     functor columns) with optional weights;
+//   * phase_space_range elements: `range | hydra::make_axes(f1, ..., fN)`,
+//     generated inside the fill, unweighted or weighted by
+//     range.GetEventWeightFunctor() (each element's own weight);
 //   * the events themselves: `events | hydra::make_axes(f1, ..., fN)` names N
 //     device-described functors of the daughters (the reference's
 //     `events | dalitz_calculator`); the fill computes them on the fly, with
@@ -28,6 +31,7 @@
 
 #include <hydra/Decays.h>
 #include <hydra/PhaseSpaceFunctors.h>
+#include <hydra/PhaseSpaceRange.h>
 #include <hydra/detail/Runtime.h>
 #include <hydra/device/System.h>
 #include <hydra/multivector.h>
@@ -70,6 +74,26 @@ EventAxes<sizeof...(F), Decays<hydra::tuple<Particles...>, hydra::device::sys_t>
     const Decays<hydra::tuple<Particles...>, hydra::device::sys_t>& ev, AxesSpec<F...> const& s) {
   return EventAxes<sizeof...(F), Decays<hydra::tuple<Particles...>, hydra::device::sys_t>>(
       ev, s.axes);
+}
+
+// range | make_axes(f...): the axes of phase_space_range elements, generated
+// inside DenseHistogram::Fill.
+template <size_t D, size_t N>
+class RangeAxes {
+ public:
+  RangeAxes(const PhaseSpaceRange<N>& r, std::array<hpsp_functor, D> const& axes)
+      : range_(&r), axes_(axes) {}
+  const PhaseSpaceRange<N>& range() const { return *range_; }
+  const std::array<hpsp_functor, D>& axes() const { return axes_; }
+
+ private:
+  const PhaseSpaceRange<N>* range_;
+  std::array<hpsp_functor, D> axes_;
+};
+
+template <size_t N, typename... F>
+RangeAxes<sizeof...(F), N> operator|(const PhaseSpaceRange<N>& r, AxesSpec<F...> const& s) {
+  return RangeAxes<sizeof...(F), N>(r, s.axes);
 }
 
 template <typename T, size_t N, typename Backend = hydra::device::sys_t>
@@ -118,6 +142,17 @@ class DenseHistogram<double, N, hydra::device::sys_t> {
                        PhaseSpaceWeight<Particles...> const& weight) {
     const hpsp_decay& d = weight.descriptor();
     return fill_events(axes, HPSP_HIST_WEIGHT_PHSP, nullptr, &d);
+  }
+
+  // Fills over phase_space_range elements (range_semantics.inl:154-167):
+  // unweighted, or weighted by each element's weight
+  template <size_t NR>
+  DenseHistogram& Fill(RangeAxes<N, NR> const& axes) {
+    return fill_range(axes, HPSP_HIST_WEIGHT_ONE);
+  }
+  template <size_t NR>
+  DenseHistogram& Fill(RangeAxes<N, NR> const& axes, RangeEventWeight) {
+    return fill_range(axes, HPSP_HIST_WEIGHT_GENERATION);
   }
 
   size_t GetNBins() const { return (size_t)nbins_; }
@@ -186,6 +221,18 @@ class DenseHistogram<double, N, hydra::device::sys_t> {
         dw ? *dw : detail::make_decay<ND>(ev.GetMotherMass(), masses, "DenseHistogram");
     detail::hpsp_check(hpsp_histogram_fill_events(&h_, &d, ax.axes().data(), ev.data(), ev.ld(),
                                                   ev.size(), mode, w, contents_.data(), nullptr),
+                       "DenseHistogram::Fill");
+    detail::hpsp_sync("DenseHistogram::Fill");
+    return *this;
+  }
+
+  template <size_t NR>
+  DenseHistogram& fill_range(RangeAxes<N, NR> const& ax, int mode) {
+    const PhaseSpaceRange<NR>& r = ax.range();
+    if (r.size() == std::numeric_limits<size_t>::max())
+      throw std::invalid_argument("[hydra::DenseHistogram::Fill]: unbounded range");
+    detail::hpsp_check(hpsp_histogram_fill_range(&h_, &r.descriptor(), r.seed(), ax.axes().data(),
+                                                 0, r.size(), mode, contents_.data(), nullptr),
                        "DenseHistogram::Fill");
     detail::hpsp_sync("DenseHistogram::Fill");
     return *this;

@@ -17,6 +17,7 @@
 
 #include <hydra/Decays.h>
 #include <hydra/PhaseSpaceFunctors.h>
+#include <hydra/PhaseSpaceRange.h>
 #include <hydra/Types.h>
 #include <hydra/Vector4R.h>
 #include <hydra/detail/Runtime.h>

@@ -6,6 +6,7 @@
 // hpsp_functor of include/hpsp.h):
 //   ConstantFunctor          f = 1
 //   InvariantMass2{a, b}     f = (p[a] + p[b])^2 (default the pair (0, 1))
+//   InvariantMass{a, b}      f = (p[a] + p[b]).mass() (Vector4R::mass)
 //   CosHelicity{d, e}        helicity cosine of p[d] in the pair (d, e) within
 //                            the sum of the daughters: the cos(theta_K*) of
 //                            examples/phase_space/phsp_evaluating_functor.inl:160-176
@@ -29,6 +30,11 @@ namespace hydra {
 struct ConstantFunctor {};
 
 struct InvariantMass2 {
+  int a = 0;
+  int b = 1;
+};
+
+struct InvariantMass {
   int a = 0;
   int b = 1;
 };
@@ -61,6 +67,7 @@ template <typename F>
 struct is_phase_space_functor
     : std::integral_constant<bool, std::is_same<F, ConstantFunctor>::value ||
                                        std::is_same<F, InvariantMass2>::value ||
+                                       std::is_same<F, InvariantMass>::value ||
                                        std::is_same<F, CosHelicity>::value ||
                                        std::is_same<F, BreitWignerSum>::value> {};
 
@@ -75,6 +82,14 @@ inline hpsp_functor to_hpsp(const ConstantFunctor&) {
 inline hpsp_functor to_hpsp(const InvariantMass2& m) {
   hpsp_functor f{};
   f.kind = HPSP_F_MASS2;
+  f.a[0] = m.a;
+  f.b[0] = m.b;
+  return f;
+}
+
+inline hpsp_functor to_hpsp(const InvariantMass& m) {
+  hpsp_functor f{};
+  f.kind = HPSP_F_MASS;
   f.a[0] = m.a;
   f.b[0] = m.b;
   return f;

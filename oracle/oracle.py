@@ -25,6 +25,7 @@ REF_NATIVE_LIB = os.path.join(HERE, "_ref", "libhydra_ref_native.so")
 
 ENGINES = {"squares3": 0, "squares4": 1, "philox": 2, "philox_long": 3,
            "threefry": 4, "threefry_long": 5}
+MINSTD = 6  # thrust's default_random_engine (minstd_rand): phase_space_range
 
 _P = ctypes.c_void_p
 _D = ctypes.c_double
@@ -45,13 +46,13 @@ FPAR = 32  # stride of one functor's parameters in the evaluate calls
 
 def functor_params(fid, terms=None):
     """Pack functor parameters.  fid 1: terms = [(M, G, cr, ci, a, b), ...];
-    fid 3: terms = (a, b); fid 4: terms = (d, e, first, count)."""
+    fid 3 and 5: terms = (a, b); fid 4: terms = (d, e, first, count)."""
     if fid == 1:
         flat = [float(len(terms))]
         for t in terms:
             flat.extend(float(x) for x in t)
         return np.array(flat, dtype=np.float64)
-    if fid in (3, 4):
+    if fid in (3, 4, 5):
         return np.array([float(x) for x in terms], dtype=np.float64)
     return np.zeros(1)
 
@@ -85,6 +86,7 @@ class Oracle:
         L.po_uniform.restype = _D
         L.po_params.argtypes = [_I, _D, _P, _P, _P]
         L.po_mother_beta.argtypes = [_P, _P]
+        L.po_range_params.argtypes = [_I, _P, _P, _P, _P, _P]
         L.po_decay_events.argtypes = [_I, _I, _P, _D, _D, _P, _U64, _P, _U64, _P, _P]
         L.po_generate.argtypes = [_I, _I, _P, _D, _D, _P, _U64, _U64, _U64, _P, _P, _I]
         L.po_generate_mothers.argtypes = [_I, _I, _P, _D, _D, _U64, _U64, _U64, _P, _P, _P, _I]
@@ -166,6 +168,30 @@ class Oracle:
         self.L.po_generate(N, ENGINES.get(engine, engine), _ptr(m), ecm, mw, _ptr(beta), seed,
                            first, n, _ptr(cols), _ptr(wt), nthreads)
         return cols, wt
+
+    def range_params(self, mother4, masses):
+        """(rc, fTeCmTm, fWtMax, beta) of GenerateDecay(mother, masses, seed)."""
+        m = _f64(masses)
+        m4 = _f64(mother4)
+        ecm, mw = _D(), _D()
+        b = np.zeros(3)
+        rc = self.L.po_range_params(len(m), _ptr(m4), _ptr(m), ctypes.byref(ecm),
+                                    ctypes.byref(mw), _ptr(b))
+        return rc, ecm.value, mw.value, b
+
+    def phase_space_range(self, mother4, masses, seed, n, first=0, nthreads=0):
+        """Elements first..first+n-1 of phase_space_range(mother, masses, seed):
+        (weights (n,), SoA momenta (4N, n)).  The size_t seed reaches the
+        engine as (uint32)(int)seed."""
+        m = _f64(masses)
+        N = len(m)
+        rc, ecm, mw, beta = self.range_params(mother4, m)
+        assert rc == 0
+        cols = np.zeros((4 * N, n))
+        wt = np.zeros(n)
+        self.L.po_generate(N, MINSTD, _ptr(m), ecm, mw, _ptr(beta), seed & 0xFFFFFFFF, first,
+                           n, _ptr(cols), _ptr(wt), nthreads)
+        return wt, cols
 
     def generate_mothers(self, M, masses, seed, mothers, first=0, engine=0, nthreads=0):
         m = _f64(masses)
@@ -325,9 +351,36 @@ class Reference:
         L.hr_evaluate_mothers.argtypes = [_I, _D, _P, ctypes.c_int32, _U64, _P, _I, _P, _P, _P]
         L.hr_histogram_fill.argtypes = [_I, _P, _P, _P, _P, _U64, _P, _P, _P]
         L.hr_unweight_sample.argtypes = [_P, _U64, _D, _D, _D, _U64, _U64, _P, _P, _P]
+        L.hr_phase_space_range.argtypes = [_I, _P, _P, _U64, _U64, _U64, _P, _P]
+        L.hr_range_histogram3.argtypes = [_P, _P, _U64, _U64, _I, _I, _I, _U64, _D, _D, _P, _P]
 
     def set_threads(self, n):
         return self.L.hr_set_threads(n)
+
+    def phase_space_range(self, mother4, masses, seed, n, first=0):
+        """Elements first..first+n-1 of hydra::phase_space_range: (weights (n,),
+        AoS momenta (n, N, 4))."""
+        m = _f64(masses)
+        N = len(m)
+        p = np.zeros((n, N, 4))
+        w = np.zeros(n)
+        rc = self.L.hr_phase_space_range(N, _ptr(_f64(mother4)), _ptr(m), seed, first, n,
+                                         _ptr(p), _ptr(w))
+        if rc:
+            raise ValueError("bad N")
+        return w, p
+
+    def range_histogram3(self, mother4, masses, seed, n, a, b, bins, lo, hi, weighted=False):
+        """DenseHistogram<double,1>.Fill(phase_space_range(...) | (p_a+p_b).mass()
+        [, weights]): (contents (bins+2,), seconds of the Fill)."""
+        out = np.zeros(bins + 2)
+        sec = _D()
+        rc = self.L.hr_range_histogram3(_ptr(_f64(mother4)), _ptr(_f64(masses)), seed, n, a, b,
+                                        1 if weighted else 0, bins, lo, hi, _ptr(out),
+                                        ctypes.byref(sec))
+        if rc:
+            raise ValueError("bad pair")
+        return out, sec.value
 
     def rng_words(self, engine, seed, discard, n):
         out = np.zeros(n, dtype=np.uint64)

@@ -170,6 +170,17 @@ void po_threefry4x64(unsigned R, const uint64_t ctr[4], const uint64_t key[4], u
  *               block evaluation; words are served from the cached block;
  *               discard(n) adds n blocks and drops the cache (EngineR123.h:148-173).
  */
+/*   minstd_rand (thrust's default_random_engine, the engine of
+ *               phase_space_range): linear_congruential_engine<uint32, 48271,
+ *               0, 2^31-1> (linear_congruential_engine.h:138-142,
+ *               linear_congruential_engine.inl:40-61): seed(s) keeps
+ *               s mod m, or 1 when that is 0; operator() steps
+ *               x = 48271 x mod m and returns it; discard(z) multiplies by
+ *               48271^z mod m (linear_congruential_engine_discard.h:49-80).
+ *               The state lives in ctr[0].                                  */
+#define PO_MINSTD_A 48271ULL
+#define PO_MINSTD_M 2147483647ULL
+
 typedef struct {
   int engine;
   unsigned arity;
@@ -192,7 +203,10 @@ static void engine_init(po_engine* g, int engine, uint64_t seed) {
   g->engine = engine;
   g->arity = engine_arity(engine);
   uint64_t s = seed;
-  if (engine == 0 || engine == 1) {
+  if (engine == 6) {
+    uint64_t x = (uint64_t)(uint32_t)seed % PO_MINSTD_M;
+    g->ctr[0] = x ? x : 1;
+  } else if (engine == 0 || engine == 1) {
     g->key[0] = po_splitmix64(&s);
   } else {
     unsigned nkey = (engine == 2) ? 1 : (engine == 3) ? 2 : (engine == 4) ? 2 : 4;
@@ -210,8 +224,21 @@ static void ctr_incr(uint64_t* v, unsigned width, uint64_t n) {
   }
 }
 
+/* a^z mod m by square-and-multiply over the bits of z */
+static uint64_t minstd_pow(uint64_t z) {
+  uint64_t r = 1, b = PO_MINSTD_A;
+  while (z) {
+    if (z & 1) r = r * b % PO_MINSTD_M;
+    b = b * b % PO_MINSTD_M;
+    z >>= 1;
+  }
+  return r;
+}
+
 static void engine_discard(po_engine* g, uint64_t n) {
-  if (g->engine == 0 || g->engine == 1) {
+  if (g->engine == 6) {
+    g->ctr[0] = minstd_pow(n) * g->ctr[0] % PO_MINSTD_M;
+  } else if (g->engine == 0 || g->engine == 1) {
     g->ctr[0] += n;
   } else {
     ctr_incr(g->ctr, g->arity, n);
@@ -230,6 +257,10 @@ static void engine_block(po_engine* g) {
 }
 
 static uint64_t engine_next(po_engine* g) {
+  if (g->engine == 6) {
+    g->ctr[0] = g->ctr[0] * PO_MINSTD_A % PO_MINSTD_M;
+    return g->ctr[0];
+  }
   if (g->engine == 0) return po_squares3(g->key[0], g->ctr[0]++);
   if (g->engine == 1) return po_squares4(g->key[0], g->ctr[0]++);
   if (g->trig == g->arity) {
@@ -244,8 +275,14 @@ static uint64_t engine_next(po_engine* g) {
  * (hydra_thrust/random/detail/uniform_real_distribution.inl:62-80):
  * double(word - min) / (1 + double(max - min)), then *(b-a)+a with a=0,b=1.
  * For 32-bit words this is word * 2^-32 exactly; for 64-bit words the
- * conversion rounds and the result can equal 1.0.                          */
+ * conversion rounds and the result can equal 1.0.  minstd_rand has min 1 and
+ * max m-1: (word - 1) / (m - 1), a rounded division.                        */
 double po_uniform(int engine, uint64_t word) {
+  if (engine == 6) {
+    double r = (double)(uint32_t)((uint32_t)word - 1u);
+    r /= (1.0 + (double)(uint32_t)(2147483646u - 1u));
+    return r * (1.0 - 0.0) + 0.0;
+  }
   double r = (double)word;
   if (engine == 0 || engine == 1)
     r /= (1.0 + (double)UINT32_MAX);
@@ -259,7 +296,7 @@ static inline double engine_uniform(po_engine* g) {
 }
 
 int po_rng_words(int engine, uint64_t seed, uint64_t discard, uint64_t n, uint64_t* out) {
-  if (engine < 0 || engine > 5) return -1;
+  if (engine < 0 || engine > 6) return -1;
   po_engine g;
   engine_init(&g, engine, seed);
   engine_discard(&g, discard);
@@ -347,6 +384,21 @@ void po_mother_beta(const double* m4, double* b) {
   } else {
     b[0] = b[1] = b[2] = 0.0;
   }
+}
+
+/* GenerateDecay<N, minstd_rand>(mother, masses, seed), the functor behind
+ * phase_space_range (PhaseSpaceRange.inl:44-72, GenerateDecay.h:64-101):
+ * fTeCmTm = mother.mass() - sum(m), fWtMax from the same pdk product as the
+ * PhaseSpace constructor, and the DecayMother boost vector.  The reference
+ * checks nothing here; -1 / -2 flag what the PhaseSpace constructor would
+ * reject (a non-positive mass, fTeCmTm <= 0).  The element k of the range is
+ * then the DecayMother event k of engine 6 seeded with (uint32)(int)seed.   */
+int po_range_params(int N, const double* mother4, const double* masses, double* ecm,
+                    double* max_weight, double* beta3) {
+  int rc = po_params(N, mass(mother4), masses, ecm, max_weight);
+  if (rc) return rc;
+  po_mother_beta(mother4, beta3);
+  return 0;
 }
 
 /* GENBOD body: DecayMother::process (DecayMother.h:163-276) and
@@ -521,6 +573,13 @@ double po_functor(int fid, const double* fpar, int N, const double* p) {
     double s[4];
     for (int c = 0; c < 4; ++c) s[c] = p[4 * a + c] + p[4 * b + c];
     return mass2(s);
+  }
+  if (fid == 5) {
+    /* (p_a + p_b).mass(): Vector4R::mass (Vector4R.inl:196-208) */
+    int a = (int)fpar[0], b = (int)fpar[1];
+    double s[4];
+    for (int c = 0; c < 4; ++c) s[c] = p[4 * a + c] + p[4 * b + c];
+    return mass(s);
   }
   if (fid == 4) {
     /* helicity cosine of particle d in the pair (d, e) within the sum of

@@ -9,7 +9,8 @@
  * Conventions
  *   - engines: 0 squares3, 1 squares4, 2 philox (2x64, R=7),
  *              3 philox_long (4x64, R=7), 4 threefry (2x64, R=12),
- *              5 threefry_long (4x64, R=12)
+ *              5 threefry_long (4x64, R=12),
+ *              6 minstd_rand (thrust default_random_engine, phase_space_range)
  *   - SoA event columns: column (4*i + c) of particle i, component c
  *     (c = 0:E, 1:px, 2:py, 3:pz) starts at cols + (4*i + c) * n
  *   - AoS event rows: p[(k*N + i)*4 + c]
@@ -21,6 +22,7 @@
  *               id 4: helicity cosine of particle d in the pair (d, e) within
  *                     P = p_first + ... + p_(first+count-1),
  *                     par = {d, e, first, count}
+ *               id 5: f = (p_a + p_b).mass(), par = {a, b}
  *   - evaluate outputs: column 0 the weight, column 1+j functor j (ld = n);
  *     functor j's parameters at fpar + PO_FPAR*j
  */
@@ -48,6 +50,8 @@ double po_uniform(int engine, uint64_t word);
 
 int po_params(int N, double M, const double* masses, double* ecm, double* max_weight);
 void po_mother_beta(const double* mother4, double* beta3);
+int po_range_params(int N, const double* mother4, const double* masses, double* ecm,
+                    double* max_weight, double* beta3);
 
 void po_decay_events(int N, int engine, const double* masses, double ecm, double max_weight,
                      const double* beta3, uint64_t seed, const uint64_t* evts, uint64_t nevt,

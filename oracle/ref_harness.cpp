@@ -9,6 +9,8 @@
 //   hydra::detail::DecayMother::process  hydra/detail/functors/DecayMother.h:163-276
 //   engines squares3/4, philox(_long), threefry(_long)
 //                                        hydra/detail/random/{squares3,squares4,EngineR123}.h
+//   hydra::phase_space_range             hydra/detail/PhaseSpaceRange.inl:44-72,
+//                                        detail/functors/GenerateDecay.h
 //
 // It is test infrastructure: only tests/, bench.py's CPU-baseline leg and the
 // golden-vector script load the library built from it (oracle/_ref/).
@@ -26,6 +28,7 @@
 #include <hydra/Parameter.h>
 #include <hydra/functions/Gaussian.h>
 #include <hydra/detail/Hash.h>
+#include <hydra/Range.h>
 
 #include <omp.h>
 #include <array>
@@ -86,6 +89,12 @@ inline void row_to_array(Row const& row, double* out, std::index_sequence<I...>)
     for (int c = 0; c < 4; ++c) out[4 * p + c] = v[p].get(c);
 }
 
+// Elements 1..N of a (weight, p_1, ..., p_N) tuple into an array.
+template <size_t N, typename Tuple, size_t... I>
+inline void assign_tail(Tuple const& t, hydra::Vector4R* p, std::index_sequence<I...>) {
+  ((p[I] = hydra::get<I + 1>(t)), ...);
+}
+
 // ---------------------------------------------------------------------------
 // Functors used by AverageOn / Unweight(functor).  Their arithmetic is spelled
 // out so the GPU kernels and the C restatement can follow the same sequence.
@@ -97,6 +106,7 @@ inline void row_to_array(Row const& row, double* out, std::index_sequence<I...>)
 //   id 4: helicity cosine of particle d in the pair (d, e) within
 //         P = p_first + ... + p_(first+count-1), written as
 //         phsp_evaluating_functor.inl:160-176 writes it  par = {d, e, first, count}
+//   id 5: f = (p_a + p_b).mass()                        par = {a, b}
 struct Functor {
   int id;
   int K;
@@ -113,6 +123,10 @@ struct Functor {
     if (id == 3) {
       hydra::Vector4R s = p[a[0]] + p[b[0]];
       return s.mass2();
+    }
+    if (id == 5) {
+      hydra::Vector4R s = p[a[0]] + p[b[0]];
+      return s.mass();
     }
     if (id == 4) {
       hydra::Vector4R P = p[a[1]];
@@ -148,7 +162,7 @@ struct Functor {
 Functor make_functor(int id, const double* par) {
   Functor f{};
   f.id = id;
-  if (id == 3) {
+  if (id == 3 || id == 5) {
     f.a[0] = (int)par[0];
     f.b[0] = (int)par[1];
   }
@@ -430,6 +444,26 @@ void evaluate_mothers(double M, const double* masses, int32_t seed, uint64_t n,
   }
 }
 
+// Elements first..first+n-1 of hydra::phase_space_range(mother, masses, seed,
+// first + n), read through the range's own iterator: weights and AoS momenta.
+template <size_t N>
+void phase_space_range(const double* mother4, const double* masses, uint64_t seed,
+                       uint64_t first, uint64_t n, double* out_p, double* out_w) {
+  std::array<double, N> m;
+  for (size_t i = 0; i < N; ++i) m[i] = masses[i];
+  hydra::Vector4R mother(mother4[0], mother4[1], mother4[2], mother4[3]);
+  auto events = hydra::phase_space_range(mother, m, (size_t)seed, (size_t)(first + n));
+  auto it = events.begin() + first;
+  for (uint64_t k = 0; k < n; ++k, ++it) {
+    auto t = *it;
+    out_w[k] = hydra::get<0>(t);
+    hydra::Vector4R p[N];
+    assign_tail<N>(t, p, std::make_index_sequence<N>{});
+    for (size_t i = 0; i < N; ++i)
+      for (int c = 0; c < 4; ++c) out_p[(k * N + i) * 4 + c] = p[i].get(c);
+  }
+}
+
 // Runtime N dispatch (2..12).
 template <template <size_t> class Op, typename... Args>
 int dispatch(int N, Args&&... args) {
@@ -463,6 +497,9 @@ template <size_t N> struct UwOp {
 };
 template <size_t N> struct EvalOp {
   template <typename... A> static void run(A... a) { evaluate<N>(a...); }
+};
+template <size_t N> struct RangeOp {
+  template <typename... A> static void run(A... a) { phase_space_range<N>(a...); }
 };
 template <size_t N> struct EvalMothersOp {
   template <typename... A> static void run(A... a) { evaluate_mothers<N>(a...); }
@@ -505,7 +542,8 @@ int hr_set_threads(int nthreads) {
 }
 
 // Engine words: GRND eng(seed); eng.discard(discard); out[i] = eng().
-// engine: 0 squares3, 1 squares4, 2 philox, 3 philox_long, 4 threefry, 5 threefry_long
+// engine: 0 squares3, 1 squares4, 2 philox, 3 philox_long, 4 threefry, 5 threefry_long,
+// 6 thrust's default_random_engine (minstd_rand, seeded with the low 32 bits)
 int hr_rng_words(int engine, uint64_t seed, uint64_t discard, uint64_t n, uint64_t* out) {
   switch (engine) {
     case 0: engine_words<hydra::random::squares3>(seed, discard, n, out); return 0;
@@ -514,6 +552,8 @@ int hr_rng_words(int engine, uint64_t seed, uint64_t discard, uint64_t n, uint64
     case 3: engine_words<hydra::random::philox_long>(seed, discard, n, out); return 0;
     case 4: engine_words<hydra::random::threefry>(seed, discard, n, out); return 0;
     case 5: engine_words<hydra::random::threefry_long>(seed, discard, n, out); return 0;
+    case 6: engine_words<hydra::thrust::random::default_random_engine>(seed, discard, n, out);
+      return 0;
     default: return -1;
   }
 }
@@ -646,6 +686,45 @@ int hr_unweight_sample(const double* x, uint64_t n, double mean, double sigma, d
   auto r = hydra::unweight(hydra::device::sys, data, gauss, max_pdf, (size_t)seed, (size_t)jump);
   *n_kept = (uint64_t)r.size();
   for (uint64_t i = 0; i < n; ++i) x_out[i] = data[i];
+  return 0;
+}
+
+// phase_space_range elements (weights, AoS momenta n x N x 4).
+int hr_phase_space_range(int N, const double* mother4, const double* masses, uint64_t seed,
+                         uint64_t first, uint64_t n, double* out_p, double* out_w) {
+  return dispatch<RangeOp>(N, mother4, masses, seed, first, n, out_p, out_w);
+}
+
+// The range histogram of examples/misc/range_semantics.inl:154-167 for three
+// daughters: DenseHistogram<double, 1, device>(bins, lo, hi).Fill(events | m_ab)
+// with m_ab = (p_a + p_b).mass() over the first n elements of
+// phase_space_range(mother, masses, seed, n); weighted by each element's weight
+// when weighted != 0.  contents receives bins + 2 values (underflow, overflow
+// last); seconds the time of the Fill.
+int hr_range_histogram3(const double* mother4, const double* masses, uint64_t seed, uint64_t n,
+                        int a, int b, int weighted, uint64_t bins, double lo, double hi,
+                        double* contents, double* seconds) {
+  if (a < 0 || a > 2 || b < 0 || b > 2) return -1;
+  std::array<double, 3> m{masses[0], masses[1], masses[2]};
+  hydra::Vector4R mother(mother4[0], mother4[1], mother4[2], mother4[3]);
+  auto events = hydra::phase_space_range(mother, m, (size_t)seed, (size_t)n);
+  auto mass_ab = hydra::wrap_lambda(
+      [a, b] __hydra_dual__(double w, hydra::Vector4R A, hydra::Vector4R B, hydra::Vector4R C) {
+        hydra::Vector4R v[3] = {A, B, C};
+        return (v[a] + v[b]).mass();
+      });
+  auto weight = hydra::wrap_lambda(
+      [] __hydra_dual__(double w, hydra::Vector4R A, hydra::Vector4R B, hydra::Vector4R C) {
+        return w;
+      });
+  hydra::DenseHistogram<double, 1, hydra::device::sys_t> H((size_t)bins, lo, hi);
+  double t0 = now_s();
+  if (weighted)
+    H.Fill(events | mass_ab, events | weight);
+  else
+    H.Fill(events | mass_ab);
+  if (seconds) *seconds = now_s() - t0;
+  for (uint64_t k = 0; k < bins + 2; ++k) contents[k] = H.GetBinContent((size_t)k);
   return 0;
 }
 

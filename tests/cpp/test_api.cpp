@@ -240,6 +240,49 @@ int main(int argc, char** argv) {
     H2.Fill(dalitz_variables);
     emit_array("hist_dalitz_n", H2.GetBinsContents());
     std::printf("\"hist_bin_3_7\": %.17g,\n", H2.GetBinContent(std::array<size_t, 2>{3, 7}));
+
+    // phase_space_range as examples/misc/range_semantics.inl:154-167 uses it:
+    // K+ -> 3 pi at rest, seed 321, the m(pi pi) histogram of the lazy range
+    std::array<double, 3> pions{0.13957061, 0.13957061, 0.13957061};
+    auto range = hydra::phase_space_range(hydra::Vector4R(0.493677, 0.0, 0.0, 0.0), pions, 321,
+                                          100000);
+    hydra::DenseHistogram<double, 1, hydra::device::sys_t> Hist_Mass(
+        10, pions[0] + pions[1], 0.493677 - pions[0]);
+    Hist_Mass.Fill(range | hydra::make_axes(hydra::InvariantMass{0, 1}));
+    emit_array("range_hist_n", Hist_Mass.GetBinsContents());
+    Hist_Mass.Fill(range | hydra::make_axes(hydra::InvariantMass{0, 1}),
+                   range.GetEventWeightFunctor());
+    emit_array("range_hist_w", Hist_Mass.GetBinsContents());
+    // single elements and a copy of the first 512 into a Decays container
+    std::vector<double> el;
+    for (size_t i = 0; i < 4; ++i) {
+      auto e = range[i];
+      el.push_back(hydra::get<0>(e));
+      const hydra::Vector4R v[3] = {hydra::get<1>(e), hydra::get<2>(e), hydra::get<3>(e)};
+      for (int j = 0; j < 3; ++j)
+        for (int c = 0; c < 4; ++c) el.push_back(v[j].get(c));
+    }
+    emit_array("range_elements4", el);
+    hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> Kpi(0.493677, pions, 512);
+    hydra::EventWeights kw(512);
+    hydra::copy(range, Kpi, &kw);
+    std::vector<double> krows;
+    for (size_t i = 0; i < 512; ++i) {
+      auto row = Kpi[i];
+      const hydra::Vector4R v[3] = {std::get<0>(row), std::get<1>(row), std::get<2>(row)};
+      for (int j = 0; j < 3; ++j)
+        for (int c = 0; c < 4; ++c) krows.push_back(v[j].get(c));
+    }
+    emit_array("range_copy_rows", krows);
+    emit_array("range_copy_w", kw.to_host());
+    int err_unbounded = 0;
+    try {
+      auto inf = hydra::phase_space_range(hydra::Vector4R(0.493677, 0.0, 0.0, 0.0), pions, 321);
+      Hist_Mass.Fill(inf | hydra::make_axes(hydra::InvariantMass{0, 1}));
+    } catch (std::invalid_argument const&) {
+      err_unbounded = 1;
+    }
+    std::printf("\"range_err_unbounded\": %d,\n", err_unbounded);
   }
   std::printf("\"ok\": 1\n}\n");
   return 0;

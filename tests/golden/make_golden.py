@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-from oracle import ENGINES, Reference  # noqa: E402
+from oracle import ENGINES, MINSTD, Reference  # noqa: E402
 
 # evaluate outputs are (1 + nf, n): row 0 the weight, row 1 + j functor j
 
@@ -171,6 +171,35 @@ def main():
         out[f"uws_{tag}_kept"] = np.array([kept], dtype=np.uint64)
         out[f"uws_{tag}_x"] = xo
         out["uws_w"] = wg
+
+    # phase_space_range: minstd_rand words (a seed past 2^32 keeps its low
+    # 32 bits), and range elements read through the range's own iterator
+    for seed in (321, 2**32 + 321, 0):
+        for disc in (0, 2**40 + 12345):
+            out[f"rng_minstd_s{seed}_d{disc}"] = R.rng_words(MINSTD, seed, disc, 64)
+    RANGES = {
+        # K+ -> pi+ pi+ pi- at rest, seed 321 (examples/misc/range_semantics.inl:154-156)
+        "k3pi": ([0.493677, 0.0, 0.0, 0.0], [0.13957061] * 3, 321, 0),
+        # a moving B with the J/psi K pi masses, a seed past 2^32, an offset
+        "c1_moving": ([6.0, 0.3, -0.2, 0.5], DECAYS["c1_3body"][1], 2**40 + 987654321,
+                      2**33 + 5),
+        "n12": ([20.0, 0.0, 0.0, 0.0], DECAYS["n12"][1], 159753, 0),
+        "n2": ([3.2, 0.0, 0.4, -0.9], DECAYS["n2"][1], 77, 1000),
+    }
+    for key, (m4, ms, seed, first) in RANGES.items():
+        w, p = R.phase_space_range(m4, ms, seed, 512, first=first)
+        out[f"range_{key}_in"] = np.array(list(m4) + [float(seed), float(first)])
+        out[f"range_{key}_seed"] = np.array([seed, first], dtype=np.uint64)
+        out[f"range_{key}_w"] = w
+        out[f"range_{key}_p"] = p
+    # the range histogram of range_semantics.inl:158-167: m(pi pi) of the first
+    # two daughters in 10 bins over [2 m_pi, m_K - m_pi], 100000 elements,
+    # unweighted and weighted by the element's weight
+    mk, mpi = 0.493677, 0.13957061
+    for wtd in (0, 1):
+        c, _ = R.range_histogram3([mk, 0, 0, 0], [mpi] * 3, 321, 100000, 0, 1, 10, 2 * mpi,
+                                  mk - mpi, weighted=bool(wtd))
+        out[f"range_hist_k3pi_w{wtd}"] = c
 
     np.savez_compressed(os.path.join(HERE, "phsp_golden.npz"), **out)
     print("wrote", os.path.join(HERE, "phsp_golden.npz"), len(out), "arrays")

@@ -141,3 +141,26 @@ def unweight_sample(cols, w, max_pdf=-1.0, seed=capi.UNWEIGHT_SAMPLE_SEED, jump=
     return (int(nacc.item()), None if out is None else out.cpu().numpy(),
             None if idx is None else idx.cpu().numpy()[:n],
             None if w_out is None else w_out.cpu().numpy()[:n])
+
+
+def range_generate(dec, seed, n, first=0, ld=None, want_wt=True):
+    """hpsp_range_generate -> (weights (n,), SoA momentum columns (4N, ld))."""
+    N = dec.n
+    ld = ld or max(n, 1)
+    cols = torch.empty((4 * N, ld), dtype=torch.float64, device=DEV)
+    wt = torch.empty(max(n, 1), dtype=torch.float64, device=DEV) if want_wt else None
+    check(lib().hpsp_range_generate(ctypes.byref(dec), seed, first, n, ptr(cols), ld, ptr(wt),
+                                    stream_handle()), "hpsp_range_generate")
+    torch.cuda.synchronize()
+    return (None if wt is None else wt.cpu().numpy()[:n]), cols.cpu().numpy()
+
+
+def range_histogram(grid, lo, hi, dec, seed, axes, n, first=0, mode=capi.HIST_WEIGHT_ONE):
+    """hpsp_histogram_fill_range -> the nbins + 2 contents."""
+    h = capi.make_histogram(grid, lo, hi)
+    out = torch.empty(int(np.prod(grid)) + 2, dtype=torch.float64, device=DEV)
+    check(lib().hpsp_histogram_fill_range(ctypes.byref(h), ctypes.byref(dec), seed,
+                                          capi.functor_array(axes), first, n, mode, ptr(out),
+                                          stream_handle()), "hpsp_histogram_fill_range")
+    torch.cuda.synchronize()
+    return out.cpu().numpy()

@@ -172,3 +172,52 @@ def test_unweight_sample_validates_before_touching_the_device():
     assert call(ncols=0) == 1          # columns out with no columns
     assert call(wsb=wsb - 8) == capi.ERR_WORKSPACE
     assert call(ws=None) == capi.ERR_WORKSPACE
+
+
+@pytest.mark.parametrize("m4,m", [([0.493677, 0.0, 0.0, 0.0], [0.13957061] * 3),
+                                  ([6.0, 0.3, -0.2, 0.5], [3.0969, 0.493677, 0.13957]),
+                                  ([20.0, 0.0, 0.0, 0.0], [0.1 + 0.1 * i for i in range(12)])])
+def test_range_init_matches_oracle(oracle, m4, m):
+    d = capi.make_range(m4, m)
+    rc, ecm, mw, beta = oracle.range_params(m4, m)
+    assert rc == 0 and d.engine == capi.MINSTD_RAND and d.n == len(m)
+    assert bits(np.array([d.ecm, d.max_weight] + list(d.beta))).tolist() == \
+        bits(np.array([ecm, mw] + list(beta))).tolist()
+
+
+def test_range_validates_before_touching_the_device():
+    L = capi.lib()
+    out = ctypes.c_void_p(64)
+    d = capi.Decay()
+    m = (ctypes.c_double * 3)(0.2, 0.2, 0.2)
+    m4 = (ctypes.c_double * 4)(1.0, 0.0, 0.0, 0.0)
+    assert L.hpsp_range_init(ctypes.byref(d), m4, 3, m) == capi.OK
+    bad_m = (ctypes.c_double * 3)(0.2, 0.0, 0.2)
+    assert L.hpsp_range_init(ctypes.byref(d), m4, 3, bad_m) == capi.ERR_MASS
+    light = (ctypes.c_double * 4)(0.5, 0.0, 0.0, 0.0)
+    assert L.hpsp_range_init(ctypes.byref(d), light, 3, m) == capi.ERR_ENERGY
+    spacelike = (ctypes.c_double * 4)(1.0, 2.0, 0.0, 0.0)  # mass() = 0
+    assert L.hpsp_range_init(ctypes.byref(d), spacelike, 3, m) == capi.ERR_MASS
+    assert L.hpsp_range_init(ctypes.byref(d), m4, 1, m) == 1
+    assert L.hpsp_range_init(ctypes.byref(d), m4, 13, m) == 1
+    r = capi.make_range([1.0, 0, 0, 0], [0.2] * 3)
+    assert L.hpsp_range_generate(ctypes.byref(r), 1, 0, 0, None, 0, None, None) == capi.OK
+    assert L.hpsp_range_generate(ctypes.byref(r), 1, 0, 10, None, 10, None, None) == 1
+    assert L.hpsp_range_generate(ctypes.byref(r), 1, 0, 10, out, 5, None, None) == 1
+    # first + n + 3N past 2^64 - 1
+    assert L.hpsp_range_generate(ctypes.byref(r), 1, 2**64 - 15, 10, out, 10, None, None) == 1
+    # a Generate decay is not a range
+    g = capi.make_decay(1.0, [0.2] * 3)
+    assert L.hpsp_range_generate(ctypes.byref(g), 1, 0, 10, out, 10, None, None) == 1
+    h = capi.make_histogram([10], [0.0], [1.0])
+    ax = capi.functor_array([capi.make_functor(capi.F_MASS, (0, 1))])
+    assert L.hpsp_histogram_fill_range(h, ctypes.byref(g), 1, ax, 0, 10, 0, out, None) == 1
+    assert L.hpsp_histogram_fill_range(h, ctypes.byref(r), 1, ax, 0, 10,
+                                       capi.HIST_WEIGHT_COLUMN, out, None) == 1
+    assert L.hpsp_histogram_fill_range(h, ctypes.byref(r), 1, None, 0, 10, 0, out, None) == 1
+    assert L.hpsp_histogram_fill_range(h, ctypes.byref(r), 1, ax, 0, 10, 0, None, None) == 1
+    bad_ax = capi.functor_array([capi.make_functor(capi.F_MASS, (0, 3))])
+    assert L.hpsp_histogram_fill_range(h, ctypes.byref(r), 1, bad_ax, 0, 10, 0, out, None) == 1
+    kind6 = capi.functor_array([capi.make_functor(capi.F_ONE)])
+    kind6[0].kind = 6
+    assert L.hpsp_histogram_fill_range(h, ctypes.byref(r), 1, kind6, 0, 10, 0, out, None) == 1

@@ -116,3 +116,25 @@ def test_gpu_generate_unweight_average(golden, oracle):
     assert r["host_copy_same"] == 1
     w = np.array(r["gen_weights"])
     assert r["host_copy_weights3"] == [w[0], w[17], w[4095]]
+
+
+@pytest.mark.gpu
+def test_gpu_phase_space_range(golden):
+    """phase_space_range as examples/misc/range_semantics.inl uses it: the lazy
+    range histogrammed (counts exact, weighted sums to rounding), single
+    elements and a copy into a Decays container (weights bit for bit, momenta
+    within 1e-12 E), an unbounded range refused by the fill."""
+    r = run("gpu")
+    assert np.array_equal(np.array(r["range_hist_n"]), golden["range_hist_k3pi_w0"])
+    hw = golden["range_hist_k3pi_w1"]
+    got = np.array(r["range_hist_w"])
+    assert np.all(np.abs(got - hw) <= 1e-13 * np.maximum(hw, 1.0))
+    gw, gp = golden["range_k3pi_w"], golden["range_k3pi_p"]
+    el = np.array(r["range_elements4"]).reshape(4, 13)
+    assert np.array_equal(bits(el[:, 0]), bits(gw[:4]))
+    E = np.abs(gp[:4, :, 0:1])
+    assert np.max(np.abs(el[:, 1:].reshape(4, 3, 4) - gp[:4]) / E) <= 1e-12
+    rows = np.array(r["range_copy_rows"]).reshape(512, 3, 4)
+    assert np.max(np.abs(rows - gp) / np.abs(gp[..., 0:1])) <= 1e-12
+    assert np.array_equal(bits(r["range_copy_w"]), bits(gw))
+    assert r["range_err_unbounded"] == 1

@@ -466,3 +466,141 @@ def test_unweight_sample_golden(oracle, golden):
         k, out, _ = oracle.unweight_sample(x[None, :], w, mp, 0x8EC74D321E6B5A27, jump, order=1)
         assert k == int(golden[f"uws_{tag}_kept"][0])
         assert np.array_equal(bits(out[0]), bits(golden[f"uws_{tag}_x"]))
+
+
+# ---------------------------------------------------------------- phase_space_range
+from oracle import MINSTD  # noqa: E402
+
+RANGES = ["k3pi", "c1_moving", "n12", "n2"]
+
+
+def range_case(golden, key):
+    v = golden[f"range_{key}_in"]
+    seed, first = (int(x) for x in golden[f"range_{key}_seed"])
+    m = {"k3pi": [0.13957061] * 3, "c1_moving": DECAYS["c1_3body"][1], "n12": DECAYS["n12"][1],
+         "n2": DECAYS["n2"][1]}[key]
+    return list(v[:4]), m, seed, first
+
+
+def test_minstd_known_answer(oracle):
+    # minstd_rand's standard check value: the 10000th draw of an engine seeded
+    # with 1 is 399268537 (ISO C++ [rand.predef]); a=48271 with a dropped
+    # modulus step or the wrong multiplier fails it
+    w = oracle.rng_words(MINSTD, 1, 0, 10000)
+    assert int(w[-1]) == 399268537
+    # brute-force stepping of x -> 48271 x mod (2^31 - 1)
+    x = 321
+    for k in range(64):
+        x = x * 48271 % 2147483647
+        assert int(oracle.rng_words(MINSTD, 321, 0, 64)[k]) == x
+
+
+def test_minstd_seed_and_discard(oracle):
+    # seed(s) keeps s mod m (1 for 0); only the low 32 bits of a wider seed
+    assert np.array_equal(oracle.rng_words(MINSTD, 0, 0, 8), oracle.rng_words(MINSTD, 1, 0, 8))
+    assert np.array_equal(oracle.rng_words(MINSTD, 2147483647, 0, 8),
+                          oracle.rng_words(MINSTD, 1, 0, 8))
+    assert np.array_equal(oracle.rng_words(MINSTD, 2**32 + 5, 0, 8),
+                          oracle.rng_words(MINSTD, 5, 0, 8))
+    # discard(n) by modular exponentiation equals n draws
+    full = oracle.rng_words(MINSTD, 99, 0, 3000)
+    assert np.array_equal(oracle.rng_words(MINSTD, 99, 2500, 500), full[2500:])
+    # u = (x - 1) / (m - 1): 0 at x = 1, just below 1 at x = m - 1
+    assert oracle.uniform(MINSTD, 1) == 0.0
+    assert oracle.uniform(MINSTD, 2147483646) == 2147483645 / 2147483646
+
+
+@pytest.mark.parametrize("seed", [321, 2**32 + 321, 0])
+@pytest.mark.parametrize("disc", [0, 2**40 + 12345])
+def test_minstd_words_golden(oracle, golden, seed, disc):
+    assert np.array_equal(oracle.rng_words(MINSTD, seed, disc, 64),
+                          golden[f"rng_minstd_s{seed}_d{disc}"])
+
+
+@pytest.mark.parametrize("key", RANGES)
+def test_range_golden(oracle, golden, key):
+    m4, m, seed, first = range_case(golden, key)
+    w, cols = oracle.phase_space_range(m4, m, seed, 512, first=first)
+    assert np.array_equal(bits(w), bits(golden[f"range_{key}_w"]))
+    assert np.array_equal(bits(cols_to_rows(cols, len(m))), bits(golden[f"range_{key}_p"]))
+
+
+def test_range_equals_reference(oracle, reference):
+    rng = np.random.default_rng(4321)
+    for N in range(2, 13):
+        m = list(rng.uniform(0.1, 1.0, N))
+        M = sum(m) * rng.uniform(1.05, 3.0)
+        pv = rng.normal(0, 2.0, 3)
+        m4 = [math.sqrt(M * M + (pv ** 2).sum())] + list(pv)
+        seed = int(rng.integers(0, 2**64, dtype=np.uint64))
+        first = int(rng.integers(0, 2**45))
+        w, cols = oracle.phase_space_range(m4, m, seed, 300, first=first)
+        wr, pr = reference.phase_space_range(m4, m, seed, 300, first=first)
+        assert np.array_equal(bits(w), bits(wr)), N
+        assert np.array_equal(bits(cols_to_rows(cols, N)), bits(pr)), N
+
+
+def test_range_closed_forms(oracle):
+    # element k is the DecayMother event k of engine MINSTD with fTeCmTm from
+    # the mother's mass; at rest: energy sum M, momentum sum 0, daughter masses
+    M = 5.279
+    m = DECAYS["c1_3body"][1]
+    w, cols = oracle.phase_space_range([M, 0, 0, 0], m, 7, 400)
+    p = cols_to_rows(cols, 3)
+    assert np.max(np.abs(p[:, :, 0].sum(axis=1) - M)) < 1e-13 * M
+    assert np.max(np.abs(p[:, :, 1:].sum(axis=1))) < 1e-13 * M
+    mass = np.sqrt(p[:, :, 0] ** 2 - (p[:, :, 1:] ** 2).sum(axis=-1))
+    assert np.max(np.abs(mass - np.array(m))) < 1e-10
+    # the weights are the generation weights of those events
+    rc, ecm, mw = oracle.params(M, m)
+    pe, we = oracle.decay_events(M, m, 7, np.arange(400, dtype=np.uint64), engine=MINSTD)
+    assert np.array_equal(bits(we), bits(w))
+    # two-body: |p| = p*, weight 1 to rounding
+    w2, c2 = oracle.phase_space_range([3.0, 0, 0, 0], [1.0, 0.5], 11, 200)
+    p2 = cols_to_rows(c2, 2)
+    pstar = math.sqrt((9.0 - 1.5 ** 2) * (9.0 - 0.5 ** 2)) / 6.0
+    assert np.max(np.abs(np.sqrt((p2[:, 0, 1:] ** 2).sum(axis=-1)) - pstar)) < 1e-14 * 3
+    assert np.max(np.abs(w2 - 1.0)) < 1e-14
+    # a seed is taken modulo 2^32
+    wa, _ = oracle.phase_space_range([M, 0, 0, 0], m, 2**32 + 7, 50)
+    assert np.array_equal(bits(wa), bits(w[:50]))
+
+
+def test_range_mother_mass_and_boost(oracle):
+    # fTeCmTm comes from the mother's four-vector; the final boost carries the
+    # reference's (E, px, py) / E components
+    m4 = [6.0, 0.3, -0.2, 0.5]
+    M = math.sqrt(36.0 - 0.09 - 0.04 - 0.25)
+    m = DECAYS["c1_3body"][1]
+    rc, ecm, mw, beta = oracle.range_params(m4, m)
+    rc2, ecm2, mw2 = oracle.params(M, m)
+    assert (rc, rc2) == (0, 0) and ecm == ecm2 and mw == mw2
+    d3 = math.sqrt(0.09 + 0.04 + 0.25)
+    assert np.allclose(beta, np.array([6.0, 0.3, -0.2]) * (d3 / 6.0) / d3, rtol=1e-15)
+    assert oracle.range_params([0.3, 0, 0, 0], [0.2, 0.2], )[0] == -2
+    assert oracle.range_params([1.0, 0, 0, 0], [0.2, 0.0])[0] == -1
+
+
+def test_range_histogram_equals_reference(oracle, reference, golden):
+    # the range histogram of range_semantics.inl: the oracle's elements, the
+    # pair mass (functor 5) and the oracle's fill reproduce the reference's
+    # DenseHistogram over the lazy range bit for bit
+    mk, mpi = 0.493677, 0.13957061
+    w, cols = oracle.phase_space_range([mk, 0, 0, 0], [mpi] * 3, 321, 100000)
+    p = cols_to_rows(cols, 3)
+    x = np.array([oracle.functor(5, [0, 1], p[k]) for k in range(100000)])
+    for wtd in (0, 1):
+        got = oracle.histogram_fill([10], [2 * mpi], [mk - mpi], x[None, :],
+                                    w if wtd else None)
+        assert np.array_equal(bits(got), bits(golden[f"range_hist_k3pi_w{wtd}"]))
+    ref, _ = reference.range_histogram3([mk, 0, 0, 0], [mpi] * 3, 321, 100000, 0, 1, 10,
+                                        2 * mpi, mk - mpi)
+    assert np.array_equal(bits(ref), bits(golden["range_hist_k3pi_w0"]))
+
+
+def test_mass_functor_closed_form(oracle):
+    # (p_a + p_b).mass(): sqrt of a positive square, 0 otherwise
+    p = np.array([[5.0, 1.0, 2.0, 2.0], [3.0, -1.0, 0.0, 0.0], [1.0, 0.0, 0.0, 2.0]])
+    assert oracle.functor(5, [0, 1], p) == math.sqrt(64.0 - 0.0 - 4.0 - 4.0)
+    assert oracle.functor(5, [2, 2], p) == 0.0  # spacelike sum
+    assert oracle.functor(5, [0, 1], p) == math.sqrt(oracle.functor(3, [0, 1], p))
