@@ -29,6 +29,8 @@ ENGINES = {"squares3": 0, "squares4": 1, "philox": 2, "philox_long": 3,
 MINSTD_RAND = 6  # phase_space_range's engine (hpsp_range_* only)
 FP64, MIXED = 0, 1
 F_ONE, F_BW_SUM, F_MASS2_01, F_MASS2, F_COS_HELICITY, F_MASS = 0, 1, 2, 3, 4, 5
+F_DECAY_PLANE_ANGLE = 6
+MAX_PARTICLES = 16
 MAX_EVAL_FUNCTORS = 8
 
 
@@ -45,6 +47,11 @@ class Functor(ctypes.Structure):
                 ("m", ctypes.c_double * MAX_TERMS), ("g", ctypes.c_double * MAX_TERMS),
                 ("cr", ctypes.c_double * MAX_TERMS), ("ci", ctypes.c_double * MAX_TERMS),
                 ("a", ctypes.c_int32 * MAX_TERMS), ("b", ctypes.c_int32 * MAX_TERMS)]
+
+
+class Particles(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("cols", ctypes.c_void_p * MAX_PARTICLES),
+                ("ld", ctypes.c_uint64 * MAX_PARTICLES)]
 
 
 class Stats(ctypes.Structure):
@@ -145,6 +152,10 @@ _SIGS = {
     "hpsp_histogram_fill": (_I, [ctypes.POINTER(Histogram), _P, _U64, _U64, _P, _P, _P]),
     "hpsp_histogram_fill_events": (_I, [ctypes.POINTER(Histogram), ctypes.POINTER(Decay),
                                         ctypes.POINTER(Functor), _P, _U64, _U64, _I, _P, _P, _P]),
+    "hpsp_particles_evaluate": (_I, [ctypes.POINTER(Particles), ctypes.POINTER(Functor), _I,
+                                     _U64, _P, _U64, _P]),
+    "hpsp_histogram_fill_particles": (_I, [ctypes.POINTER(Histogram), ctypes.POINTER(Particles),
+                                           ctypes.POINTER(Functor), _U64, _P, _P, _P]),
     "hpsp_range_init": (_I, [ctypes.POINTER(Decay), _P, _I, _P]),
     "hpsp_range_generate": (_I, [ctypes.POINTER(Decay), _U64, _U64, _U64, _P, _U64, _P, _P]),
     "hpsp_histogram_fill_range": (_I, [ctypes.POINTER(Histogram), ctypes.POINTER(Decay), _U64,
@@ -226,11 +237,17 @@ def make_range(mother4, masses):
 def make_functor(kind=F_ONE, terms=()):
     """F_BW_SUM: terms = [(M, G, cr, ci, a, b), ...]; F_MASS2, F_MASS and
     F_COS_HELICITY: terms = (a, b) (for the helicity cosine, a is the particle
-    whose angle is measured and b its partner)."""
+    whose angle is measured and b its partner; (a, b, first, count) names the
+    parent's particles); F_DECAY_PLANE_ANGLE: terms = (d, e, h)."""
     f = Functor()
     f.kind = kind
     if kind in (F_MASS2, F_COS_HELICITY, F_MASS):
         f.a[0], f.b[0] = int(terms[0]), int(terms[1])
+        if kind == F_COS_HELICITY and len(terms) == 4:
+            f.a[1], f.b[1] = int(terms[2]), int(terms[3])
+        return f
+    if kind == F_DECAY_PLANE_ANGLE:
+        f.a[0], f.b[0], f.a[1] = int(terms[0]), int(terms[1]), int(terms[2])
         return f
     f.nterms = len(terms)
     for k, (M, G, cr, ci, a, b) in enumerate(terms):

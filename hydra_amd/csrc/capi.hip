@@ -301,6 +301,65 @@ int hpsp_histogram_fill_events(const hpsp_histogram* h, const hpsp_decay* d,
   return hip_status(launch_histogram_fill(d->n, a, as_stream(stream)));
 }
 
+// ---- stored events as particles ---------------------------------------------
+
+namespace {
+
+int make_particlesk(const hpsp_particles* p, uint64_t n, ParticlesK* k) {
+  if (!p || p->n < 1 || p->n > HPSP_MAX_PARTICLES) return HPSP_ERR_INVALID_ARGUMENT;
+  memset(k, 0, sizeof(*k));
+  k->n = p->n;
+  for (int i = 0; i < p->n; ++i) {
+    if (n > 0 && (!p->cols[i] || p->ld[i] < n)) return HPSP_ERR_INVALID_ARGUMENT;
+    k->cols[i] = p->cols[i];
+    k->ld[i] = p->ld[i];
+  }
+  return HPSP_OK;
+}
+
+}  // namespace
+
+int hpsp_particles_evaluate(const hpsp_particles* p, const hpsp_functor* fs, int nf, uint64_t n,
+                            double* out, uint64_t ld, void* stream) {
+  PartEvalArgs a;
+  memset(&a, 0, sizeof(a));
+  int rc = make_particlesk(p, n, &a.p);
+  if (rc) return rc;
+  if (nf < 1 || nf > HPSP_MAX_EVAL_FUNCTORS || !fs) return HPSP_ERR_INVALID_ARGUMENT;
+  for (int j = 0; j < nf; ++j) {
+    rc = make_functork(&fs[j], p->n, &a.f[j], 0, true);
+    if (rc) return rc;
+  }
+  if (n > 0 && (!out || ld < n)) return HPSP_ERR_INVALID_ARGUMENT;
+  a.nf = nf;
+  a.n = n;
+  a.out = out;
+  a.ld = ld;
+  return hip_status(launch_particles_evaluate(a, as_stream(stream)));
+}
+
+int hpsp_histogram_fill_particles(const hpsp_histogram* h, const hpsp_particles* p,
+                                  const hpsp_functor* axes, uint64_t n, const double* w,
+                                  double* contents, void* stream) {
+  HistArgs a;
+  memset(&a, 0, sizeof(a));
+  int rc = make_histk(h, &a.h);
+  if (rc) return rc;
+  rc = make_particlesk(p, n, &a.parts);
+  if (rc) return rc;
+  if (!contents || !axes) return HPSP_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < h->ndim; ++i) {
+    rc = make_functork(&axes[i], p->n, &a.axes[i], 0, true);
+    if (rc) return rc;
+  }
+  a.weight_mode = w ? HPSP_HIST_WEIGHT_COLUMN : HPSP_HIST_WEIGHT_ONE;
+  a.w = w;
+  a.n = n;
+  a.contents = contents;
+  a.lds_slots = hist_lds_slots();
+  return hip_status(launch_histogram_fill(-1, a, as_stream(stream)));
+}
+
 // ---- phase_space_range ----------------------------------------------------
 
 int hpsp_range_init(hpsp_decay* d, const double mother4[4], int n, const double* masses) {

@@ -16,7 +16,10 @@
 //   events   the events of a Decays container in SoA columns, each axis a
 //            functor of the event (the lazy `events | dalitz_calculator` of
 //            phsp_basic.inl:196-214, fused into the fill), weights from a
-//            column, the PhaseSpaceWeight of the event, or 1.
+//            column, the PhaseSpaceWeight of the event, or 1;
+//   particles  stored four-vectors given particle by particle (a melded
+//            decay chain, phsp_chain.inl:244), read only where a functor
+//            uses them; weights from a column or 1.
 #include <algorithm>
 #include <array>
 #include <utility>
@@ -61,6 +64,12 @@ __global__ __launch_bounds__(kHistBlock) void k_hist_fill(const HistArgs a) {
 #pragma unroll
       for (int d = 0; d < HPSP_HIST_MAX_DIM; ++d)
         x[d] = (d < a.h.ndim) ? a.x[(uint64_t)d * a.ldx + k] : 0.0;
+      w = a.w ? a.w[k] : 1.0;
+    } else if constexpr (N < 0) {
+      const MemAcc acc{a.parts, k};
+#pragma unroll
+      for (int d = 0; d < HPSP_HIST_MAX_DIM; ++d)
+        x[d] = (d < a.h.ndim) ? functor_eval_acc<double, true, true>(a.axes[d], acc) : 0.0;
       w = a.w ? a.w[k] : 1.0;
     } else {
       event_axes<N>(a, k, x, w);
@@ -116,7 +125,27 @@ int launch_histogram_fill(int N, const HistArgs& in, hipStream_t s) {
   const uint64_t want = (a.n + kHistBlock - 1) / kHistBlock;
   const uint64_t cap = a.use_lds && nslots > 2048 ? 256 : 1024;
   const unsigned grid = (unsigned)std::min<uint64_t>(want, cap);
-  kHist[N == 0 ? 0 : N - 1](a, grid, lds, s);
+  if (N < 0)
+    hist_one<-1>(a, grid, lds, s);
+  else
+    kHist[N == 0 ? 0 : N - 1](a, grid, lds, s);
+  return (int)hipGetLastError();
+}
+
+// events | functors over stored particles: one event per lane, the functor
+// columns written lane-contiguous.
+__global__ __launch_bounds__(kBlock) void k_particles_eval(const PartEvalArgs a) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < a.n; k += stride) {
+    const MemAcc acc{a.p, k};
+    for (int j = 0; j < a.nf; ++j)
+      a.out[(uint64_t)j * a.ld + k] = functor_eval_acc<double, true, true>(a.f[j], acc);
+  }
+}
+
+int launch_particles_evaluate(const PartEvalArgs& a, hipStream_t s) {
+  if (a.n == 0) return (int)hipSuccess;
+  hipLaunchKernelGGL(k_particles_eval, dim3(grid_for(a.n, 1u << 16)), dim3(kBlock), 0, s, a);
   return (int)hipGetLastError();
 }
 

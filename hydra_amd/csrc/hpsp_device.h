@@ -561,23 +561,69 @@ __device__ __forceinline__ Real dot4(const Real (&a)[4], const Real (&b)[4]) {
   return a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
 }
 
-template <typename Real, bool Exact = true, int P>
-__device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p)[P][4]) {
+// Four-vectors of one event in registers (the generation kernels).
+template <typename Real, int P>
+struct RegAcc {
+  const Real (&p)[P][4];
+  __device__ __forceinline__ void get(int i, Real (&v)[4]) const { pick4(p, i, v); }
+  // p[first] + ... + p[last - 1], left to right, through the unrolled array
+  __device__ __forceinline__ void sum(int first, int last, Real (&Pv)[4]) const {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) Pv[c] = 0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const Real s = Pv[c] + p[i][c];
+        Pv[c] = (i == first) ? p[i][c] : ((i > first && i < last) ? s : Pv[c]);
+      }
+    }
+  }
+};
+
+// Three-momentum products of Vector4R (Vector4R.inl:372-386, 426-454).
+template <typename Real>
+__device__ __forceinline__ Real dot3(const Real (&a)[4], const Real (&b)[4]) {
+#pragma clang fp contract(off)
+  Real t = a[1] * b[1];
+  t += a[2] * b[2];
+  t += a[3] * b[3];
+  return t;
+}
+
+// v / v.d3mag(): operator/ multiplies by the reciprocal (Vector4R.inl:76-86)
+template <typename Real>
+__device__ __forceinline__ void unit3(Real (&v)[4]) {
+#pragma clang fp contract(off)
+  const Real m2 = v[1] * v[1] + v[2] * v[2] + v[3] * v[3];
+  Real m;
+  if constexpr (sizeof(Real) == 4)
+    m = __builtin_sqrtf(m2);
+  else
+    m = sqrt_cr(m2);
+  const Real inv = (Real)1.0 / m;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] *= inv;
+}
+
+// Functor value of one event; acc.get(i, v) yields four-vector i and
+// acc.sum(first, last, P) the left-to-right sum of four-vectors [first, last).
+// The decay-plane angle (atan2 and two normalisations) is compiled only where
+// Plane is set, the kernels over stored particles: in the generation kernels
+// its registers would be reserved for every functor.
+template <typename Real, bool Exact, bool Plane = false, typename Acc>
+__device__ __forceinline__ double functor_eval_acc(const FunctorK& f, const Acc& acc) {
 #pragma clang fp contract(off)
   if (f.kind == HPSP_F_ONE) return 1.0;
-  if (f.kind == HPSP_F_MASS2_01) {
-    const Real s0 = p[0][0] + p[1][0], s1 = p[0][1] + p[1][1];
-    const Real s2 = p[0][2] + p[1][2], s3 = p[0][3] + p[1][3];
-    return (double)(s0 * s0 - s1 * s1 - s2 * s2 - s3 * s3);
-  }
-  if (f.kind == HPSP_F_MASS2 || f.kind == HPSP_F_MASS) {
+  if (f.kind == HPSP_F_MASS2_01 || f.kind == HPSP_F_MASS2 || f.kind == HPSP_F_MASS) {
     Real u[4], v[4], s[4];
-    pick4(p, f.a[0], u);
-    pick4(p, f.b[0], v);
+    acc.get(f.kind == HPSP_F_MASS2_01 ? 0 : f.a[0], u);
+    acc.get(f.kind == HPSP_F_MASS2_01 ? 1 : f.b[0], v);
 #pragma unroll
     for (int c = 0; c < 4; ++c) s[c] = u[c] + v[c];
     const Real m2 = dot4(s, s);
-    if (f.kind == HPSP_F_MASS2) return (double)m2;
+    if (f.kind != HPSP_F_MASS) return (double)m2;
     // Vector4R::mass: 0 unless the square is positive
     if constexpr (sizeof(Real) == 4)
       return m2 > 0.0f ? (double)__builtin_sqrtf(m2) : 0.0;
@@ -589,19 +635,11 @@ __device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p
   if (f.kind == HPSP_F_COS_HELICITY) {
     // P = p[first] + ... + p[first + count - 1] left to right (first, count
     // in a[1], b[1]); q = d + p[b0]; d = p[a0]
-    const int first = f.a[1], last = f.a[1] + f.b[1];
-    Real Pv[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < P; ++i) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const Real sum = Pv[c] + p[i][c];
-        Pv[c] = (i == first) ? p[i][c] : ((i > first && i < last) ? sum : Pv[c]);
-      }
-    }
+    Real Pv[4];
+    acc.sum(f.a[1], f.a[1] + f.b[1], Pv);
     Real d[4], e[4], q[4];
-    pick4(p, f.a[0], d);
-    pick4(p, f.b[0], e);
+    acc.get(f.a[0], d);
+    acc.get(f.b[0], e);
 #pragma unroll
     for (int c = 0; c < 4; ++c) q[c] = d[c] + e[c];
     const Real pd = dot4(Pv, d);
@@ -621,11 +659,41 @@ __device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p
         return num * rcp_fast(sqrt_fast(den));
     }
   }
+  if (Plane && f.kind == HPSP_F_DECAY_PLANE_ANGLE) {
+    // angle between the plane of (d, e) and the plane of h about K = d + e,
+    // in [0, 2 pi) (phsp_chain.inl:191-207): the components of d and h
+    // transverse to K's three-momentum, d' = K x d_perp, chi = atan2(y, x)
+    Real d[4], e[4], h[4], K[4];
+    acc.get(f.a[0], d);
+    acc.get(f.b[0], e);
+    acc.get(f.a[1], h);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) K[c] = d[c] + e[c];
+    const Real kk = dot3(K, K);
+    const Real cd = dot3(K, d) / kk;
+    const Real chh = dot3(K, h) / kk;
+    Real dp[4], hp[4], dq[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      dp[c] = d[c] - K[c] * cd;
+      hp[c] = h[c] - K[c] * chh;
+    }
+    dq[0] = 0;
+    dq[1] = K[2] * dp[3] - K[3] * dp[2];
+    dq[2] = K[3] * dp[1] - K[1] * dp[3];
+    dq[3] = K[1] * dp[2] - K[2] * dp[1];
+    unit3(dp);
+    unit3(dq);
+    const double x = (double)dot3(dp, hp), y = (double)dot3(dq, hp);
+    double chi = atan2(y, x);
+    if (chi < 0.0) chi += 2.0 * 3.141592653589793;
+    return chi;
+  }
   Real Ar = 0, Ai = 0;
   for (int k = 0; k < f.nterms; ++k) {
     Real u[4], w[4], v[4];
-    pick4(p, f.a[k], u);
-    pick4(p, f.b[k], w);
+    acc.get(f.a[k], u);
+    acc.get(f.b[k], w);
 #pragma unroll
     for (int c = 0; c < 4; ++c) v[c] = u[c] + w[c];
     const Real s = v[0] * v[0] - v[1] * v[1] - v[2] * v[2] - v[3] * v[3];
@@ -653,6 +721,11 @@ __device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p
     Ai += ti;
   }
   return (double)(Ar * Ar + Ai * Ai);
+}
+
+template <typename Real, bool Exact = true, int P>
+__device__ __forceinline__ double functor_eval(const FunctorK& f, const Real (&p)[P][4]) {
+  return functor_eval_acc<Real, Exact>(f, RegAcc<Real, P>{p});
 }
 
 // ---------------------------------------------------------------------------

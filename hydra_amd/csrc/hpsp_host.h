@@ -82,23 +82,38 @@ inline DecayK make_decayk(const hpsp_decay* d, uint64_t seed) {
 
 // nvec: four-vectors the functor sees; first_daughter: index of the first
 // decay product among them (0, or 1 when the mother comes first).
+// The decay-plane angle only where the kernels compile it (allow_plane: the
+// entry points over stored particles).
 inline int make_functork(const hpsp_functor* f, int nvec, FunctorK* out,
-                         int first_daughter = 0) {
+                         int first_daughter = 0, bool allow_plane = false) {
   memset(out, 0, sizeof(*out));
   if (!f) {
     out->kind = HPSP_F_ONE;
     return HPSP_OK;
   }
-  if (f->kind < HPSP_F_ONE || f->kind > HPSP_F_MASS) return HPSP_ERR_INVALID_ARGUMENT;
+  const int last_kind = allow_plane ? HPSP_F_DECAY_PLANE_ANGLE : HPSP_F_MASS;
+  if (f->kind < HPSP_F_ONE || f->kind > last_kind) return HPSP_ERR_INVALID_ARGUMENT;
   out->kind = f->kind;
-  if (f->kind == HPSP_F_MASS2 || f->kind == HPSP_F_COS_HELICITY || f->kind == HPSP_F_MASS) {
+  if (f->kind == HPSP_F_MASS2 || f->kind == HPSP_F_COS_HELICITY || f->kind == HPSP_F_MASS ||
+      f->kind == HPSP_F_DECAY_PLANE_ANGLE) {
     if (f->a[0] < 0 || f->a[0] >= nvec || f->b[0] < 0 || f->b[0] >= nvec)
       return HPSP_ERR_INVALID_ARGUMENT;
     out->a[0] = f->a[0];
     out->b[0] = f->b[0];
-    if (f->kind == HPSP_F_COS_HELICITY) {  // parent: the sum of the daughters
-      out->a[1] = first_daughter;
-      out->b[1] = nvec - first_daughter;
+    if (f->kind == HPSP_F_COS_HELICITY) {
+      if (f->b[1] > 0) {  // an explicit parent p[a1] .. p[a1 + b1 - 1]
+        if (f->a[1] < 0 || f->a[1] >= nvec || f->b[1] > nvec - f->a[1])
+          return HPSP_ERR_INVALID_ARGUMENT;
+        out->a[1] = f->a[1];
+        out->b[1] = f->b[1];
+      } else {  // the sum of the daughters
+        out->a[1] = first_daughter;
+        out->b[1] = nvec - first_daughter;
+      }
+    }
+    if (f->kind == HPSP_F_DECAY_PLANE_ANGLE) {
+      if (f->a[1] < 0 || f->a[1] >= nvec) return HPSP_ERR_INVALID_ARGUMENT;
+      out->a[1] = f->a[1];
     }
   }
   if (f->kind == HPSP_F_BW_SUM) {

@@ -46,6 +46,46 @@ struct EvalArgs {
   uint64_t ld;
 };
 
+// Four-vectors of stored events: particle i, component c, event k at
+// cols[i][c * ld[i] + k].
+struct ParticlesK {
+  const double* cols[HPSP_MAX_PARTICLES];
+  uint64_t ld[HPSP_MAX_PARTICLES];
+  int n;
+};
+
+// Reads the particles a functor asks for from memory (wave-uniform particle
+// index, lane-contiguous events): only the columns in use are loaded.
+struct MemAcc {
+  const ParticlesK& p;
+  uint64_t k;
+  __device__ __forceinline__ void get(int i, double (&v)[4]) const {
+    const double* c = p.cols[i] + k;
+    const uint64_t ld = p.ld[i];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = c[(uint64_t)q * ld];
+  }
+  __device__ __forceinline__ void sum(int first, int last, double (&Pv)[4]) const {
+#pragma clang fp contract(off)
+    get(first, Pv);
+    for (int i = first + 1; i < last; ++i) {
+      double v[4];
+      get(i, v);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Pv[q] = Pv[q] + v[q];
+    }
+  }
+};
+
+struct PartEvalArgs {
+  ParticlesK p;
+  FunctorK f[HPSP_MAX_EVAL_FUNCTORS];
+  int nf;
+  uint64_t n;
+  double* out;  // nf columns of ld doubles
+  uint64_t ld;
+};
+
 // Dense histogram geometry with the reference's derived quantities:
 // delta = upper - lower, gridf = (double)grid, row-major strides.
 struct HistK {
@@ -65,6 +105,7 @@ struct HistArgs {
   FunctorK axes[HPSP_HIST_MAX_DIM];
   const double* cols;       // events source: 4N momentum columns
   uint64_t ld;
+  ParticlesK parts;         // particles source
   int weight_mode;          // hpsp_hist_weight
   const double* w;
   uint64_t n;
@@ -132,7 +173,9 @@ int launch_generate_mixed(int N, int engine, const GenArgs& a, hipStream_t s);
 int launch_average_f64(int N, int engine, const AvgArgs& a, hipStream_t s);
 int launch_average_mixed(int N, int engine, const AvgArgs& a, hipStream_t s);
 int launch_evaluate_f64(int N, int engine, bool mothers, const EvalArgs& a, hipStream_t s);
-int launch_histogram_fill(int N, const HistArgs& a, hipStream_t s);  // N = 0: columns
+// N = 0: value columns; N >= 2: events of N daughters; N = -1: particles
+int launch_histogram_fill(int N, const HistArgs& a, hipStream_t s);
+int launch_particles_evaluate(const PartEvalArgs& a, hipStream_t s);
 void launch_zero(double* p, uint64_t n, hipStream_t s);
 int launch_range_generate(int N, const RangeArgs& a, hipStream_t s);
 int launch_range_histogram(int N, const RangeArgs& a, hipStream_t s);

@@ -14,6 +14,7 @@
 
 #include <array>
 #include <cstring>
+#include <stdexcept>
 #include <tuple>
 #include <utility>
 #include <vector>
@@ -121,6 +122,43 @@ class PhaseSpaceReweight {
   hpsp_functor f_;
 };
 
+// Decays::Meld(others...) (Decays.h:286-293): the particles of one container
+// followed by those of the others, row by row, as the decay chain of
+// examples/phase_space/phsp_chain.inl:244 joins B0 -> J/psi K pi with
+// J/psi -> mu+ mu-.  A view over the containers' device columns (they must
+// outlive it); rows are read as tuples of P four-vectors.
+template <size_t P>
+class MeldedDecays {
+  template <size_t... I>
+  static auto row_type(std::index_sequence<I...>) -> hydra::tuple<decltype((void)I, Vector4R())...>;
+
+ public:
+  using value_type = decltype(row_type(std::make_index_sequence<P>{}));
+  static constexpr size_t nparticles = P;
+
+  MeldedDecays(hpsp_particles const& p, size_t n) : p_(p), n_(n) {}
+  size_t size() const { return n_; }
+  const hpsp_particles& particles() const { return p_; }
+
+  value_type operator[](size_t k) const {
+    std::array<double, 4 * P> v;
+    for (size_t i = 0; i < P; ++i)
+      detail::hpsp_check(hpsp_memcpy2d_to_host(v.data() + 4 * i, sizeof(double), p_.cols[i] + k,
+                                               p_.ld[i] * sizeof(double), sizeof(double), 4),
+                         "Decays::Meld");
+    return make(v, std::make_index_sequence<P>{});
+  }
+
+ private:
+  template <size_t... I>
+  static value_type make(const std::array<double, 4 * P>& v, std::index_sequence<I...>) {
+    return value_type(Vector4R(v[4 * I], v[4 * I + 1], v[4 * I + 2], v[4 * I + 3])...);
+  }
+
+  hpsp_particles p_;
+  size_t n_;
+};
+
 template <typename Tuple, typename Backend>
 class Decays;
 
@@ -166,8 +204,9 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
                          "Decays::GetDaugtherRange");
       return Vector4R(v[0], v[1], v[2], v[3]);
     }
-    // device pointer of component c (0:E 1:px 2:py 3:pz)
+    // device pointer of component c (0:E 1:px 2:py 3:pz); components ld() apart
     const double* data(int c) const { return c_->column(i_, c); }
+    size_t ld() const { return c_->ld(); }
 
    private:
     const Decays* c_;
@@ -232,6 +271,30 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   ParticleRange GetDaugtherRange(placeholders::placeholder<I>) const {
     static_assert(I < N, "particle index out of range");
     return ParticleRange(this, I);
+  }
+
+  // ---- the particles of this container, and Meld (Decays.h:286-293)
+  void append_particles(hpsp_particles& p) const {
+    for (size_t i = 0; i < N; ++i) {
+      p.cols[p.n] = column(i, 0);
+      p.ld[p.n] = fCap;
+      ++p.n;
+    }
+  }
+  hpsp_particles particles() const {
+    hpsp_particles p{};
+    append_particles(p);
+    return p;
+  }
+  template <typename... Others>
+  MeldedDecays<(N + ... + Others::N)> Meld(Others const&... others) const {
+    static_assert((N + ... + Others::N) <= HPSP_MAX_PARTICLES, "too many particles to meld");
+    hpsp_particles p{};
+    append_particles(p);
+    (others.append_particles(p), ...);
+    if (((others.size() < fSize) || ...))
+      throw std::invalid_argument("[hydra::Decays::Meld]: a melded container is shorter");
+    return MeldedDecays<(N + ... + Others::N)>(p, fSize);
   }
 
   // ---- weights (Decays.h:235-250)

@@ -8,6 +8,8 @@
 // reference's does.  Data come from
 //   * value columns: a multivector of N columns (for example Evaluate's
 //     functor columns) with optional weights;
+//   * melded events: `events.Meld(others) | hydra::make_axes(f1, ..., fN)`,
+//     each axis read from the particles it uses, weights optional;
 //   * phase_space_range elements: `range | hydra::make_axes(f1, ..., fN)`,
 //     generated inside the fill, unweighted or weighted by
 //     range.GetEventWeightFunctor() (each element's own weight);
@@ -74,6 +76,28 @@ EventAxes<sizeof...(F), Decays<hydra::tuple<Particles...>, hydra::device::sys_t>
     const Decays<hydra::tuple<Particles...>, hydra::device::sys_t>& ev, AxesSpec<F...> const& s) {
   return EventAxes<sizeof...(F), Decays<hydra::tuple<Particles...>, hydra::device::sys_t>>(
       ev, s.axes);
+}
+
+// melded | make_axes(f...): axes of stored events given particle by particle
+// (Decays::Meld), read by DenseHistogram::Fill where the functors use them.
+template <size_t D>
+class ParticleAxes {
+ public:
+  ParticleAxes(hpsp_particles const& p, size_t n, std::array<hpsp_functor, D> const& axes)
+      : p_(p), n_(n), axes_(axes) {}
+  const hpsp_particles& particles() const { return p_; }
+  size_t size() const { return n_; }
+  const std::array<hpsp_functor, D>& axes() const { return axes_; }
+
+ private:
+  hpsp_particles p_;
+  size_t n_;
+  std::array<hpsp_functor, D> axes_;
+};
+
+template <size_t P, typename... F>
+ParticleAxes<sizeof...(F)> operator|(MeldedDecays<P> const& m, AxesSpec<F...> const& s) {
+  return ParticleAxes<sizeof...(F)>(m.particles(), m.size(), s.axes);
 }
 
 // range | make_axes(f...): the axes of phase_space_range elements, generated
@@ -155,6 +179,14 @@ class DenseHistogram<double, N, hydra::device::sys_t> {
     return fill_range(axes, HPSP_HIST_WEIGHT_GENERATION);
   }
 
+  // Fills over melded events (phsp_chain.inl:244-281), weights optional
+  DenseHistogram& Fill(ParticleAxes<N> const& axes) { return fill_particles(axes, nullptr); }
+  DenseHistogram& Fill(ParticleAxes<N> const& axes, EventWeights const& weights) {
+    if (weights.size() < axes.size())
+      throw std::invalid_argument("[hydra::DenseHistogram::Fill]: fewer weights than events");
+    return fill_particles(axes, weights.data());
+  }
+
   size_t GetNBins() const { return (size_t)nbins_; }
   std::array<size_t, N> GetGrid() const { return grid_; }
   double GetBinContent(size_t bin) const {
@@ -221,6 +253,14 @@ class DenseHistogram<double, N, hydra::device::sys_t> {
         dw ? *dw : detail::make_decay<ND>(ev.GetMotherMass(), masses, "DenseHistogram");
     detail::hpsp_check(hpsp_histogram_fill_events(&h_, &d, ax.axes().data(), ev.data(), ev.ld(),
                                                   ev.size(), mode, w, contents_.data(), nullptr),
+                       "DenseHistogram::Fill");
+    detail::hpsp_sync("DenseHistogram::Fill");
+    return *this;
+  }
+
+  DenseHistogram& fill_particles(ParticleAxes<N> const& ax, const double* w) {
+    detail::hpsp_check(hpsp_histogram_fill_particles(&h_, &ax.particles(), ax.axes().data(),
+                                                     ax.size(), w, contents_.data(), nullptr),
                        "DenseHistogram::Fill");
     detail::hpsp_sync("DenseHistogram::Fill");
     return *this;

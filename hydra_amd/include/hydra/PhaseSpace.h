@@ -16,6 +16,7 @@
 #include <utility>
 
 #include <hydra/Decays.h>
+#include <hydra/EventFunctors.h>
 #include <hydra/PhaseSpaceFunctors.h>
 #include <hydra/PhaseSpaceRange.h>
 #include <hydra/Types.h>
@@ -176,6 +177,29 @@ class PhaseSpace {
     detail::hpsp_sync("PhaseSpace::Generate");
     daughters.set_weights_cache_valid(same && mothers.size() > 0);
     return make_range(daughters.begin(), daughters.begin() + (std::ptrdiff_t)mothers.size());
+  }
+
+  // Generate(mothers, daughters) with the mothers one particle of another
+  // container (Generate(ThreeBodyDecay.GetDaugtherRange(_0), TwoBodyDecay),
+  // phsp_chain.inl:234-238): its columns are read in place.
+  template <typename MotherRange, typename... P,
+            typename = decltype(std::declval<MotherRange const&>().data(0))>
+  Range<typename Decays<hydra::tuple<P...>, hydra::device::sys_t>::iterator> Generate(
+      MotherRange const& mothers, Decays<hydra::tuple<P...>, hydra::device::sys_t>& daughters) {
+    static_assert(sizeof...(P) == N, "container arity differs from PhaseSpace<N>");
+    const size_t n = mothers.size();
+    if (daughters.size() < n) daughters.resize(n);
+    const hpsp_decay d = descriptor(Vector4R(fMotherMass, 0, 0, 0));
+    const bool same = same_decay(daughters.GetMotherMass(), daughters.GetMasses()) &&
+                      daughters.GetMaxWeight() == fMaxWeight;
+    detail::hpsp_check(
+        hpsp_generate_mothers(&d, fSeed, 0, n, mothers.data(0), mothers.ld(), HPSP_FP64,
+                              daughters.data(), daughters.ld(), 0, nullptr,
+                              same ? daughters.weights_cache_for_write() : nullptr, nullptr),
+        "PhaseSpace::Generate");
+    detail::hpsp_sync("PhaseSpace::Generate");
+    daughters.set_weights_cache_valid(same && n > 0);
+    return make_range(daughters.begin(), daughters.begin() + (std::ptrdiff_t)n);
   }
 
   // ---- Evaluate (PhaseSpace.inl:289-348) --------------------------------

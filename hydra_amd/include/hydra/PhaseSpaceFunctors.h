@@ -9,7 +9,15 @@
 //   InvariantMass{a, b}      f = (p[a] + p[b]).mass() (Vector4R::mass)
 //   CosHelicity{d, e}        helicity cosine of p[d] in the pair (d, e) within
 //                            the sum of the daughters: the cos(theta_K*) of
-//                            examples/phase_space/phsp_evaluating_functor.inl:160-176
+//                            examples/phase_space/phsp_evaluating_functor.inl:160-176;
+//                            CosHelicity{d, e, first, count}: within
+//                            p[first] + ... + p[first + count - 1] (a sub-decay
+//                            of melded events)
+//   DecayPlaneAngle{d, e, h} angle in [0, 2 pi) between the plane of (d, e) and
+//                            the plane of h about d + e, the delta phi of
+//                            examples/phase_space/phsp_chain.inl:191-207; over
+//                            stored events only (events | make_functors,
+//                            melded events, their histograms)
 //   BreitWignerSum           f = |sum_k c_k / (M_k^2 - s_k - i M_k G_k)|^2,
 //                            s_k = (p[a_k] + p[b_k])^2
 // Indices count the four-vectors the functor sees: the daughters, or for the
@@ -42,6 +50,14 @@ struct InvariantMass {
 struct CosHelicity {
   int d;
   int e;
+  int first = 0;
+  int count = 0;  // 0: all the daughters
+};
+
+struct DecayPlaneAngle {
+  int d;
+  int e;
+  int h;
 };
 
 // One relativistic Breit-Wigner term c / (M^2 - s - i M G) of the pair (a, b).
@@ -69,6 +85,7 @@ struct is_phase_space_functor
                                        std::is_same<F, InvariantMass2>::value ||
                                        std::is_same<F, InvariantMass>::value ||
                                        std::is_same<F, CosHelicity>::value ||
+                                       std::is_same<F, DecayPlaneAngle>::value ||
                                        std::is_same<F, BreitWignerSum>::value> {};
 
 namespace detail {
@@ -100,6 +117,17 @@ inline hpsp_functor to_hpsp(const CosHelicity& h) {
   f.kind = HPSP_F_COS_HELICITY;
   f.a[0] = h.d;
   f.b[0] = h.e;
+  f.a[1] = h.first;
+  f.b[1] = h.count;
+  return f;
+}
+
+inline hpsp_functor to_hpsp(const DecayPlaneAngle& g) {
+  hpsp_functor f{};
+  f.kind = HPSP_F_DECAY_PLANE_ANGLE;
+  f.a[0] = g.d;
+  f.b[0] = g.e;
+  f.a[1] = g.h;
   return f;
 }
 

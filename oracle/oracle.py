@@ -46,13 +46,14 @@ FPAR = 32  # stride of one functor's parameters in the evaluate calls
 
 def functor_params(fid, terms=None):
     """Pack functor parameters.  fid 1: terms = [(M, G, cr, ci, a, b), ...];
-    fid 3 and 5: terms = (a, b); fid 4: terms = (d, e, first, count)."""
+    fid 3 and 5: terms = (a, b); fid 4: terms = (d, e, first, count);
+    fid 6: terms = (d, e, h)."""
     if fid == 1:
         flat = [float(len(terms))]
         for t in terms:
             flat.extend(float(x) for x in t)
         return np.array(flat, dtype=np.float64)
-    if fid in (3, 4, 5):
+    if fid in (3, 4, 5, 6):
         return np.array([float(x) for x in terms], dtype=np.float64)
     return np.zeros(1)
 
@@ -352,6 +353,7 @@ class Reference:
         L.hr_histogram_fill.argtypes = [_I, _P, _P, _P, _P, _U64, _P, _P, _P]
         L.hr_unweight_sample.argtypes = [_P, _U64, _D, _D, _D, _U64, _U64, _P, _P, _P]
         L.hr_phase_space_range.argtypes = [_I, _P, _P, _U64, _U64, _U64, _P, _P]
+        L.hr_chain.argtypes = [_U64, _I, _P, _P, _P, _P, _P]
         L.hr_range_histogram3.argtypes = [_P, _P, _U64, _U64, _I, _I, _I, _U64, _D, _D, _P, _P]
 
     def set_threads(self, n):
@@ -369,6 +371,18 @@ class Reference:
         if rc:
             raise ValueError("bad N")
         return w, p
+
+    def chain(self, n, functor=(0, None)):
+        """The decay chain of phsp_chain.inl: (melded rows (n, 5, 4), functor
+        values (n,), three-body weights (n,), seconds of the two Generates)."""
+        fid, terms = functor
+        p = np.zeros((n, 5, 4))
+        f = np.zeros(n)
+        w = np.zeros(n)
+        sec = _D()
+        self.L.hr_chain(n, fid, _ptr(functor_params(fid, terms)), _ptr(p), _ptr(f), _ptr(w),
+                        ctypes.byref(sec))
+        return p, f, w, sec.value
 
     def range_histogram3(self, mother4, masses, seed, n, a, b, bins, lo, hi, weighted=False):
         """DenseHistogram<double,1>.Fill(phase_space_range(...) | (p_a+p_b).mass()

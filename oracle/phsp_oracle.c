@@ -560,6 +560,14 @@ static inline double dot4(const double* a, const double* b) {
   return a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
 }
 
+/* Vector4R::dot: the three-momentum product (Vector4R.inl:441-454) */
+static inline double dot3(const double* a, const double* b) {
+  double t = a[1] * b[1];
+  t += a[2] * b[2];
+  t += a[3] * b[3];
+  return t;
+}
+
 double po_functor(int fid, const double* fpar, int N, const double* p) {
   (void)N;
   if (fid == 0) return 1.0;
@@ -573,6 +581,36 @@ double po_functor(int fid, const double* fpar, int N, const double* p) {
     double s[4];
     for (int c = 0; c < 4; ++c) s[c] = p[4 * a + c] + p[4 * b + c];
     return mass2(s);
+  }
+  if (fid == 6) {
+    /* angle between the decay planes, phsp_chain.inl:191-207 with
+     * Kstar = d + e and the three-momentum products of Vector4R::dot,
+     * cross, d3mag and operator/ (Vector4R.inl:76-86, 372-386, 426-454)   */
+    int d = (int)fpar[0], e = (int)fpar[1], h = (int)fpar[2];
+    const double* D = p + 4 * d;
+    const double* H = p + 4 * h;
+    double K[4], dp[4], hp[4], dq[4];
+    for (int c = 0; c < 4; ++c) K[c] = D[c] + p[4 * e + c];
+    double kk = dot3(K, K);
+    double cd = dot3(K, D) / kk;
+    double ch = dot3(K, H) / kk;
+    for (int c = 0; c < 4; ++c) {
+      dp[c] = D[c] - K[c] * cd;
+      hp[c] = H[c] - K[c] * ch;
+    }
+    dq[0] = 0.0;
+    dq[1] = K[2] * dp[3] - K[3] * dp[2];
+    dq[2] = K[3] * dp[1] - K[1] * dp[3];
+    dq[3] = K[1] * dp[2] - K[2] * dp[1];
+    double inv = 1.0 / sqrt(dp[1] * dp[1] + dp[2] * dp[2] + dp[3] * dp[3]);
+    for (int c = 0; c < 4; ++c) dp[c] *= inv;
+    inv = 1.0 / sqrt(dq[1] * dq[1] + dq[2] * dq[2] + dq[3] * dq[3]);
+    for (int c = 0; c < 4; ++c) dq[c] *= inv;
+    double x = dot3(dp, hp);
+    double y = dot3(dq, hp);
+    double chi = atan2(y, x);
+    if (chi < 0.0) chi += 2.0 * PO_PI;
+    return chi;
   }
   if (fid == 5) {
     /* (p_a + p_b).mass(): Vector4R::mass (Vector4R.inl:196-208) */

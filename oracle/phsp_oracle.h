@@ -23,6 +23,8 @@
  *                     P = p_first + ... + p_(first+count-1),
  *                     par = {d, e, first, count}
  *               id 5: f = (p_a + p_b).mass(), par = {a, b}
+ *               id 6: angle between the planes (d, e) and (d + e, h),
+ *                     par = {d, e, h}
  *   - evaluate outputs: column 0 the weight, column 1+j functor j (ld = n);
  *     functor j's parameters at fpar + PO_FPAR*j
  */

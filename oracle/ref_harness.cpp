@@ -107,6 +107,8 @@ inline void assign_tail(Tuple const& t, hydra::Vector4R* p, std::index_sequence<
 //         P = p_first + ... + p_(first+count-1), written as
 //         phsp_evaluating_functor.inl:160-176 writes it  par = {d, e, first, count}
 //   id 5: f = (p_a + p_b).mass()                        par = {a, b}
+//   id 6: angle between the planes of (d, e) and of h about d + e,
+//         written as phsp_chain.inl:191-207 writes it   par = {d, e, h}
 struct Functor {
   int id;
   int K;
@@ -127,6 +129,18 @@ struct Functor {
     if (id == 5) {
       hydra::Vector4R s = p[a[0]] + p[b[0]];
       return s.mass();
+    }
+    if (id == 6) {
+      hydra::Vector4R d = p[a[0]], h = p[a[1]];
+      hydra::Vector4R K = d + p[b[0]];
+      hydra::Vector4R d_perp = d - (K.dot(d) / K.dot(K)) * K;
+      hydra::Vector4R h_perp = h - (K.dot(h) / K.dot(K)) * K;
+      hydra::Vector4R d_prime = K.cross(d_perp);
+      d_perp = d_perp / d_perp.d3mag();
+      d_prime = d_prime / d_prime.d3mag();
+      double chi = atan2(d_prime.dot(h_perp), d_perp.dot(h_perp));
+      if (chi < 0.0) chi += 2.0 * PI;
+      return chi;
     }
     if (id == 4) {
       hydra::Vector4R P = p[a[1]];
@@ -165,6 +179,11 @@ Functor make_functor(int id, const double* par) {
   if (id == 3 || id == 5) {
     f.a[0] = (int)par[0];
     f.b[0] = (int)par[1];
+  }
+  if (id == 6) {
+    f.a[0] = (int)par[0];
+    f.b[0] = (int)par[1];
+    f.a[1] = (int)par[2];
   }
   if (id == 4) {
     f.a[0] = (int)par[0];
@@ -686,6 +705,48 @@ int hr_unweight_sample(const double* x, uint64_t n, double mean, double sigma, d
   auto r = hydra::unweight(hydra::device::sys, data, gauss, max_pdf, (size_t)seed, (size_t)jump);
   *n_kept = (uint64_t)r.size();
   for (uint64_t i = 0; i < n; ++i) x_out[i] = data[i];
+  return 0;
+}
+
+// The decay chain of examples/phase_space/phsp_chain.inl:216-244:
+// PhaseSpace<3>(B0 -> J/psi K pi) generates n events of a B0 at rest,
+// PhaseSpace<2>(J/psi -> mu+ mu-) decays each event's J/psi
+// (Generate(ThreeBodyDecay.GetDaugtherRange(_0), TwoBodyDecay)), both with
+// their default seeds, and ThreeBodyDecay.Meld(TwoBodyDecay) joins them.
+// out_p receives the melded rows (n x 5 x 4: J/psi, K, pi, mu+, mu-), out_f
+// functor fid of each melded row (n values; par as make_functor), out_w the
+// three-body weights (ThreeBodyDecay | GetEventWeightFunctor()).
+int hr_chain(uint64_t n, int fid, const double* fpar, double* out_p, double* out_f,
+             double* out_w, double* seconds) {
+  const double B0_mass = 5.27955, Jpsi_mass = 3.0969, K_mass = 0.493677,
+               pi_mass = 0.13957061, mu_mass = 0.1056583745;
+  hydra::Vector4R B0(B0_mass, 0.0, 0.0, 0.0);
+  hydra::PhaseSpace<3> three(B0_mass, {Jpsi_mass, K_mass, pi_mass});
+  hydra::PhaseSpace<2> two(Jpsi_mass, {mu_mass, mu_mass});
+  hydra::Decays<hydra::tuple<P0, P1, P2>, hydra::device::sys_t> tb(
+      B0_mass, {Jpsi_mass, K_mass, pi_mass}, n);
+  hydra::Decays<hydra::tuple<P3, P4>, hydra::device::sys_t> db(Jpsi_mass, {mu_mass, mu_mass},
+                                                               n);
+  double t0 = now_s();
+  three.Generate(B0, tb);
+  two.Generate(tb.GetDaugtherRange(hydra::placeholders::_0), db);
+  if (seconds) *seconds = now_s() - t0;
+  auto chain = tb.Meld(db);
+  Functor f = make_functor(fid, fpar);
+  auto it = chain.begin();
+  for (uint64_t k = 0; k < n; ++k, ++it) {
+    auto row = *it;
+    hydra::Vector4R v[5] = {hydra::Vector4R(P0(hydra::get<0>(row)).Value()),
+                            hydra::Vector4R(P1(hydra::get<1>(row)).Value()),
+                            hydra::Vector4R(P2(hydra::get<2>(row)).Value()),
+                            hydra::Vector4R(P3(hydra::get<3>(row)).Value()),
+                            hydra::Vector4R(P4(hydra::get<4>(row)).Value())};
+    for (int i = 0; i < 5; ++i)
+      for (int c = 0; c < 4; ++c) out_p[(k * 5 + i) * 4 + c] = v[i].get(c);
+    out_f[k] = f.eval(v);
+  }
+  auto w = tb | tb.GetEventWeightFunctor();
+  for (uint64_t k = 0; k < n; ++k) out_w[k] = w[k];
   return 0;
 }
 

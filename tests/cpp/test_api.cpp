@@ -29,6 +29,13 @@ declarg(Weight, double)
 declarg(M13Sq, double)
 declarg(M23Sq, double)
 declarg(CosTheta, double)
+// the decay chain of examples/phase_space/phsp_chain.inl
+declarg(Jpsi, hydra::Vector4R)
+declarg(Kaon, hydra::Vector4R)
+declarg(Pion, hydra::Vector4R)
+declarg(MuonP, hydra::Vector4R)
+declarg(MuonM, hydra::Vector4R)
+declarg(DeltaPhi, double)
 
 using namespace hydra::arguments;
 
@@ -283,6 +290,59 @@ int main(int argc, char** argv) {
       err_unbounded = 1;
     }
     std::printf("\"range_err_unbounded\": %d,\n", err_unbounded);
+
+    // the decay chain of examples/phase_space/phsp_chain.inl:216-262
+    {
+      const double B0_mass = 5.27955, Jpsi_mass = 3.0969, K_mass = 0.493677,
+                   pi_mass = 0.13957061, mup_mass = 0.1056583745, mum_mass = 0.1056583745;
+      hydra::Vector4R B0(B0_mass, 0.0, 0.0, 0.0);
+      hydra::PhaseSpace<3> ThreeBodyPHSP(B0_mass, {Jpsi_mass, K_mass, pi_mass});
+      hydra::PhaseSpace<2> TwoBodyPHSP(Jpsi_mass, {mup_mass, mum_mass});
+      const size_t nentries = 512;
+      auto ThreeBodyDecay = hydra::Decays<hydra::tuple<Jpsi, Kaon, Pion>, hydra::device::sys_t>(
+          B0_mass, {Jpsi_mass, K_mass, pi_mass}, nentries);
+      auto TwoBodyDecay = hydra::Decays<hydra::tuple<MuonP, MuonM>, hydra::device::sys_t>(
+          Jpsi_mass, {mup_mass, mup_mass}, nentries);
+      ThreeBodyPHSP.Generate(B0, ThreeBodyDecay);
+      TwoBodyPHSP.Generate(ThreeBodyDecay.GetDaugtherRange(hydra::placeholders::_0), TwoBodyDecay);
+      auto melded = ThreeBodyDecay.Meld(TwoBodyDecay);
+      auto Variables = hydra::make_functors(
+          hydra::InvariantMass2{0, 2}, hydra::InvariantMass2{1, 2}, hydra::CosHelicity{1, 2, 0, 3},
+          hydra::DecayPlaneAngle{1, 2, 3});
+      auto chain = melded | Variables;
+      auto weights = ThreeBodyDecay | ThreeBodyDecay.GetEventWeightFunctor();
+      hydra::multivector<hydra::tuple<M13Sq, M23Sq, CosTheta, DeltaPhi>, hydra::device::sys_t>
+          variables(chain.size());
+      hydra::copy(chain, variables);
+      std::vector<double> vars;
+      for (size_t i = 0; i < nentries; ++i) {
+        auto row = variables[i];
+        vars.push_back(double(std::get<0>(row)));
+        vars.push_back(double(std::get<1>(row)));
+        vars.push_back(double(std::get<2>(row)));
+        vars.push_back(double(std::get<3>(row)));
+      }
+      emit_array("chain_vars", vars);
+      auto c7 = chain[7];
+      emit_array("chain_row7", {std::get<0>(c7), std::get<1>(c7), std::get<2>(c7), std::get<3>(c7)});
+      std::vector<double> mrow;
+      for (size_t i : {size_t(0), size_t(300)}) {
+        auto m = melded[i];
+        const hydra::Vector4R v[5] = {std::get<0>(m), std::get<1>(m), std::get<2>(m),
+                                      std::get<3>(m), std::get<4>(m)};
+        for (int j = 0; j < 5; ++j)
+          for (int c = 0; c < 4; ++c) mrow.push_back(v[j].get(c));
+      }
+      emit_array("chain_melded_0_300", mrow);
+      emit_array("chain_weights", weights.to_host());
+      // histograms of the angles, weighted (the example's TH1D fills)
+      hydra::DenseHistogram<double, 2, hydra::device::sys_t> Angles({20, 16}, {-1.0, 0.0},
+                                                                   {1.0, 6.283185307179586});
+      Angles.Fill(melded | hydra::make_axes(hydra::CosHelicity{1, 2, 0, 3},
+                                            hydra::DecayPlaneAngle{1, 2, 3}),
+                  weights);
+      emit_array("chain_hist_w", Angles.GetBinsContents());
+    }
   }
   std::printf("\"ok\": 1\n}\n");
   return 0;

@@ -201,6 +201,23 @@ def main():
                                   mk - mpi, weighted=bool(wtd))
         out[f"range_hist_k3pi_w{wtd}"] = c
 
+    # the decay chain of examples/phase_space/phsp_chain.inl:216-244: melded
+    # rows (J/psi, K, pi, mu+, mu-), the example's four variables (M^2(J/psi
+    # pi), M^2(K pi), cos theta_K* within the B0, the decay-plane angle), the
+    # three-body weights, and the default seeds of both PhaseSpace objects
+    CHAIN_VARS = [(3, (0, 2)), (3, (1, 2)), (4, (1, 2, 0, 3)), (6, (1, 2, 3))]
+    vals = []
+    for fv in CHAIN_VARS:
+        p, f, w, _ = R.chain(512, fv)
+        vals.append(f)
+    out["chain_p"] = p
+    out["chain_vars"] = np.array(vals)
+    out["chain_w"] = w
+    seeds = []
+    for M, m in ((5.27955, [3.0969, 0.493677, 0.13957061]), (3.0969, [0.1056583745] * 2)):
+        seeds.append(R.params(M, m)[3])
+    out["chain_seeds"] = np.array(seeds, dtype=np.uint64)
+
     np.savez_compressed(os.path.join(HERE, "phsp_golden.npz"), **out)
     print("wrote", os.path.join(HERE, "phsp_golden.npz"), len(out), "arrays")
 

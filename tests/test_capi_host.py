@@ -221,3 +221,51 @@ def test_range_validates_before_touching_the_device():
     kind6 = capi.functor_array([capi.make_functor(capi.F_ONE)])
     kind6[0].kind = 6
     assert L.hpsp_histogram_fill_range(h, ctypes.byref(r), 1, kind6, 0, 10, 0, out, None) == 1
+
+
+def test_particles_validate_before_touching_the_device():
+    L = capi.lib()
+    out = ctypes.c_void_p(64)
+    p = capi.Particles()
+    p.n = 3
+    for i in range(3):
+        p.cols[i] = 64 + 8 * i
+        p.ld[i] = 100
+    f = capi.functor_array([capi.make_functor(capi.F_DECAY_PLANE_ANGLE, (0, 1, 2))])
+    assert L.hpsp_particles_evaluate(ctypes.byref(p), f, 1, 0, None, 0, None) == capi.OK
+    assert L.hpsp_particles_evaluate(ctypes.byref(p), f, 1, 10, None, 10, None) == 1  # no out
+    assert L.hpsp_particles_evaluate(ctypes.byref(p), f, 1, 10, out, 5, None) == 1    # ld < n
+    assert L.hpsp_particles_evaluate(ctypes.byref(p), f, 1, 200, out, 200, None) == 1  # ld[i] < n
+    assert L.hpsp_particles_evaluate(ctypes.byref(p), f, 0, 10, out, 10, None) == 1
+    assert L.hpsp_particles_evaluate(ctypes.byref(p), f, 9, 10, out, 10, None) == 1
+    bad = capi.functor_array([capi.make_functor(capi.F_DECAY_PLANE_ANGLE, (0, 1, 3))])
+    assert L.hpsp_particles_evaluate(ctypes.byref(p), bad, 1, 10, out, 10, None) == 1
+    cos_bad = capi.functor_array([capi.make_functor(capi.F_COS_HELICITY, (0, 1, 1, 3))])
+    assert L.hpsp_particles_evaluate(ctypes.byref(p), cos_bad, 1, 10, out, 10, None) == 1
+    p0 = capi.Particles()
+    assert L.hpsp_particles_evaluate(ctypes.byref(p0), f, 1, 10, out, 10, None) == 1
+    p.n = 17
+    assert L.hpsp_particles_evaluate(ctypes.byref(p), f, 1, 10, out, 10, None) == 1
+    p.n = 3
+    p.cols[1] = None
+    assert L.hpsp_particles_evaluate(ctypes.byref(p), f, 1, 10, out, 10, None) == 1
+    p.cols[1] = 72
+    h = capi.make_histogram([10], [0.0], [1.0])
+    assert L.hpsp_histogram_fill_particles(h, ctypes.byref(p), bad, 10, None, out, None) == 1
+    assert L.hpsp_histogram_fill_particles(h, ctypes.byref(p), f, 10, None, None, None) == 1
+    assert L.hpsp_histogram_fill_particles(h, ctypes.byref(p0), f, 10, None, out, None) == 1
+
+
+def test_plane_angle_only_over_stored_particles():
+    """The generation, Evaluate and event-column entry points do not compile
+    the decay-plane angle and refuse it."""
+    L = capi.lib()
+    out = ctypes.c_void_p(64)
+    d = capi.make_decay(5.279, [3.0969, 0.493677, 0.13957])
+    f = capi.functor_array([capi.make_functor(capi.F_DECAY_PLANE_ANGLE, (0, 1, 2))])
+    assert L.hpsp_evaluate(ctypes.byref(d), f, 1, 1, 0, 10, out, 10, None) == 1
+    h = capi.make_histogram([10], [0.0], [7.0])
+    assert L.hpsp_histogram_fill_events(h, ctypes.byref(d), f, out, 10, 10, 0, None, out,
+                                        None) == 1
+    r = capi.make_range([5.279, 0, 0, 0], [3.0969, 0.493677, 0.13957])
+    assert L.hpsp_histogram_fill_range(h, ctypes.byref(r), 1, f, 0, 10, 0, out, None) == 1

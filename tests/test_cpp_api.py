@@ -138,3 +138,31 @@ def test_gpu_phase_space_range(golden):
     assert np.max(np.abs(rows - gp) / np.abs(gp[..., 0:1])) <= 1e-12
     assert np.array_equal(bits(r["range_copy_w"]), bits(gw))
     assert r["range_err_unbounded"] == 1
+
+
+@pytest.mark.gpu
+def test_gpu_decay_chain(golden, oracle):
+    """phsp_chain.inl end to end through the C++ API: Generate, Generate over
+    GetDaugtherRange(_0), Meld, the example's variables copied into a
+    multivector, the weights, and a weighted histogram of the two angles."""
+    r = run("gpu")
+    gp = golden["chain_p"]
+    m = np.array(r["chain_melded_0_300"]).reshape(2, 5, 4)
+    exp = gp[[0, 300]]
+    assert np.max(np.abs(m - exp) / np.abs(exp[..., 0:1])) <= 1e-12
+    v = np.array(r["chain_vars"]).reshape(512, 4).T
+    ev = golden["chain_vars"]
+    M2 = 5.27955 ** 2
+    assert np.max(np.abs(v[0] - ev[0])) <= 1e-11 * M2
+    assert np.max(np.abs(v[1] - ev[1])) <= 1e-11 * M2
+    assert np.max(np.abs(v[2] - ev[2])) <= 1e-9
+    d = np.abs(v[3] - ev[3])
+    assert np.max(np.minimum(d, 2 * np.pi - d)) <= 1e-9
+    assert r["chain_row7"] == list(v[:, 7])
+    w = np.array(r["chain_weights"])
+    assert np.max(np.abs(w - golden["chain_w"]) / golden["chain_w"]) <= 1e-10
+    h = np.array(r["chain_hist_w"])
+    grid, lo, hi = [20, 16], [-1.0, 0.0], [1.0, 6.283185307179586]
+    exp_h = oracle.histogram_fill(grid, lo, hi, v[2:4], w)
+    scale = oracle.histogram_fill(grid, lo, hi, v[2:4], np.abs(w))
+    assert np.all(np.abs(h - exp_h) <= 1e-13 * scale + 1e-300)

@@ -604,3 +604,51 @@ def test_mass_functor_closed_form(oracle):
     assert oracle.functor(5, [0, 1], p) == math.sqrt(64.0 - 0.0 - 4.0 - 4.0)
     assert oracle.functor(5, [2, 2], p) == 0.0  # spacelike sum
     assert oracle.functor(5, [0, 1], p) == math.sqrt(oracle.functor(3, [0, 1], p))
+
+
+# ---------------------------------------------------------------- decay chain (Meld)
+B0_CHAIN = (5.27955, [3.0969, 0.493677, 0.13957061])
+JPSI_CHAIN = (3.0969, [0.1056583745] * 2)
+CHAIN_VARS = [(3, (0, 2)), (3, (1, 2)), (4, (1, 2, 0, 3)), (6, (1, 2, 3))]
+
+
+def chain_oracle(oracle, golden, n):
+    """phsp_chain.inl:234-244 restated: Generate(B0), Generate over the J/psi
+    column of each event, the rows melded: (n, 5, 4)."""
+    s3, s2 = (int(x) for x in golden["chain_seeds"])
+    c3, _ = oracle.generate(B0_CHAIN[0], B0_CHAIN[1], s3, n)
+    c2, _ = oracle.generate_mothers(JPSI_CHAIN[0], JPSI_CHAIN[1], s2, c3[0:4])
+    return np.concatenate([cols_to_rows(c3, 3), cols_to_rows(c2, 2)], axis=1)
+
+
+def test_chain_golden(oracle, golden):
+    rows = chain_oracle(oracle, golden, 512)
+    assert np.array_equal(bits(rows), bits(golden["chain_p"]))
+    for j, (fid, t) in enumerate(CHAIN_VARS):
+        got = np.array([oracle.functor(fid, list(t), rows[k]) for k in range(512)])
+        assert np.array_equal(bits(got), bits(golden["chain_vars"][j])), fid
+
+
+def test_chain_seeds_equal_reference(reference, golden):
+    got = [reference.params(M, m)[3] for M, m in (B0_CHAIN, JPSI_CHAIN)]
+    assert got == [int(x) for x in golden["chain_seeds"]]
+
+
+def test_decay_plane_angle_closed_forms(oracle):
+    # K along z: d in the x-z plane, h rotated about z by phi -> chi = phi;
+    # h in the plane of (d, K) on d's side -> 0; swapping the sides -> pi;
+    # chi is unchanged by boosts of h along K's direction and by scaling h
+    K_d = np.array([1.0, 0.3, 0.0, 0.5])
+    e = np.array([1.2, -0.3, 0.0, 0.7])
+    for phi in (0.3, 1.7, 3.0, 4.4, 6.1):
+        h = np.array([0.9, 0.4 * math.cos(phi), 0.4 * math.sin(phi), -0.2])
+        p = np.vstack([K_d, e, h])
+        chi = oracle.functor(6, [0, 1, 2], p)
+        assert abs(chi - phi) < 1e-13, (phi, chi)
+        h2 = h.copy()
+        h2[3] += 5.0
+        h2[1:3] *= 3.0
+        assert abs(oracle.functor(6, [0, 1, 2], np.vstack([K_d, e, h2])) - phi) < 1e-13
+    assert oracle.functor(6, [0, 1, 2], np.vstack([K_d, e, [1.0, 0.2, 0.0, 0.1]])) == 0.0
+    assert abs(oracle.functor(6, [0, 1, 2], np.vstack([K_d, e, [1.0, -0.2, 0.0, 0.1]]))
+               - math.pi) < 1e-15
