@@ -90,6 +90,12 @@ class Timer:
 
 
 def run_c2(torch, capi, n, steps, warmup):
+    """The headline step: Generate into SoA columns with the PhaseSpaceWeight
+    column, then Unweight() as the C++ Decays does it on a freshly generated
+    container (hpsp_unweight_regenerate: flags from the weight column, every
+    event generated again at its destination).  The moving Unweight
+    (hpsp_unweight: flags, then every event read and moved) is timed beside it
+    and gives the same container bit for bit."""
     lib = capi.lib()
     dec = capi.make_decay(M3, MASSES3)
     N = 3
@@ -101,17 +107,22 @@ def run_c2(torch, capi, n, steps, warmup):
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     nacc = torch.zeros(1, dtype=torch.int64, device=dev)
     sh = capi.stream_handle()
-    tg, tu, tf = Timer(torch), Timer(torch), Timer(torch)
+    tg, tu, tf, tm = Timer(torch), Timer(torch), Timer(torch), Timer(torch)
 
     def gen():
         capi.check(lib.hpsp_generate(ctypes.byref(dec), SEED, 0, n, capi.FP64, capi.ptr(cols), n,
                                      0, None, capi.ptr(evw), sh), "generate")
 
     def unweight(with_output=True):
+        capi.check(lib.hpsp_unweight_regenerate(ctypes.byref(dec), SEED, 0, n, capi.ptr(evw),
+                                                UW_SEED, capi.ptr(out) if with_output else None,
+                                                n, None, None, capi.ptr(ws), ws_bytes,
+                                                capi.ptr(nacc), sh), "unweight_regenerate")
+
+    def unweight_move():
         capi.check(lib.hpsp_unweight(ctypes.byref(dec), None, -1.0, UW_SEED, capi.ptr(cols), n,
-                                     n, capi.ptr(evw), capi.ptr(out) if with_output else None,
-                                     n, None, None, capi.ptr(ws), ws_bytes, capi.ptr(nacc), sh),
-                   "unweight")
+                                     n, capi.ptr(evw), capi.ptr(out), n, None, None,
+                                     capi.ptr(ws), ws_bytes, capi.ptr(nacc), sh), "unweight")
 
     for _ in range(warmup):
         gen()
@@ -130,6 +141,9 @@ def run_c2(torch, capi, n, steps, warmup):
         tf.start()
         unweight(with_output=False)
         tf.stop()
+        tm.start()
+        unweight_move()
+        tm.stop()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -138,17 +152,22 @@ def run_c2(torch, capi, n, steps, warmup):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1000.0 / steps
-    scatter_ms = tu.avg_ms - tf.avg_ms
+    regen_ms = tu.avg_ms - tf.avg_ms
+    move_ms = tm.avg_ms - tf.avg_ms
     res = {"ms_per_step": ms, "events_per_s": n / (ms / 1000.0), "n_acc": int(nacc.item()),
            "generate_ms": tg.avg_ms, "unweight_ms": tu.avg_ms, "flags_scan_ms": tf.avg_ms,
-           "scatter_ms": scatter_ms}
+           "regen_scatter_ms": regen_ms, "move_unweight_ms": tm.avg_ms,
+           "move_scatter_ms": move_ms,
+           "move_step_ms": tg.avg_ms + tm.avg_ms}
     gen_bytes = n * (32 * N + 8)
-    uw_bytes = n * 64 * N
+    regen_bytes = n * 32 * N
+    move_bytes = n * 64 * N
     res["generate_GBs"] = gen_bytes / (tg.avg_ms * 1e6)
-    res["unweight_GBs"] = uw_bytes / (tu.avg_ms * 1e6)
-    res["scatter_GBs"] = uw_bytes / (scatter_ms * 1e6)
+    res["regen_scatter_GBs"] = regen_bytes / (regen_ms * 1e6)
+    res["move_scatter_GBs"] = move_bytes / (move_ms * 1e6)
     res["generate_bytes_per_launch"] = gen_bytes
-    res["scatter_bytes_per_launch"] = uw_bytes
+    res["regen_scatter_bytes_per_launch"] = regen_bytes
+    res["move_scatter_bytes_per_launch"] = move_bytes
     del cols, evw, out, ws
     torch.cuda.empty_cache()
     return res
@@ -702,11 +721,12 @@ def main():
         print("# c5 done", file=sys.stderr, flush=True)
     extras["c2"] = c2
 
-    if c2["generate_ms"] >= c2["scatter_ms"]:
+    if c2["generate_ms"] >= c2["regen_scatter_ms"]:
         dom = ("k_generate_f64<3, 0, false>", c2["generate_GBs"], c2["generate_bytes_per_launch"],
                "generate")
     else:
-        dom = ("k_uw_scatter<3>", c2["scatter_GBs"], c2["scatter_bytes_per_launch"], "scatter")
+        dom = ("k_uw_regen<3, 0>", c2["regen_scatter_GBs"], c2["regen_scatter_bytes_per_launch"],
+               "regen")
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
@@ -742,7 +762,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (counter-based RNG, seed 159753)",
         "config": {"workload": "3-body decay 5.279 -> (3.0969, 0.493677, 0.13957) GeV, FP64, "
-                               "Generate into SoA + Unweight()", "events_per_step": args.n,
+                               "Generate into SoA + Unweight() (the unweighted events "
+                               "regenerated at their destinations)", "events_per_step": args.n,
                    "n_daughters": 3, "engine": "squares3", "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,

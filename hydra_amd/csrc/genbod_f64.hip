@@ -7,6 +7,7 @@
 #include <utility>
 
 #include "genbod_f64.h"
+#include "uw_dst.h"
 
 namespace hpsp {
 
@@ -63,6 +64,66 @@ void k_generate_f64(const GenArgs a) {
       if (a.wt) st(uniform_ptr(a.wt + k0) + lane, wt);
       if (a.evw) st(uniform_ptr(a.evw + k0) + lane, event_weight_f64<N>(p, a.d.m, a.d.maxw));
     }
+  }
+}
+
+// Decays::Unweight on a container that holds events gen_first + k of one
+// Generate (the flags and tile offsets already computed from its
+// PhaseSpaceWeight column): instead of reading every event and moving it to
+// its place in the stable partition, each event is generated again, with the
+// same operations and so the same bits, and written straight to its place.
+// The events are a function of (decay, seed, index) alone, so the momenta are
+// never read back: 32N bytes written per event (+16 with the weight column)
+// instead of 64N moved.
+//
+// One group of 64 events per wave, as in k_generate_f64; a wave finds its
+// group's offset itself, from the tile offset and the popcounts of the flag
+// words before it in the tile (wave-uniform loads, no LDS, no barrier).
+// Groups are dealt to XCDs in contiguous ranges, so that neighbouring groups,
+// which write neighbouring pieces of the accepted and rejected runs, meet in
+// one L2.
+template <int N, int ENGINE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
+void k_uw_regen(const UnweightArgs a) {
+  constexpr uint64_t kGroupsPerTile = kUwTile / kWave;
+  constexpr uint64_t kWavesPerBlock = kBlock / kWave;
+  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
+  const uint64_t nblocks = (ngroups + kWavesPerBlock - 1) / kWavesPerBlock;
+  const uint64_t per_xcd = (nblocks + kXcds - 1) / kXcds;
+  const uint64_t blk = (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const int lane = __lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t g = blk * kWavesPerBlock + (uint64_t)wave;
+  if (blk >= nblocks || g >= ngroups) return;  // wave-uniform
+  const uint64_t k0 = g * kWave;
+  const uint64_t tile = g / kGroupsPerTile;
+  uint64_t goff = a.tile_off[tile];
+  for (uint64_t i = tile * kGroupsPerTile; i < g; ++i) goff += (uint64_t)__popcll(a.flags[i]);
+  const uint64_t nacc = *a.n_acc;
+  const uint64_t fw = a.flags[g];
+
+  EventWords<N, ENGINE> words;
+  words.init(a.d, a.gen_first + k0, lane);
+  double p[N][4];
+  genbod_f64<N, ENGINE>(a.d, words, p);
+  if (a.d.do_boost) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) boost_folded(p[i], a.d);
+  }
+  const uint64_t k = k0 + (uint64_t)lane;
+  if (k < a.n) {
+    const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    bool kept;
+    const uint64_t dst = stable_dst(fw, below, lane, goff, nacc, k0, &kept);
+    if (a.cols_out) {
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          *(glob_u(a.cols_out + (uint64_t)(4 * i + q) * a.ld_out) + dst) = p[i][q];
+    }
+    if (a.w_out) *(glob(a.w_out) + dst) = *(glob(a.w_in) + k);
+    if (a.idx_out) *(glob(a.idx_out) + dst) = k;
   }
 }
 
@@ -224,8 +285,17 @@ static void avg_one(const AvgArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((k_average_f64<N, E, false>), dim3(g), dim3(kBlock), 0, s, a);
 }
 
+template <int N, int E>
+static void regen_one(const UnweightArgs& a, hipStream_t s) {
+  const uint64_t nblocks = (a.n + kBlock - 1) / kBlock;
+  const uint64_t grid = (nblocks + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL((k_uw_regen<N, E>), dim3((unsigned)grid), dim3(kBlock), 0, s, a);
+}
+
 template <int I>
 static constexpr auto gen_at = &gen_one<2 + I / kNumE, I % kNumE>;
+template <int I>
+static constexpr auto regen_at = &regen_one<2 + I / kNumE, I % kNumE>;
 template <int I>
 static constexpr auto avg_at = &avg_one<2 + I / kNumE, I % kNumE>;
 
@@ -234,12 +304,23 @@ static constexpr auto gen_table(std::integer_sequence<int, I...>) {
   return std::array<void (*)(const GenArgs&, hipStream_t), sizeof...(I)>{gen_at<I>...};
 }
 template <int... I>
+static constexpr auto regen_table(std::integer_sequence<int, I...>) {
+  return std::array<void (*)(const UnweightArgs&, hipStream_t), sizeof...(I)>{regen_at<I>...};
+}
+template <int... I>
 static constexpr auto avg_table(std::integer_sequence<int, I...>) {
   return std::array<void (*)(const AvgArgs&, hipStream_t), sizeof...(I)>{avg_at<I>...};
 }
 
 static const auto kGen = gen_table(std::make_integer_sequence<int, kNumN * kNumE>{});
 static const auto kAvg = avg_table(std::make_integer_sequence<int, kNumN * kNumE>{});
+static const auto kRegen = regen_table(std::make_integer_sequence<int, kNumN * kNumE>{});
+
+int launch_unweight_regen(int N, int engine, const UnweightArgs& a, hipStream_t s) {
+  if (a.n == 0) return (int)hipSuccess;
+  kRegen[(N - 2) * kNumE + engine](a, s);
+  return (int)hipGetLastError();
+}
 
 int launch_generate_f64(int N, int engine, const GenArgs& a, hipStream_t s) {
   kGen[(N - 2) * kNumE + engine](a, s);

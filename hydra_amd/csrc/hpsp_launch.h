@@ -165,6 +165,8 @@ struct UnweightArgs {
   uint64_t jump;        // counter offset of the uniforms (rule 1)
   int order;            // 0: both parts stable; 1: sequential swap partition
   uint64_t* tidx;       // order 1: original index of the r-th kept sample
+  int regen;            // events regenerated at their destinations, not moved
+  uint64_t gen_first;   // regen: generation index of the container's row 0
 };
 
 // Kernel launchers; each returns a hipError_t value.
@@ -184,6 +186,9 @@ int launch_event_weight(int N, const WeightArgs& a, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s);
+// after launch_unweight with regen set: the stable partition of events
+// gen_first .. gen_first + n - 1 of the generation stream, generated again
+int launch_unweight_regen(int N, int engine, const UnweightArgs& a, hipStream_t s);
 // hydra::unweight over ncols value columns and a weight column (rule 1).
 int launch_unweight_sample(int ncols, const UnweightArgs& a, double* block_max,
                            uint64_t nblocks, double* max_dev, hipStream_t s);

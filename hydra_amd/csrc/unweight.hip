@@ -19,11 +19,10 @@
 #include <array>
 #include <utility>
 
-#include "hpsp_launch.h"
+#include "uw_dst.h"
 
 namespace hpsp {
 
-constexpr int kItems = kUwTile / kBlock;  // 4 groups of 64 events per wave
 
 template <int N>
 __device__ __forceinline__ void load_event_uw(const double* cols, uint64_t ld, uint64_t k,
@@ -245,38 +244,6 @@ __global__ __launch_bounds__(kBlock) void k_uw_scatter(const UnweightArgs a) {
   }
 }
 
-// Destinations of one group of 64 samples in the stable partition: kept lanes
-// to goff + (kept lanes below), the others to nacc + (k0 - goff) + (rejected
-// lanes below).
-__device__ __forceinline__ uint64_t stable_dst(uint64_t fw, uint64_t below, int lane,
-                                               uint64_t goff, uint64_t nacc, uint64_t k0,
-                                               bool* kept) {
-  const uint32_t acc_below = (uint32_t)__popcll(fw & below);
-  *kept = (fw >> lane) & 1ULL;
-  return *kept ? goff + acc_below : nacc + k0 - goff + ((uint32_t)lane - acc_below);
-}
-
-// Group offsets of a tile in LDS (lanes 0..15 of wave 0 scan the popcounts).
-__device__ __forceinline__ void tile_group_offsets(const UnweightArgs& a, uint64_t tile,
-                                                   uint64_t* goff) {
-  const int lane = __lane_id();
-  const int wave = threadIdx.x / kWave;
-  const uint64_t ngroups = kUwTile / kWave;
-  if (wave == 0) {
-    const uint64_t gi = tile * ngroups + (uint64_t)lane;
-    const uint32_t own =
-        (lane < (int)ngroups && gi * kWave < a.n) ? (uint32_t)__popcll(a.flags[gi]) : 0u;
-    uint32_t c = own;
-#pragma unroll
-    for (int off = 1; off < (int)ngroups; off <<= 1) {
-      const uint32_t o = __shfl_up(c, off, kWave);
-      if (lane >= off) c += o;
-    }
-    if (lane < (int)ngroups) goff[lane] = a.tile_off[tile] + (c - own);
-  }
-  __syncthreads();
-}
-
 // Order 1: the original index of every kept sample by its rank.
 __global__ __launch_bounds__(kBlock) void k_uw_kept_index(const UnweightArgs a) {
   __shared__ uint64_t goff[kUwTile / kWave];
@@ -430,7 +397,7 @@ static void uw_one(const UnweightArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_uw_scan_top, dim3(1), dim3(kBlock), 0, s, a.run_sum, nruns, a.n_acc);
   hipLaunchKernelGGL(k_uw_scan_tiles, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
                      ntiles, a.run_sum, a.tile_off);
-  if (a.cols_out || a.idx_out) {
+  if ((a.cols_out || a.idx_out) && !a.regen) {
     const uint64_t grid = (ntiles + kXcds - 1) / kXcds * kXcds;
     hipLaunchKernelGGL(k_uw_scatter<N>, dim3((unsigned)grid), dim3(kBlock), 0, s, a);
   }

@@ -1,0 +1,52 @@
+// Destinations of the stable partition (Decays::Unweight, Decays.inl:
+// 434-453) from the flag words and tile offsets, shared by the moving scatter
+// (unweight.hip) and the regenerating one (genbod_f64.hip).
+#pragma once
+
+#include "hpsp_launch.h"
+
+namespace hpsp {
+
+constexpr int kItems = kUwTile / kBlock;  // 4 groups of 64 events per wave
+
+// Destinations of one group of 64 samples in the stable partition: kept lanes
+// to goff + (kept lanes below), the others to nacc + (k0 - goff) + (rejected
+// lanes below).
+__device__ __forceinline__ uint64_t stable_dst(uint64_t fw, uint64_t below, int lane,
+                                               uint64_t goff, uint64_t nacc, uint64_t k0,
+                                               bool* kept) {
+  const uint32_t acc_below = (uint32_t)__popcll(fw & below);
+  *kept = (fw >> lane) & 1ULL;
+  return *kept ? goff + acc_below : nacc + k0 - goff + ((uint32_t)lane - acc_below);
+}
+
+// Group offsets of a tile in LDS (lanes 0..15 of wave 0 scan the popcounts).
+__device__ __forceinline__ void tile_group_offsets(const UnweightArgs& a, uint64_t tile,
+                                                   uint64_t* goff) {
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave;
+  const uint64_t ngroups = kUwTile / kWave;
+  if (wave == 0) {
+    const uint64_t gi = tile * ngroups + (uint64_t)lane;
+    const uint32_t own =
+        (lane < (int)ngroups && gi * kWave < a.n) ? (uint32_t)__popcll(a.flags[gi]) : 0u;
+    uint32_t c = own;
+#pragma unroll
+    for (int off = 1; off < (int)ngroups; off <<= 1) {
+      const uint32_t o = __shfl_up(c, off, kWave);
+      if (lane >= off) c += o;
+    }
+    if (lane < (int)ngroups) goff[lane] = a.tile_off[tile] + (c - own);
+  }
+  __syncthreads();
+}
+
+// Tile of a block: each XCD takes a contiguous range of tiles, so that
+// neighbouring tiles, which write neighbouring pieces of the accepted and
+// rejected runs, share an L2.
+__device__ __forceinline__ uint64_t xcd_tile(uint64_t ntiles) {
+  const uint64_t per_xcd = (ntiles + kXcds - 1) / kXcds;
+  return (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+}
+
+}  // namespace hpsp

@@ -239,11 +239,16 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   size_t size() const { return fSize; }
   size_t capacity() const { return fCap; }
   bool empty() const { return fSize == 0; }
-  void clear() { fSize = 0; fWValid = false; }
+  void clear() {
+    fSize = 0;
+    fWValid = false;
+    fGenValid = false;
+  }
   void reserve(size_t n) {
     if (n > fCap) regrow(n);
   }
   void resize(size_t n) {
+    fGenValid = false;
     if (n > fCap) regrow(n);
     fSize = n;
     fWValid = false;
@@ -310,6 +315,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
 
   // ---- unweighting (Decays.inl:415-511)
   Range<iterator> Unweight(size_t seed = 0x180ec6d33cfd0aba) {
+    if (fGenValid && fWValid) return unweight_regenerate(seed);
     const hpsp_decay d = detail::make_decay<N>(fMotherMass, fMasses, "Decays::Unweight");
     return unweight(d, nullptr, -1.0, seed);
   }
@@ -330,14 +336,30 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
 
   // ---- raw device access (SoA)
   double* column(size_t particle, int component) {
+    fGenValid = false;
     return fCols.data() + (4 * particle + component) * fCap;
   }
   const double* column(size_t particle, int component) const {
     return fCols.data() + (4 * particle + component) * fCap;
   }
-  double* data() { return fCols.data(); }
+  double* data() {
+    fGenValid = false;
+    return fCols.data();
+  }
   const double* data() const { return fCols.data(); }
   size_t ld() const { return fCap; }
+
+  // The events are 0 .. size() - 1 of one FP64 PhaseSpace::Generate(mother,
+  // ...) with decay d and seed, untouched since (every non-const access to
+  // the columns forgets it).  Unweight() then generates them again at their
+  // destinations instead of moving them: the same bits, 8 bytes read per
+  // event instead of 32N.
+  void set_generated(hpsp_decay const& d, uint64_t seed) {
+    fGen = d;
+    fGenSeed = seed;
+    fGenValid = true;
+  }
+  bool generated() const { return fGenValid; }
 
   // PhaseSpaceWeight column kept alongside the momenta (null when unknown).
   const double* weights_cache() const { return fWValid ? fW.data() : nullptr; }
@@ -412,10 +434,36 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     fW.swap(wout);
     fWValid = fWValid && wc != nullptr;
     fCap = n ? n : 1;
+    fGenValid = false;
     return Range<iterator>(begin(), begin() + (std::ptrdiff_t)acc);
   }
 
  private:
+  Range<iterator> unweight_regenerate(size_t seed) {
+    const size_t n = fSize;
+    detail::device_buffer<double> out(4 * N * (n ? n : 1));
+    detail::device_buffer<double> wout(n ? n : 1);
+    const size_t wsb = hpsp_unweight_workspace_size(n);
+    detail::device_buffer<unsigned char> ws(wsb);
+    detail::device_buffer<uint64_t> nacc(1);
+    detail::hpsp_check(hpsp_unweight_regenerate(&fGen, fGenSeed, 0, n, fW.data(), (uint64_t)seed,
+                                                out.data(), n ? n : 1, nullptr, wout.data(),
+                                                ws.data(), wsb, nacc.data(), nullptr),
+                       "Decays::Unweight");
+    return adopt(out, wout, nacc);
+  }
+
+  Range<iterator> adopt(detail::device_buffer<double>& out, detail::device_buffer<double>& wout,
+                        detail::device_buffer<uint64_t>& nacc) {
+    uint64_t acc = 0;
+    detail::hpsp_check(hpsp_memcpy_to_host(&acc, nacc.data(), sizeof(acc)), "Decays::Unweight");
+    fCols.swap(out);
+    fW.swap(wout);
+    fCap = fSize ? fSize : 1;
+    fGenValid = false;
+    return Range<iterator>(begin(), begin() + (std::ptrdiff_t)acc);
+  }
+
   Range<iterator> unweight(const hpsp_decay& d, const hpsp_functor* f, double max_value,
                            size_t seed) {
     const size_t n = fSize;
@@ -430,13 +478,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
                                      win ? wout.data() : nullptr, ws.data(), wsb, nacc.data(),
                                      nullptr),
                        "Decays::Unweight");
-    uint64_t acc = 0;
-    detail::hpsp_check(hpsp_memcpy_to_host(&acc, nacc.data(), sizeof(acc)), "Decays::Unweight");
-    fCols.swap(out);
-    fW.swap(wout);
-    fCap = n ? n : 1;
-    if (!n) fCap = 1;
-    return Range<iterator>(begin(), begin() + (std::ptrdiff_t)acc);
+    return adopt(out, wout, nacc);
   }
 
   double fMotherMass = 0.0;
@@ -447,6 +489,9 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   detail::device_buffer<double> fCols;
   detail::device_buffer<double> fW;
   bool fWValid = false;
+  hpsp_decay fGen{};
+  uint64_t fGenSeed = 0;
+  bool fGenValid = false;
 };
 
 // Decays in host memory (the reference's Decays<tuple<P...>, host::sys_t>):

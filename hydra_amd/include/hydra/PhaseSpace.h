@@ -156,6 +156,7 @@ class PhaseSpace {
                        "PhaseSpace::Generate");
     detail::hpsp_sync("PhaseSpace::Generate");
     events.set_weights_cache_valid(same && events.size() > 0);
+    if (same && events.size() > 0) events.set_generated(d, (uint64_t)fSeed);
     return make_range(events.begin(), events.end());
   }
 

@@ -164,3 +164,20 @@ def range_histogram(grid, lo, hi, dec, seed, axes, n, first=0, mode=capi.HIST_WE
                                           stream_handle()), "hpsp_histogram_fill_range")
     torch.cuda.synchronize()
     return out.cpu().numpy()
+
+
+def unweight_regenerate(dec, gen_seed, first, n, w, seed, ld=None, want_out=True, want_idx=True,
+                        want_w=True):
+    """hpsp_unweight_regenerate -> (kept, cols, idx, w_out) as tensors."""
+    ld = ld or max(n, 1)
+    ws_bytes = lib().hpsp_unweight_workspace_size(n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
+    out = torch.empty((4 * dec.n, ld), dtype=torch.float64, device=DEV) if want_out else None
+    idx = torch.empty(max(n, 1), dtype=torch.int64, device=DEV) if want_idx else None
+    w_out = torch.empty(max(n, 1), dtype=torch.float64, device=DEV) if want_w else None
+    nacc = torch.zeros(1, dtype=torch.int64, device=DEV)
+    check(lib().hpsp_unweight_regenerate(ctypes.byref(dec), gen_seed, first, n, ptr(w), seed,
+                                         ptr(out), ld, ptr(idx), ptr(w_out), ptr(ws), ws_bytes,
+                                         ptr(nacc), stream_handle()), "hpsp_unweight_regenerate")
+    torch.cuda.synchronize()
+    return int(nacc.item()), out, idx, w_out

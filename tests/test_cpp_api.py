@@ -65,6 +65,9 @@ def test_gpu_generate_unweight_average(golden, oracle):
     refA = golden["uw_c1_p"][:4, 0]
     assert np.max(np.abs(first - refA) / refA[:, 0:1]) <= 1e-12
     assert r["unweight_f_nacc"] == int(golden["uwf_c1_nacc"][0])
+    # Unweight() regenerates a freshly generated container and moves one whose
+    # columns were handed out for writing: the same container either way
+    assert r["gen_flags"] == [1, 0] and r["unweight_paths_same"] == 1
     for key, gkey in (("avg_bw3", "avg_c1_bw3"), ("integrate_m2", "avg_c1_m2")):
         mean, sig = r[key]
         gm, gs = golden[gkey]
