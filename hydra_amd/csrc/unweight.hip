@@ -33,19 +33,24 @@ __device__ __forceinline__ void load_event_uw(const double* cols, uint64_t ld, u
     for (int q = 0; q < 4; ++q) p[i][q] = cols[(uint64_t)(4 * i + q) * ld + k];
 }
 
+// N = 0: the weights come from a.w_in alone (no functor), and the kernel
+// holds no momenta: few registers, many waves in flight for the one stream it
+// reads.  Tiles are dealt to XCDs in contiguous ranges.
 template <int N>
 __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
 #pragma clang fp contract(off)
   __shared__ uint32_t cnt[kBlock / kWave];
   const int lane = __lane_id();
   const int wave = threadIdx.x / kWave;
-  const uint64_t tile = blockIdx.x;
+  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
+  const uint64_t tile = xcd_tile(ntiles);
+  if (tile >= ntiles) return;  // whole block
   const double maxv = a.maxv_dev ? *a.maxv_dev : a.maxv;
-  const bool w_only = a.w_in && !a.use_f;
+  constexpr bool w_only = (N == 0);
   // with a weight column, the wave's four loads are issued before any flag
   // word is stored
   double wcol[kItems];
-  if (w_only) {
+  if constexpr (w_only) {
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
       const uint64_t g = tile * (kUwTile / kWave) + (uint64_t)j * (kBlock / kWave) + wave;
@@ -61,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
     bool acc = false;
     if (k < a.n) {
       double w;
-      if (w_only) {
+      if constexpr (w_only) {
         w = wcol[j];
       } else {
         double p[N][4];
@@ -387,10 +392,15 @@ __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
 
 constexpr int kNumN = HPSP_MAX_N - 1;
 
+static void launch_flags(int N, const UnweightArgs& a, uint64_t ntiles, hipStream_t s);
+
 template <int N>
 static void uw_one(const UnweightArgs& a, hipStream_t s) {
   const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
-  hipLaunchKernelGGL(k_uw_flags<N>, dim3((unsigned)ntiles), dim3(kBlock), 0, s, a);
+  if (a.w_in && !a.use_f)
+    launch_flags(0, a, ntiles, s);
+  else
+    launch_flags(N, a, ntiles, s);
   const uint64_t nruns = (ntiles + kScanRun - 1) / kScanRun;
   hipLaunchKernelGGL(k_uw_scan_blocks, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
                      ntiles, a.run_sum);
@@ -423,7 +433,7 @@ int launch_unweight_sample(int ncols, const UnweightArgs& in, double* block_max,
     a.maxv_dev = max_dev;
   }
   const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
-  hipLaunchKernelGGL(k_uw_flags<2>, dim3((unsigned)ntiles), dim3(kBlock), 0, s, a);
+  launch_flags(0, a, ntiles, s);
   const uint64_t nruns = (ntiles + kScanRun - 1) / kScanRun;
   hipLaunchKernelGGL(k_uw_scan_blocks, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
                      ntiles, a.run_sum);
@@ -438,6 +448,23 @@ int launch_unweight_sample(int ncols, const UnweightArgs& in, double* block_max,
                        a.cols_out ? ncols : 0);
   }
   return (int)hipGetLastError();
+}
+
+template <int N>
+static void flags_one(const UnweightArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_uw_flags<N>, dim3(grid), dim3(kBlock), 0, s, a);
+}
+template <int... I>
+static constexpr auto flags_table(std::integer_sequence<int, I...>) {
+  return std::array<void (*)(const UnweightArgs&, unsigned, hipStream_t), sizeof...(I)>{
+      &flags_one<(I == 0 ? 0 : I + 1)>...};
+}
+// index 0: weight column only; N - 1: momenta of N daughters
+static const auto kFlags = flags_table(std::make_integer_sequence<int, HPSP_MAX_N>{});
+
+static void launch_flags(int N, const UnweightArgs& a, uint64_t ntiles, hipStream_t s) {
+  const unsigned grid = (unsigned)((ntiles + kXcds - 1) / kXcds * kXcds);
+  kFlags[N == 0 ? 0 : N - 1](a, grid, s);
 }
 
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s) {
