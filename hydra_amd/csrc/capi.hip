@@ -131,7 +131,8 @@ static int generate_common(const hpsp_decay* d, uint64_t seed, uint64_t first, u
   if (rc) return rc;
   if (prec != HPSP_FP64 && prec != HPSP_MIXED) return HPSP_ERR_INVALID_ARGUMENT;
   if (n == 0) return HPSP_OK;
-  if (!cols || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!cols && !wt && !evw) return HPSP_ERR_INVALID_ARGUMENT;
+  if (cols && ld < n) return HPSP_ERR_INVALID_ARGUMENT;
   if (mothers && mld < n) return HPSP_ERR_INVALID_ARGUMENT;
   GenArgs a;
   memset(&a, 0, sizeof(a));

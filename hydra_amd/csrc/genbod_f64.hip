@@ -46,7 +46,9 @@ void k_generate_f64(const GenArgs a) {
     }
 
     if (live) {
-      if (a.out_f32) {
+      if (!a.cols) {
+        // weights only: the momenta are formed for them and not stored
+      } else if (a.out_f32) {
         float* c = static_cast<float*>(a.cols) + k0;
 #pragma unroll
         for (int i = 0; i < N; ++i)

@@ -219,7 +219,9 @@ __global__ __launch_bounds__(kBlock) void k_generate_mixed(const GenArgs a) {
       final_boost_f32<N>(p, a.d);
     }
     if (live) {
-      if (a.out_f32) {
+      if (!a.cols) {
+        // weights only
+      } else if (a.out_f32) {
         float* c = static_cast<float*>(a.cols) + k0;
 #pragma unroll
         for (int i = 0; i < N; ++i)

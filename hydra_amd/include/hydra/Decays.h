@@ -243,11 +243,13 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     fSize = 0;
     fWValid = false;
     fGenValid = false;
+    fLazy = false;
   }
   void reserve(size_t n) {
     if (n > fCap) regrow(n);
   }
   void resize(size_t n) {
+    materialize();
     fGenValid = false;
     if (n > fCap) regrow(n);
     fSize = n;
@@ -259,6 +261,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
 
   value_type operator[](size_t k) const {
     std::array<double, 4 * N> v;
+    materialize();
     if (fSize)
       detail::hpsp_check(hpsp_memcpy2d_to_host(v.data(), sizeof(double), fCols.data() + k,
                                                fCap * sizeof(double), sizeof(double), 4 * N),
@@ -336,18 +339,44 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
 
   // ---- raw device access (SoA)
   double* column(size_t particle, int component) {
+    materialize();
     fGenValid = false;
     return fCols.data() + (4 * particle + component) * fCap;
   }
   const double* column(size_t particle, int component) const {
+    materialize();
     return fCols.data() + (4 * particle + component) * fCap;
   }
   double* data() {
+    materialize();
     fGenValid = false;
     return fCols.data();
   }
-  const double* data() const { return fCols.data(); }
+  const double* data() const {
+    materialize();
+    return fCols.data();
+  }
   size_t ld() const { return fCap; }
+
+  // For a writer that replaces every column: no materialisation, and the
+  // container forgets which events it held.
+  double* columns_for_overwrite() {
+    fLazy = false;
+    fGenValid = false;
+    return fCols.data();
+  }
+
+  // PhaseSpace::Generate(mother, events) with this container's decay: only
+  // the PhaseSpaceWeight column has been written; the momenta of events
+  // 0 .. size() - 1 of (d, seed) are formed when first read (any access to
+  // the columns, a row, a particle range, a copy to the host) or written
+  // straight to their places by Unweight().  A function of (d, seed, index)
+  // alone, so the deferred write gives the bits an eager one would.
+  void set_generated_lazy(hpsp_decay const& d, uint64_t seed) {
+    set_generated(d, seed);
+    fLazy = true;
+  }
+  bool lazy() const { return fLazy; }
 
   // The events are 0 .. size() - 1 of one FP64 PhaseSpace::Generate(mother,
   // ...) with decay d and seed, untouched since (every non-const access to
@@ -380,7 +409,19 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     fSize = n;
   }
 
+  // Write the deferred momenta of a lazy Generate.
+  void materialize() const {
+    if (!fLazy) return;
+    fLazy = false;
+    if (fSize == 0) return;
+    detail::hpsp_check(hpsp_generate(&fGen, fGenSeed, 0, fSize, HPSP_FP64, fCols.data(), fCap, 0,
+                                     nullptr, nullptr, nullptr),
+                       "Decays (deferred Generate)");
+    detail::hpsp_sync("Decays (deferred Generate)");
+  }
+
   void regrow(size_t cap) {
+    materialize();
     detail::device_buffer<double> cols(4 * N * cap);
     detail::device_buffer<double> w(cap);
     const size_t keep = fSize < cap ? fSize : cap;
@@ -405,6 +446,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
  public:
   Range<iterator> UnweightBy(EventWeights const& w, double max_pdf, size_t rng_seed,
                              size_t rng_jump) {
+    materialize();
     const size_t n = fSize;
     if (w.size() < n) throw std::invalid_argument("[hydra::unweight]: fewer weights than events");
     detail::device_buffer<double> out(4 * N * (n ? n : 1));
@@ -461,11 +503,13 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     fW.swap(wout);
     fCap = fSize ? fSize : 1;
     fGenValid = false;
+    fLazy = false;
     return Range<iterator>(begin(), begin() + (std::ptrdiff_t)acc);
   }
 
   Range<iterator> unweight(const hpsp_decay& d, const hpsp_functor* f, double max_value,
                            size_t seed) {
+    materialize();
     const size_t n = fSize;
     detail::device_buffer<double> out(4 * N * (n ? n : 1));
     detail::device_buffer<double> wout(n ? n : 1);
@@ -486,12 +530,13 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   std::array<double, N> fMasses{};
   size_t fSize = 0;
   size_t fCap = 0;
-  detail::device_buffer<double> fCols;
+  mutable detail::device_buffer<double> fCols;
   detail::device_buffer<double> fW;
   bool fWValid = false;
   hpsp_decay fGen{};
   uint64_t fGenSeed = 0;
   bool fGenValid = false;
+  mutable bool fLazy = false;
 };
 
 // Decays in host memory (the reference's Decays<tuple<P...>, host::sys_t>):
@@ -569,6 +614,7 @@ template <typename... Particles>
 Decays<hydra::tuple<Particles...>, hydra::device::sys_t>&
 Decays<hydra::tuple<Particles...>, hydra::device::sys_t>::operator=(
     Decays<hydra::tuple<Particles...>, hydra::host::sys_t> const& other) {
+  columns_for_overwrite();  // every column is replaced: nothing to keep
   resize(other.size());
   fWValid = false;
   if (fSize)

@@ -150,13 +150,24 @@ class PhaseSpace {
     // this generator's (same masses and fMaxWeight, bit for bit)
     const bool same = same_decay(events.GetMotherMass(), events.GetMasses()) &&
                       events.GetMaxWeight() == fMaxWeight;
-    detail::hpsp_check(hpsp_generate(&d, fSeed, 0, events.size(), HPSP_FP64, events.data(),
-                                     events.ld(), 0, nullptr,
-                                     same ? events.weights_cache_for_write() : nullptr, nullptr),
-                       "PhaseSpace::Generate");
-    detail::hpsp_sync("PhaseSpace::Generate");
-    events.set_weights_cache_valid(same && events.size() > 0);
-    if (same && events.size() > 0) events.set_generated(d, (uint64_t)fSeed);
+    double* cols = events.columns_for_overwrite();
+    if (same && events.size() > 0) {
+      // the weights now, the momenta when first read or at their places in
+      // Unweight() (the container keeps (decay, seed) to form them)
+      detail::hpsp_check(hpsp_generate(&d, fSeed, 0, events.size(), HPSP_FP64, nullptr,
+                                       events.ld(), 0, nullptr, events.weights_cache_for_write(),
+                                       nullptr),
+                         "PhaseSpace::Generate");
+      detail::hpsp_sync("PhaseSpace::Generate");
+      events.set_weights_cache_valid(true);
+      events.set_generated_lazy(d, (uint64_t)fSeed);
+    } else {
+      detail::hpsp_check(hpsp_generate(&d, fSeed, 0, events.size(), HPSP_FP64, cols, events.ld(),
+                                       0, nullptr, nullptr, nullptr),
+                         "PhaseSpace::Generate");
+      detail::hpsp_sync("PhaseSpace::Generate");
+      events.set_weights_cache_valid(false);
+    }
     return make_range(events.begin(), events.end());
   }
 
@@ -166,13 +177,14 @@ class PhaseSpace {
       hydra::device::vector<Vector4R> const& mothers,
       Decays<hydra::tuple<P...>, hydra::device::sys_t>& daughters) {
     static_assert(sizeof...(P) == N, "container arity differs from PhaseSpace<N>");
+    daughters.columns_for_overwrite();
     if (daughters.size() < mothers.size()) daughters.resize(mothers.size());
     const hpsp_decay d = descriptor(Vector4R(fMotherMass, 0, 0, 0));
     const bool same = same_decay(daughters.GetMotherMass(), daughters.GetMasses()) &&
                       daughters.GetMaxWeight() == fMaxWeight;
     detail::hpsp_check(
         hpsp_generate_mothers(&d, fSeed, 0, mothers.size(), mothers.data(), mothers.ld(),
-                              HPSP_FP64, daughters.data(), daughters.ld(), 0, nullptr,
+                              HPSP_FP64, daughters.columns_for_overwrite(), daughters.ld(), 0, nullptr,
                               same ? daughters.weights_cache_for_write() : nullptr, nullptr),
         "PhaseSpace::Generate");
     detail::hpsp_sync("PhaseSpace::Generate");
@@ -189,13 +201,14 @@ class PhaseSpace {
       MotherRange const& mothers, Decays<hydra::tuple<P...>, hydra::device::sys_t>& daughters) {
     static_assert(sizeof...(P) == N, "container arity differs from PhaseSpace<N>");
     const size_t n = mothers.size();
+    daughters.columns_for_overwrite();
     if (daughters.size() < n) daughters.resize(n);
     const hpsp_decay d = descriptor(Vector4R(fMotherMass, 0, 0, 0));
     const bool same = same_decay(daughters.GetMotherMass(), daughters.GetMasses()) &&
                       daughters.GetMaxWeight() == fMaxWeight;
     detail::hpsp_check(
         hpsp_generate_mothers(&d, fSeed, 0, n, mothers.data(0), mothers.ld(), HPSP_FP64,
-                              daughters.data(), daughters.ld(), 0, nullptr,
+                              daughters.columns_for_overwrite(), daughters.ld(), 0, nullptr,
                               same ? daughters.weights_cache_for_write() : nullptr, nullptr),
         "PhaseSpace::Generate");
     detail::hpsp_sync("PhaseSpace::Generate");

@@ -104,7 +104,7 @@ void copy(PhaseSpaceRange<N> const& range,
   static_assert(sizeof...(Particles) == N, "the container needs N daughters");
   const size_t n = events.size();
   if (w && w->size() < n) throw std::invalid_argument("[hydra::copy]: too few weight slots");
-  range.generate(0, n, events.data(), events.ld(), w ? w->data() : nullptr);
+  range.generate(0, n, events.columns_for_overwrite(), events.ld(), w ? w->data() : nullptr);
   events.set_weights_cache_valid(false);
   detail::hpsp_sync("hydra::copy");
 }

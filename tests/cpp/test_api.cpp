@@ -112,6 +112,8 @@ int main(int argc, char** argv) {
     const size_t n = 4096;
     hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> Events(P_mass, masses, n);
     phsp.Generate(Parent, Events);
+    // the momenta are formed when first read (here, by the rows below)
+    std::printf("\"lazy_after_generate\": %d,\n", Events.lazy() ? 1 : 0);
     std::vector<double> rows;
     for (size_t i = 0; i < 16; ++i) {
       auto ev = Events[i];
@@ -120,6 +122,27 @@ int main(int argc, char** argv) {
         for (int q = 0; q < 4; ++q) rows.push_back(v->get(q));
     }
     emit_array("gen_rows16", rows);
+    std::printf("\"lazy_after_read\": %d,\n", Events.lazy() ? 1 : 0);
+    // a copy taken while lazy forms the same momenta when read, and a
+    // container of other masses is filled eagerly with the same momenta
+    hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> L1(P_mass, masses, n);
+    phsp.Generate(Parent, L1);
+    auto L2 = L1;
+    double masses_b[3]{A_mass, B_mass, C_mass * 1.0000001};
+    hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> Eager(P_mass, masses_b, n);
+    phsp.Generate(Parent, Eager);
+    hydra::Decays<hydra::tuple<A, B, C>, hydra::host::sys_t> h_l2(L2), h_eager(Eager), h_ev(Events);
+    int lazy_same = (L1.lazy() && !Eager.lazy()) ? 1 : 0;
+    for (size_t c = 0; c < 12 && lazy_same; ++c)
+      for (size_t i = 0; i < n; ++i)
+        if (std::memcmp(h_l2.data() + c * h_l2.ld() + i, h_eager.data() + c * h_eager.ld() + i,
+                        sizeof(double)) != 0 ||
+            std::memcmp(h_l2.data() + c * h_l2.ld() + i, h_ev.data() + c * h_ev.ld() + i,
+                        sizeof(double)) != 0) {
+          lazy_same = 0;
+          break;
+        }
+    std::printf("\"lazy_same\": %d,\n", lazy_same);
     auto weights = Events | Events.GetEventWeightFunctor();
     emit_array("gen_weights", weights.to_host());
     auto pi_range = Events.GetDaugtherRange(hydra::placeholders::_2);

@@ -68,6 +68,9 @@ def test_gpu_generate_unweight_average(golden, oracle):
     # Unweight() regenerates a freshly generated container and moves one whose
     # columns were handed out for writing: the same container either way
     assert r["gen_flags"] == [1, 0] and r["unweight_paths_same"] == 1
+    # Generate writes the weights and defers the momenta to their first read
+    assert r["lazy_after_generate"] == 1 and r["lazy_after_read"] == 0
+    assert r["lazy_same"] == 1
     for key, gkey in (("avg_bw3", "avg_c1_bw3"), ("integrate_m2", "avg_c1_m2")):
         mean, sig = r[key]
         gm, gs = golden[gkey]

@@ -70,3 +70,24 @@ def test_regen_empty():
     w = torch.empty(1, dtype=torch.float64, device=DEV)
     k, _, _, _ = unweight_regenerate(dec, 1, 0, 0, w, UW_SEED)
     assert k == 0
+
+
+@pytest.mark.parametrize("engine", ["squares3", "philox"])
+def test_weights_only_generate(engine):
+    """hpsp_generate with no columns writes the same weights; with the
+    regenerating Unweight it gives the same container as the eager step."""
+    import ctypes
+    from hydra_amd import capi
+    from hydra_amd.capi import check, lib, ptr, stream_handle
+    dec = make_decay(5.279, [3.0969, 0.493677, 0.13957], engine=engine)
+    n = 50000
+    cols, wt, evw = generate(dec, 99, n, first=1234)
+    wt2 = torch.empty(n, dtype=torch.float64, device=DEV)
+    evw2 = torch.empty(n, dtype=torch.float64, device=DEV)
+    check(lib().hpsp_generate(ctypes.byref(dec), 99, 1234, n, capi.FP64, None, n, 0, ptr(wt2),
+                              ptr(evw2), stream_handle()), "hpsp_generate")
+    torch.cuda.synchronize()
+    assert np.array_equal(bits(wt), bits(wt2)) and np.array_equal(bits(evw), bits(evw2))
+    k1, out1, idx1, _ = unweight(dec, cols, UW_SEED, w_in=evw)
+    k2, out2, idx2, _ = unweight_regenerate(dec, 99, 1234, n, evw2, UW_SEED)
+    assert k1 == k2 and np.array_equal(bits(out1), bits(out2))
