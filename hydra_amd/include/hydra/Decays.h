@@ -409,15 +409,18 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     fSize = n;
   }
 
-  // Write the deferred momenta of a lazy Generate.
+  // Write the deferred momenta of a lazy Generate.  Const accessors call it,
+  // so concurrent const use of one container from several host threads needs
+  // the caller's synchronisation (as any use of the reference's containers).
   void materialize() const {
     if (!fLazy) return;
+    if (fSize > 0) {
+      detail::hpsp_check(hpsp_generate(&fGen, fGenSeed, 0, fSize, HPSP_FP64, fCols.data(), fCap,
+                                       0, nullptr, nullptr, nullptr),
+                         "Decays (deferred Generate)");
+      detail::hpsp_sync("Decays (deferred Generate)");
+    }
     fLazy = false;
-    if (fSize == 0) return;
-    detail::hpsp_check(hpsp_generate(&fGen, fGenSeed, 0, fSize, HPSP_FP64, fCols.data(), fCap, 0,
-                                     nullptr, nullptr, nullptr),
-                       "Decays (deferred Generate)");
-    detail::hpsp_sync("Decays (deferred Generate)");
   }
 
   void regrow(size_t cap) {
