@@ -168,6 +168,70 @@ __global__ __launch_bounds__(kBlock) void k_uw_scan_tiles(const uint32_t* tile_c
 // Tiles are mapped so that each XCD takes a contiguous range of them:
 // neighbouring tiles write neighbouring pieces of the accepted and rejected
 // runs, and the cache lines they share then meet in one L2.
+// The scatter for N <= 6: one group of 64 events per wave, 256 per block.
+// Every event is read where it lies (lane-contiguous), staged in LDS at its
+// rank in the block's partition order, and each column is written by all 256
+// threads as the block's two runs (accepted, rejected), so that only the ends
+// of a block's runs are partial cache lines.  Blocks are dealt to XCDs in
+// contiguous ranges.
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_uw_scatter_staged(const UnweightArgs a) {
+  constexpr int C = 4 * N;
+  __shared__ double sh[C][kBlock];
+  constexpr uint64_t kGroupsPerTile = kUwTile / kWave;
+  constexpr int kWavesPerBlock = kBlock / kWave;
+  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
+  const uint64_t nblocks = (ngroups + kWavesPerBlock - 1) / kWavesPerBlock;
+  const uint64_t per_xcd = (nblocks + kXcds - 1) / kXcds;
+  const uint64_t blk = (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (blk >= nblocks) return;  // whole block
+  const int lane = __lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t g0 = blk * kWavesPerBlock;
+  const uint64_t tile = g0 / kGroupsPerTile;
+  uint64_t goff = a.tile_off[tile];
+  for (uint64_t i = tile * kGroupsPerTile; i < g0; ++i) goff += (uint64_t)__popcll(a.flags[i]);
+  uint32_t acc_wave = 0, a_blk = 0;
+#pragma unroll
+  for (int w = 0; w < kWavesPerBlock; ++w) {
+    const uint64_t gw = g0 + (uint64_t)w;
+    const uint32_t c = (gw < ngroups) ? (uint32_t)__popcll(a.flags[gw]) : 0u;
+    acc_wave = (w == wave) ? a_blk : acc_wave;
+    a_blk += c;
+  }
+  const uint64_t nacc = *a.n_acc;
+  const uint64_t kb = g0 * kWave;
+  const uint64_t g = g0 + (uint64_t)wave;
+  const uint64_t k0 = g * kWave;
+  const uint64_t k = k0 + (uint64_t)lane;
+  if (g < ngroups && k < a.n) {
+    const uint64_t fw = a.flags[g];
+    const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    const uint32_t acc_below = acc_wave + (uint32_t)__popcll(fw & below);
+    const bool kept = (fw >> lane) & 1ULL;
+    const uint32_t t = (uint32_t)(wave * kWave + lane);
+    const uint32_t r = kept ? acc_below : a_blk + (t - acc_below);
+    const uint64_t dst = (r < a_blk) ? goff + r : nacc + (kb - goff) + (r - a_blk);
+    if (a.cols_out) {
+      double v[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) v[c] = *(glob_u(a.cols_in + (uint64_t)c * a.ld_in + k0) + lane);
+#pragma unroll
+      for (int c = 0; c < C; ++c) sh[c][r] = v[c];
+    }
+    if (a.w_out) *(glob(a.w_out) + dst) = *(glob(a.w_in) + k);
+    if (a.idx_out) *(glob(a.idx_out) + dst) = k;
+  }
+  __syncthreads();
+  const uint64_t live = (a.n - kb) < (uint64_t)kBlock ? (a.n - kb) : (uint64_t)kBlock;
+  const uint32_t t = threadIdx.x;
+  if (a.cols_out && t < live) {
+    const uint64_t dst = (t < a_blk) ? goff + t : nacc + (kb - goff) + (t - a_blk);
+#pragma unroll
+    for (int c = 0; c < C; ++c) *(glob_u(a.cols_out + (uint64_t)c * a.ld_out) + dst) = sh[c][t];
+  }
+}
+
 template <int N>
 __global__ __launch_bounds__(kBlock) void k_uw_scatter(const UnweightArgs a) {
   __shared__ uint64_t goff[kUwTile / kWave];
@@ -408,8 +472,14 @@ static void uw_one(const UnweightArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_uw_scan_tiles, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
                      ntiles, a.run_sum, a.tile_off);
   if ((a.cols_out || a.idx_out) && !a.regen) {
-    const uint64_t grid = (ntiles + kXcds - 1) / kXcds * kXcds;
-    hipLaunchKernelGGL(k_uw_scatter<N>, dim3((unsigned)grid), dim3(kBlock), 0, s, a);
+    if constexpr (N <= 6) {
+      const uint64_t nblocks = (a.n + kBlock - 1) / kBlock;
+      const uint64_t grid = (nblocks + kXcds - 1) / kXcds * kXcds;
+      hipLaunchKernelGGL(k_uw_scatter_staged<N>, dim3((unsigned)grid), dim3(kBlock), 0, s, a);
+    } else {
+      const uint64_t grid = (ntiles + kXcds - 1) / kXcds * kXcds;
+      hipLaunchKernelGGL(k_uw_scatter<N>, dim3((unsigned)grid), dim3(kBlock), 0, s, a);
+    }
   }
 }
 
