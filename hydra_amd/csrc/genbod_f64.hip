@@ -86,15 +86,17 @@ void k_generate_f64(const GenArgs a) {
 // (8% faster at N = 3).  Larger N stores from registers (their LDS would cost
 // occupancy).  Blocks are dealt to XCDs in contiguous ranges, so that
 // neighbouring blocks, which write neighbouring pieces of both runs, meet in
-// one L2.  A block's offset comes from the tile offset and the popcounts of
-// the flag words before it in the tile (wave-uniform loads).
+// one L2.  A block's offset comes from the tile offset and a 16-lane scan of
+// the popcounts of its tile's flag words (block_part).
 template <int N, int ENGINE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
 void k_uw_regen(const UnweightArgs a) {
   constexpr int C = 4 * N;
   constexpr bool kStage = N <= 6;
-  __shared__ double sh[kStage ? C : 1][kStage ? kBlock : 1];
-  constexpr uint64_t kGroupsPerTile = kUwTile / kWave;
+  // columns staged at once: all of them up to 8, else half, so that a block's
+  // LDS (12 KB at N = 3) does not cap the waves in flight below the registers
+  constexpr int CS = !kStage ? 1 : (C <= 8 ? C : C / 2);
+  __shared__ double sh[CS][kStage ? kBlock : 1];
   constexpr int kWavesPerBlock = kBlock / kWave;
   const uint64_t ngroups = (a.n + kWave - 1) / kWave;
   const uint64_t nblocks = (ngroups + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -104,28 +106,21 @@ void k_uw_regen(const UnweightArgs a) {
   const int lane = __lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const uint64_t g0 = blk * kWavesPerBlock;  // the block's first group
-  const uint64_t tile = g0 / kGroupsPerTile;
-  uint64_t goff = a.tile_off[tile];  // accepted events before the block
-  for (uint64_t i = tile * kGroupsPerTile; i < g0; ++i) goff += (uint64_t)__popcll(a.flags[i]);
-  // accepted events of the block before each of its waves
-  uint32_t acc_wave = 0, a_blk = 0;
-#pragma unroll
-  for (int w = 0; w < kWavesPerBlock; ++w) {
-    const uint64_t gw = g0 + (uint64_t)w;
-    const uint32_t c = (gw < ngroups) ? (uint32_t)__popcll(a.flags[gw]) : 0u;
-    acc_wave = (w == wave) ? a_blk : acc_wave;
-    a_blk += c;
-  }
+  const BlockPart bp = block_part(a, g0, wave, ngroups);
+  const uint64_t goff = bp.goff;  // accepted events before the block
+  const uint32_t acc_wave = bp.acc_wave, a_blk = bp.a_blk;
   const uint64_t nacc = *a.n_acc;
   const uint64_t kb = g0 * kWave;  // the block's first event
   const uint64_t g = g0 + (uint64_t)wave;
   const uint64_t k0 = g * kWave;
   const uint64_t k = k0 + (uint64_t)lane;
-  if (g < ngroups) {  // wave-uniform
-    const uint64_t fw = a.flags[g];
+  const bool live_wave = g < ngroups;  // wave-uniform
+  double p[N][4];
+  uint32_t r = 0;  // the event's rank in the block's partition order
+  if (live_wave) {
+    const uint64_t fw = bp.fw;
     EventWords<N, ENGINE> words;
     words.init(a.d, a.gen_first + k0, lane);
-    double p[N][4];
     genbod_f64<N, ENGINE>(a.d, words, p);
     if (a.d.do_boost) {
 #pragma unroll
@@ -135,34 +130,39 @@ void k_uw_regen(const UnweightArgs a) {
     const uint32_t acc_below = acc_wave + (uint32_t)__popcll(fw & below);
     const bool kept = (fw >> lane) & 1ULL;
     const uint32_t t = (uint32_t)(wave * kWave + lane);
-    // the event's rank in the block's partition order
-    const uint32_t r = kept ? acc_below : a_blk + (t - acc_below);
+    r = kept ? acc_below : a_blk + (t - acc_below);
     if (k < a.n) {
       const uint64_t dst = (r < a_blk) ? goff + r : nacc + (kb - goff) + (r - a_blk);
-      if constexpr (kStage) {
+      if constexpr (!kStage) {
+        if (a.cols_out) {
 #pragma unroll
-        for (int i = 0; i < N; ++i)
+          for (int i = 0; i < N; ++i)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) sh[4 * i + q][r] = p[i][q];
-      } else if (a.cols_out) {
-#pragma unroll
-        for (int i = 0; i < N; ++i)
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            *(glob_u(a.cols_out + (uint64_t)(4 * i + q) * a.ld_out) + dst) = p[i][q];
+            for (int q = 0; q < 4; ++q)
+              *(glob_u(a.cols_out + (uint64_t)(4 * i + q) * a.ld_out) + dst) = p[i][q];
+        }
       }
       if (a.w_out) *(glob(a.w_out) + dst) = *(glob(a.w_in) + k);
       if (a.idx_out) *(glob(a.idx_out) + dst) = k;
     }
   }
   if constexpr (kStage) {
-    __syncthreads();
     const uint64_t live = (a.n - kb) < (uint64_t)kBlock ? (a.n - kb) : (uint64_t)kBlock;
     const uint32_t t = threadIdx.x;
-    if (a.cols_out && t < live) {
-      const uint64_t dst = (t < a_blk) ? goff + t : nacc + (kb - goff) + (t - a_blk);
+    const uint64_t dst = (t < a_blk) ? goff + t : nacc + (kb - goff) + (t - a_blk);
 #pragma unroll
-      for (int c = 0; c < C; ++c) *(glob_u(a.cols_out + (uint64_t)c * a.ld_out) + dst) = sh[c][t];
+    for (int h = 0; h < C; h += CS) {
+      if (h) __syncthreads();  // the previous columns have been read
+      if (live_wave && k < a.n) {
+#pragma unroll
+        for (int c = h; c < h + CS; ++c) sh[c - h][r] = p[c / 4][c % 4];
+      }
+      __syncthreads();
+      if (a.cols_out && t < live) {
+#pragma unroll
+        for (int c = h; c < h + CS; ++c)
+          *(glob_u(a.cols_out + (uint64_t)c * a.ld_out) + dst) = sh[c - h][t];
+      }
     }
   }
 }

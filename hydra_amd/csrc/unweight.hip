@@ -177,8 +177,10 @@ __global__ __launch_bounds__(kBlock) void k_uw_scan_tiles(const uint32_t* tile_c
 template <int N>
 __global__ __launch_bounds__(kBlock) void k_uw_scatter_staged(const UnweightArgs a) {
   constexpr int C = 4 * N;
-  __shared__ double sh[C][kBlock];
-  constexpr uint64_t kGroupsPerTile = kUwTile / kWave;
+  // all columns staged at once: staging half of them (more waves in flight)
+  // made this read-and-write kernel 9% slower at N = 3
+  constexpr int CS = C;
+  __shared__ double sh[CS][kBlock];
   constexpr int kWavesPerBlock = kBlock / kWave;
   const uint64_t ngroups = (a.n + kWave - 1) / kWave;
   const uint64_t nblocks = (ngroups + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -188,47 +190,49 @@ __global__ __launch_bounds__(kBlock) void k_uw_scatter_staged(const UnweightArgs
   const int lane = __lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const uint64_t g0 = blk * kWavesPerBlock;
-  const uint64_t tile = g0 / kGroupsPerTile;
-  uint64_t goff = a.tile_off[tile];
-  for (uint64_t i = tile * kGroupsPerTile; i < g0; ++i) goff += (uint64_t)__popcll(a.flags[i]);
-  uint32_t acc_wave = 0, a_blk = 0;
-#pragma unroll
-  for (int w = 0; w < kWavesPerBlock; ++w) {
-    const uint64_t gw = g0 + (uint64_t)w;
-    const uint32_t c = (gw < ngroups) ? (uint32_t)__popcll(a.flags[gw]) : 0u;
-    acc_wave = (w == wave) ? a_blk : acc_wave;
-    a_blk += c;
-  }
+  const BlockPart bp = block_part(a, g0, wave, ngroups);
+  const uint64_t goff = bp.goff;
+  const uint32_t acc_wave = bp.acc_wave, a_blk = bp.a_blk;
   const uint64_t nacc = *a.n_acc;
   const uint64_t kb = g0 * kWave;
   const uint64_t g = g0 + (uint64_t)wave;
   const uint64_t k0 = g * kWave;
   const uint64_t k = k0 + (uint64_t)lane;
-  if (g < ngroups && k < a.n) {
-    const uint64_t fw = a.flags[g];
+  const bool live_ev = g < ngroups && k < a.n;
+  double v[C];
+  uint32_t r = 0;  // the event's rank in the block's partition order
+  if (live_ev) {
+    const uint64_t fw = bp.fw;
     const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
     const uint32_t acc_below = acc_wave + (uint32_t)__popcll(fw & below);
     const bool kept = (fw >> lane) & 1ULL;
     const uint32_t t = (uint32_t)(wave * kWave + lane);
-    const uint32_t r = kept ? acc_below : a_blk + (t - acc_below);
+    r = kept ? acc_below : a_blk + (t - acc_below);
     const uint64_t dst = (r < a_blk) ? goff + r : nacc + (kb - goff) + (r - a_blk);
     if (a.cols_out) {
-      double v[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) v[c] = *(glob_u(a.cols_in + (uint64_t)c * a.ld_in + k0) + lane);
-#pragma unroll
-      for (int c = 0; c < C; ++c) sh[c][r] = v[c];
     }
     if (a.w_out) *(glob(a.w_out) + dst) = *(glob(a.w_in) + k);
     if (a.idx_out) *(glob(a.idx_out) + dst) = k;
   }
-  __syncthreads();
+  if (!a.cols_out) return;  // kernel-uniform
   const uint64_t live = (a.n - kb) < (uint64_t)kBlock ? (a.n - kb) : (uint64_t)kBlock;
   const uint32_t t = threadIdx.x;
-  if (a.cols_out && t < live) {
-    const uint64_t dst = (t < a_blk) ? goff + t : nacc + (kb - goff) + (t - a_blk);
+  const uint64_t dst = (t < a_blk) ? goff + t : nacc + (kb - goff) + (t - a_blk);
 #pragma unroll
-    for (int c = 0; c < C; ++c) *(glob_u(a.cols_out + (uint64_t)c * a.ld_out) + dst) = sh[c][t];
+  for (int h = 0; h < C; h += CS) {
+    if (h) __syncthreads();  // the previous columns have been read
+    if (live_ev) {
+#pragma unroll
+      for (int c = h; c < h + CS; ++c) sh[c - h][r] = v[c];
+    }
+    __syncthreads();
+    if (t < live) {
+#pragma unroll
+      for (int c = h; c < h + CS; ++c)
+        *(glob_u(a.cols_out + (uint64_t)c * a.ld_out) + dst) = sh[c - h][t];
+    }
   }
 }
 
