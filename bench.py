@@ -51,6 +51,14 @@ def genbod_flops(N):
     return 3 * (3 * N - 4) + 3 * N + 15 * (N - 1) + 4 + s1 + s2 + 5 * N
 
 
+def weight_flops(N):
+    """Algorithmic FP64 operations of PhaseSpaceWeight::Evaluate for one event
+    (Decays.inl:215-231): per running sum, R + p_i (4), two masses (4 mul,
+    3 sub, sqrt each; Vector4R.inl:196-208), pdk (Decays.inl:245-248: 8
+    add/sub, 3 mul, sqrt, 2*a, div), w *= and R += p_i (5): 39."""
+    return 39 * (N - 1)
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -733,23 +741,27 @@ def main():
         print("# c5 done", file=sys.stderr, flush=True)
     extras["c2"] = c2
 
-    # the larger kernel of the step: the regenerating scatter or the weights
-    # pass of Generate (which stores 8 B per event and is bound by its FP64
-    # arithmetic, not by HBM)
-    if c2["regen_scatter_ms"] >= c2["generate_weights_ms"]:
-        dom = ("k_uw_regen<3, 0>", c2["regen_scatter_GBs"], c2["regen_scatter_bytes_per_launch"],
-               "regen")
-    else:
-        dom = ("k_generate_f64<3, 0, false> (weights only)", c2["generate_weights_GBs"], 8 * args.n,
-               "generate_weights")
+    # The step has two kernels of about equal time: the regenerating scatter,
+    # which writes 12.9 of the step's 14.0 GB and is bound by HBM (the
+    # roofline), and the weights pass of Generate, which stores 8 B per event
+    # and is bound by the issue of its FP64 arithmetic (reported beside it
+    # against the FP64 peak, in algorithmic operations).
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         with open(tf) as fh:
-            traffic = json.load(fh).get(dom[3], {}).get("bytes_per_launch")
-    roof = {"bound": "hbm", "achieved": dom[1], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": dom[1] / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom[0],
-            "algorithmic_bytes_per_launch": dom[2]}
+            traffic = json.load(fh).get("regen", {}).get("bytes_per_launch")
+    w_flops = genbod_flops(3) + weight_flops(3)
+    w_tflops = args.n * w_flops / (c2["generate_weights_ms"] * 1e9)
+    roof = {"bound": "hbm", "achieved": c2["regen_scatter_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": c2["regen_scatter_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "k_uw_regen<3, 0>", "ms": c2["regen_scatter_ms"],
+            "algorithmic_bytes_per_launch": c2["regen_scatter_bytes_per_launch"],
+            "weights_pass": {"kernel": "k_generate_f64<3, 0, false> (no columns)",
+                             "bound": "fp64 issue", "ms": c2["generate_weights_ms"],
+                             "flops_per_event": w_flops, "achieved": w_tflops,
+                             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": w_tflops / FP64_PEAK_TFLOPS}}
     cpu = None
     if args.cpu_baseline:
         cpu = cpu_baseline()
