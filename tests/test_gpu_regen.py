@@ -30,7 +30,7 @@ def same(dec, gen_seed, first, n, uw_seed):
     return k1
 
 
-@pytest.mark.parametrize("N", [2, 3, 4, 7, 12])
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 6, 7, 12])
 @pytest.mark.parametrize("engine", list(ENGINES))
 def test_regen_equals_move(N, engine):
     rng = np.random.default_rng(300 + 13 * N + ENGINES[engine])
@@ -50,6 +50,17 @@ def test_regen_moving_mother_and_sizes():
     dec = make_decay(5.279, m, mother4=[6.0, 0.3, -0.2, 0.5])
     for n in (1, 63, 64, 65, 1023, 1024, 1025, 4096 + 17):
         same(dec, 159753, 7, n, UW_SEED)
+
+
+@pytest.mark.parametrize("N", [2, 6, 7])
+def test_regen_ragged_sizes(N):
+    """Partial blocks and tiles at the end, for one staging round (N = 2),
+    two (N = 6) and none (N = 7, stored from registers)."""
+    rng = np.random.default_rng(40 + N)
+    m = list(rng.uniform(0.1, 0.5, N))
+    dec = make_decay(sum(m) * 2.0 + 1.0, m)
+    for n in (1, 255, 257, 1000, 1025, 3 * 1024 + 200):
+        same(dec, 11 + n, 3, n, UW_SEED + n)
 
 
 def test_regen_headline_size_and_outputs():
