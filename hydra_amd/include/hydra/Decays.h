@@ -377,6 +377,10 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     fLazy = true;
   }
   bool lazy() const { return fLazy; }
+  // True once a read has had to form deferred momenta: this container is
+  // read after Generate, so PhaseSpace::Generate writes its momenta with the
+  // weights in one pass from then on instead of deferring them.
+  bool read_after_generate() const { return fReadAfterGen; }
 
   // The events are 0 .. size() - 1 of one FP64 PhaseSpace::Generate(mother,
   // ...) with decay d and seed, untouched since (every non-const access to
@@ -419,6 +423,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
                                        0, nullptr, nullptr, nullptr),
                          "Decays (deferred Generate)");
       detail::hpsp_sync("Decays (deferred Generate)");
+      fReadAfterGen = true;
     }
     fLazy = false;
   }
@@ -540,6 +545,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   uint64_t fGenSeed = 0;
   bool fGenValid = false;
   mutable bool fLazy = false;
+  mutable bool fReadAfterGen = false;
 };
 
 // Decays in host memory (the reference's Decays<tuple<P...>, host::sys_t>):

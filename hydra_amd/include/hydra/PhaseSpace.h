@@ -151,7 +151,7 @@ class PhaseSpace {
     const bool same = same_decay(events.GetMotherMass(), events.GetMasses()) &&
                       events.GetMaxWeight() == fMaxWeight;
     double* cols = events.columns_for_overwrite();
-    if (same && events.size() > 0) {
+    if (same && events.size() > 0 && !events.read_after_generate()) {
       // the weights now, the momenta when first read or at their places in
       // Unweight() (the container keeps (decay, seed) to form them)
       detail::hpsp_check(hpsp_generate(&d, fSeed, 0, events.size(), HPSP_FP64, nullptr,
@@ -161,6 +161,16 @@ class PhaseSpace {
       detail::hpsp_sync("PhaseSpace::Generate");
       events.set_weights_cache_valid(true);
       events.set_generated_lazy(d, (uint64_t)fSeed);
+    } else if (same && events.size() > 0) {
+      // a container that is read after Generate: momenta and weights in one
+      // pass (deferring would form the momenta twice); Unweight() still
+      // regenerates them at their places instead of moving them
+      detail::hpsp_check(hpsp_generate(&d, fSeed, 0, events.size(), HPSP_FP64, cols, events.ld(),
+                                       0, nullptr, events.weights_cache_for_write(), nullptr),
+                         "PhaseSpace::Generate");
+      detail::hpsp_sync("PhaseSpace::Generate");
+      events.set_weights_cache_valid(true);
+      events.set_generated(d, (uint64_t)fSeed);
     } else {
       detail::hpsp_check(hpsp_generate(&d, fSeed, 0, events.size(), HPSP_FP64, cols, events.ld(),
                                        0, nullptr, nullptr, nullptr),

@@ -178,6 +178,35 @@ int main(int argc, char** argv) {
         }
     std::printf("\"unweight_paths_same\": %d,\n\"gen_flags\": [%d, %d],\n", paths_same, gen_before,
                 gen_after);
+    // Events was read after its deferred Generate: the next Generate into it
+    // writes momenta and weights in one pass, with the same bits, and its
+    // Unweight() still regenerates (the result of E2's above)
+    const int read_after = Events.read_after_generate() ? 1 : 0;
+    phsp.Generate(Parent, Events);
+    const int lazy_again = Events.lazy() ? 1 : 0;
+    const int gen_again = Events.generated() ? 1 : 0;
+    hydra::Decays<hydra::tuple<A, B, C>, hydra::host::sys_t> h_again(Events);
+    int eager_same = 1;
+    for (size_t c = 0; c < 12 && eager_same; ++c)
+      for (size_t i = 0; i < n; ++i)
+        if (std::memcmp(h_again.data() + c * h_again.ld() + i, h_ev.data() + c * h_ev.ld() + i,
+                        sizeof(double)) != 0) {
+          eager_same = 0;
+          break;
+        }
+    auto E7 = Events;
+    auto acc7 = E7.Unweight();
+    hydra::Decays<hydra::tuple<A, B, C>, hydra::host::sys_t> h7(E7);
+    int unweight7_same = (acc7.size() == accepted.size()) ? 1 : 0;
+    for (size_t c = 0; c < 12 && unweight7_same; ++c)
+      for (size_t i = 0; i < h2.size(); ++i)
+        if (std::memcmp(h2.data() + c * h2.ld() + i, h7.data() + c * h7.ld() + i,
+                        sizeof(double)) != 0) {
+          unweight7_same = 0;
+          break;
+        }
+    std::printf("\"eager_after_read\": [%d, %d, %d, %d, %d],\n", read_after, lazy_again, gen_again,
+                eager_same, unweight7_same);
     auto E3 = Events;
     auto acc_f = E3.Unweight(hydra::InvariantMass2{}, -1.0, 0x39abdc4529b1661cULL);
     std::printf("\"unweight_f_nacc\": %zu,\n", acc_f.size());

@@ -71,6 +71,9 @@ def test_gpu_generate_unweight_average(golden, oracle):
     # Generate writes the weights and defers the momenta to their first read
     assert r["lazy_after_generate"] == 1 and r["lazy_after_read"] == 0
     assert r["lazy_same"] == 1
+    # once read after a deferred Generate, a container is generated eagerly
+    # (momenta and weights in one pass): same bits, Unweight() still regenerates
+    assert r["eager_after_read"] == [1, 0, 1, 1, 1]
     for key, gkey in (("avg_bw3", "avg_c1_bw3"), ("integrate_m2", "avg_c1_m2")):
         mean, sig = r[key]
         gm, gs = golden[gkey]
