@@ -137,6 +137,18 @@ struct EventWords {
       r123_words<ENGINE, W>(d.key, wave_first_evt + (uint64_t)lane + 3 * (uint64_t)N, w);
     }
   }
+  // The next group: wave_first_evt is its first event, 64 after this
+  // group's.  With the squares engines its first 64 words are this group's
+  // second ones, so each lane evaluates one new word instead of two.
+  __device__ __forceinline__ void advance(const DecayK& d, uint64_t wave_first_evt, int lane) {
+    if constexpr (is_squares<ENGINE>()) {
+      ww.lo = ww.hi;
+      ww.hi = squares_word<ENGINE>(d.key[0],
+                                   wave_first_evt + 3 * (uint64_t)N + kWave + (uint64_t)lane);
+    } else {
+      init(d, wave_first_evt, lane);
+    }
+  }
   __device__ __forceinline__ uint64_t operator()(int j) const {
     if constexpr (is_squares<ENGINE>())
       return ww.get(j);

@@ -11,6 +11,10 @@
 
 namespace hpsp {
 
+// Each wave takes runs of a.run consecutive groups of 64 events (grid-stride
+// over runs): with the squares engines a group's second 64 words are the next
+// group's first, so within a run each lane evaluates one word per group
+// instead of two.
 template <int N, int ENGINE, bool MOTHERS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
 void k_generate_f64(const GenArgs a) {
@@ -18,53 +22,64 @@ void k_generate_f64(const GenArgs a) {
   // wave index as a scalar so that column bases stay in SGPRs and every
   // store is (scalar base) + lane * 8
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < a.n; base += stride) {
-    const uint64_t k0 = base + (uint64_t)wave * kWave;
-    const uint64_t k = k0 + (uint64_t)lane;
-    const bool live = k < a.n;
-
+  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
+  const uint64_t run_len = a.run;
+  const uint64_t nruns = (ngroups + run_len - 1) / run_len;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t run = (uint64_t)blockIdx.x * (kBlock / kWave) + (uint64_t)wave; run < nruns;
+       run += nwaves) {
+    const uint64_t g_first = run * run_len;
+    const uint64_t g_end = (g_first + run_len < ngroups) ? g_first + run_len : ngroups;
     EventWords<N, ENGINE> words;
-    words.init(a.d, a.first + k0, lane);
-    double p[N][4];
-    const double wt = genbod_f64<N, ENGINE>(a.d, words, p);
+    for (uint64_t g = g_first; g < g_end; ++g) {
+      const uint64_t k0 = g * kWave;
+      const uint64_t k = k0 + (uint64_t)lane;
+      const bool live = k < a.n;
 
-    if constexpr (MOTHERS) {
-      // applyBoostTo(mother) with beta = p/E (DecayMothers.h:228-233).
-      const uint64_t km = live ? k : 0;
-      const double e = a.mothers[km];
-      const double bx = a.mothers[a.mld + km] / e;
-      const double by = a.mothers[2 * a.mld + km] / e;
-      const double bz = a.mothers[3 * a.mld + km] / e;
-#pragma unroll
-      for (int i = 0; i < N; ++i) boost_general(p[i], bx, by, bz);
-    } else {
-      if (a.d.do_boost) {
-#pragma unroll
-        for (int i = 0; i < N; ++i) boost_folded(p[i], a.d);
-      }
-    }
+      if (g == g_first)
+        words.init(a.d, a.first + k0, lane);
+      else
+        words.advance(a.d, a.first + k0, lane);
+      double p[N][4];
+      const double wt = genbod_f64<N, ENGINE>(a.d, words, p);
 
-    if (live) {
-      if (!a.cols) {
-        // weights only: the momenta are formed for them and not stored
-      } else if (a.out_f32) {
-        float* c = static_cast<float*>(a.cols) + k0;
+      if constexpr (MOTHERS) {
+        // applyBoostTo(mother) with beta = p/E (DecayMothers.h:228-233).
+        const uint64_t km = live ? k : 0;
+        const double e = a.mothers[km];
+        const double bx = a.mothers[a.mld + km] / e;
+        const double by = a.mothers[2 * a.mld + km] / e;
+        const double bz = a.mothers[3 * a.mld + km] / e;
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            st(uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane, (float)p[i][q]);
+        for (int i = 0; i < N; ++i) boost_general(p[i], bx, by, bz);
       } else {
-        double* c = static_cast<double*>(a.cols) + k0;
+        if (a.d.do_boost) {
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            st(uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane, p[i][q]);
+          for (int i = 0; i < N; ++i) boost_folded(p[i], a.d);
+        }
       }
-      if (a.wt) st(uniform_ptr(a.wt + k0) + lane, wt);
-      if (a.evw) st(uniform_ptr(a.evw + k0) + lane, event_weight_f64<N>(p, a.d.m, a.d.maxw));
+
+      if (live) {
+        if (!a.cols) {
+          // weights only: the momenta are formed for them and not stored
+        } else if (a.out_f32) {
+          float* c = static_cast<float*>(a.cols) + k0;
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              st(uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane, (float)p[i][q]);
+        } else {
+          double* c = static_cast<double*>(a.cols) + k0;
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              st(uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane, p[i][q]);
+        }
+        if (a.wt) st(uniform_ptr(a.wt + k0) + lane, wt);
+        if (a.evw) st(uniform_ptr(a.evw + k0) + lane, event_weight_f64<N>(p, a.d.m, a.d.maxw));
+      }
     }
   }
 }
@@ -307,8 +322,15 @@ constexpr int kNumN = HPSP_MAX_N - 1;
 constexpr int kNumE = 6;
 
 template <int N, int E>
-static void gen_one(const GenArgs& a, hipStream_t s) {
-  const unsigned g = grid_for(a.n, 1u << 20);
+static void gen_one(const GenArgs& in, hipStream_t s) {
+  GenArgs a = in;
+  // runs of 4 groups once there are groups for 8 waves per SIMD even so;
+  // smaller launches keep one group per wave (latency)
+  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
+  a.run = ngroups >= kRunMinGroups ? kRunGroups : 1u;
+  const uint64_t nruns = (ngroups + a.run - 1) / a.run;
+  const uint64_t want = (nruns + kBlock / kWave - 1) / (kBlock / kWave);
+  const unsigned g = (unsigned)(want == 0 ? 1 : (want < (1u << 20) ? want : (1u << 20)));
   if (a.mothers)
     hipLaunchKernelGGL((k_generate_f64<N, E, true>), dim3(g), dim3(kBlock), 0, s, a);
   else

@@ -178,6 +178,17 @@ struct EventWordsM {
       r123_words<ENGINE, W>(d.key, wave_first_evt + (uint64_t)lane + 3 * (uint64_t)N, w);
     }
   }
+  // The next group (first event wave_first_evt): with the squares engines its
+  // first 64 words are this group's second ones.
+  __device__ __forceinline__ void advance(const DecayK& d, uint64_t wave_first_evt, int lane) {
+    if constexpr (is_squares<ENGINE>()) {
+      ww.lo = ww.hi;
+      ww.hi = squares_word<ENGINE>(d.key[0],
+                                   wave_first_evt + 3 * (uint64_t)N + kWave + (uint64_t)lane);
+    } else {
+      init(d, wave_first_evt, lane);
+    }
+  }
   __device__ __forceinline__ uint64_t operator()(int j) const {
     if constexpr (is_squares<ENGINE>())
       return ww.get(j);
@@ -194,58 +205,72 @@ __device__ __forceinline__ void final_boost_f32(float (&p)[N][4], const DecayK& 
   }
 }
 
+// Runs of a.run consecutive groups of 64 events per wave, as in
+// k_generate_f64 (one squares word per lane and group within a run).
 template <int N, int ENGINE, bool MOTHERS>
 __global__ __launch_bounds__(kBlock) void k_generate_mixed(const GenArgs a) {
   const int lane = __lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < a.n; base += stride) {
-    const uint64_t k0 = base + (uint64_t)wave * kWave;
-    const uint64_t k = k0 + (uint64_t)lane;
-    const bool live = k < a.n;
+  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
+  const uint64_t run_len = a.run;
+  const uint64_t nruns = (ngroups + run_len - 1) / run_len;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t run = (uint64_t)blockIdx.x * (kBlock / kWave) + (uint64_t)wave; run < nruns;
+       run += nwaves) {
+    const uint64_t g_first = run * run_len;
+    const uint64_t g_end = (g_first + run_len < ngroups) ? g_first + run_len : ngroups;
     EventWordsM<N, ENGINE> words;
-    words.init(a.d, a.first + k0, lane);
-    float p[N][4];
-    const double wt = genbod_mixed<N, ENGINE>(a.d, words, p);
-    if constexpr (MOTHERS) {
-      const uint64_t km = live ? k : 0;
-      const float ie = __builtin_amdgcn_rcpf((float)a.mothers[km]);
-      const float bx = (float)a.mothers[a.mld + km] * ie;
-      const float by = (float)a.mothers[2 * a.mld + km] * ie;
-      const float bz = (float)a.mothers[3 * a.mld + km] * ie;
+    for (uint64_t g = g_first; g < g_end; ++g) {
+      const uint64_t k0 = g * kWave;
+      const uint64_t k = k0 + (uint64_t)lane;
+      const bool live = k < a.n;
+      if (g == g_first)
+        words.init(a.d, a.first + k0, lane);
+      else
+        words.advance(a.d, a.first + k0, lane);
+      float p[N][4];
+      const double wt = genbod_mixed<N, ENGINE>(a.d, words, p);
+      if constexpr (MOTHERS) {
+        const uint64_t km = live ? k : 0;
+        const float ie = __builtin_amdgcn_rcpf((float)a.mothers[km]);
+        const float bx = (float)a.mothers[a.mld + km] * ie;
+        const float by = (float)a.mothers[2 * a.mld + km] * ie;
+        const float bz = (float)a.mothers[3 * a.mld + km] * ie;
 #pragma unroll
-      for (int i = 0; i < N; ++i) boost_general_f32(p[i], bx, by, bz);
-    } else {
-      final_boost_f32<N>(p, a.d);
-    }
-    if (live) {
-      if (!a.cols) {
-        // weights only
-      } else if (a.out_f32) {
-        float* c = static_cast<float*>(a.cols) + k0;
-#pragma unroll
-        for (int i = 0; i < N; ++i)
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            __builtin_nontemporal_store(p[i][q], uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane);
+        for (int i = 0; i < N; ++i) boost_general_f32(p[i], bx, by, bz);
       } else {
-        double* c = static_cast<double*>(a.cols) + k0;
-#pragma unroll
-        for (int i = 0; i < N; ++i)
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            __builtin_nontemporal_store((double)p[i][q],
-                                        uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane);
+        final_boost_f32<N>(p, a.d);
       }
-      if (a.wt) __builtin_nontemporal_store(wt, uniform_ptr(a.wt + k0) + lane);
-      if (a.evw) {
-        double pp[N][4];
+      if (live) {
+        if (!a.cols) {
+          // weights only
+        } else if (a.out_f32) {
+          float* c = static_cast<float*>(a.cols) + k0;
 #pragma unroll
-        for (int i = 0; i < N; ++i)
+          for (int i = 0; i < N; ++i)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) pp[i][q] = (double)p[i][q];
-        __builtin_nontemporal_store(event_weight_f64<N>(pp, a.d.m, a.d.maxw),
-                                    uniform_ptr(a.evw + k0) + lane);
+            for (int q = 0; q < 4; ++q)
+              __builtin_nontemporal_store(p[i][q],
+                                          uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane);
+        } else {
+          double* c = static_cast<double*>(a.cols) + k0;
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              __builtin_nontemporal_store((double)p[i][q],
+                                          uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane);
+        }
+        if (a.wt) __builtin_nontemporal_store(wt, uniform_ptr(a.wt + k0) + lane);
+        if (a.evw) {
+          double pp[N][4];
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) pp[i][q] = (double)p[i][q];
+          __builtin_nontemporal_store(event_weight_f64<N>(pp, a.d.m, a.d.maxw),
+                                      uniform_ptr(a.evw + k0) + lane);
+        }
       }
     }
   }
@@ -304,8 +329,13 @@ constexpr int kNumN = HPSP_MAX_N - 1;
 constexpr int kNumE = 6;
 
 template <int N, int E>
-static void gen_one(const GenArgs& a, hipStream_t s) {
-  const unsigned g = grid_for(a.n, 1u << 20);
+static void gen_one(const GenArgs& in, hipStream_t s) {
+  GenArgs a = in;
+  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
+  a.run = ngroups >= kRunMinGroups ? kRunGroups : 1u;
+  const uint64_t nruns = (ngroups + a.run - 1) / a.run;
+  const uint64_t want = (nruns + kBlock / kWave - 1) / (kBlock / kWave);
+  const unsigned g = (unsigned)(want == 0 ? 1 : (want < (1u << 20) ? want : (1u << 20)));
   if (a.mothers)
     hipLaunchKernelGGL((k_generate_mixed<N, E, true>), dim3(g), dim3(kBlock), 0, s, a);
   else

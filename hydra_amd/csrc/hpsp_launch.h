@@ -12,6 +12,8 @@ namespace hpsp {
 constexpr int kAvgChunk = 4096;  // events per AverageOn reduction chunk
 constexpr int kUwTile = 1024;    // events per Unweight tile (256 threads x 4)
 constexpr int kXcds = 8;         // MI355X: workgroups are dealt to 8 XCDs in turn
+constexpr uint32_t kRunGroups = 4;         // groups of 64 events per wave in a generation run
+constexpr uint64_t kRunMinGroups = 32768;  // 8 runs per SIMD of 256 CUs x 4 SIMDs
 
 struct GenArgs {
   DecayK d;
@@ -23,6 +25,7 @@ struct GenArgs {
   double* evw;
   const double* mothers;  // FP64 mother columns (E,px,py,pz) or null
   uint64_t mld;
+  uint32_t run;  // consecutive groups of 64 events per wave (>= 1; set by the launcher)
 };
 
 struct AvgArgs {
@@ -44,6 +47,7 @@ struct EvalArgs {
   uint64_t mld;
   double* out;              // 1 + nf columns of ld doubles
   uint64_t ld;
+  uint32_t run;             // consecutive groups of 64 events per wave (set by the launcher)
 };
 
 // Four-vectors of stored events: particle i, component c, event k at
