@@ -122,13 +122,11 @@ constexpr int kNumE = 6;
 template <int N, int E>
 static void eval_one(bool mothers, const EvalArgs& in, hipStream_t s) {
   EvalArgs a = in;
-  // over one mother, runs of 4 groups once there are groups for 8 waves per
-  // SIMD even so (as Generate); over mothers each lane seeds its own engine
+  // runs over one mother (as Generate); over mothers each lane seeds its own
+  // engine, one group per wave
   const uint64_t ngroups = (a.n + kWave - 1) / kWave;
-  a.run = (!mothers && ngroups >= kRunMinGroups) ? kRunGroups : 1u;
-  const uint64_t nruns = (ngroups + a.run - 1) / a.run;
-  const uint64_t want = (nruns + kBlock / kWave - 1) / (kBlock / kWave);
-  const unsigned g = (unsigned)(want == 0 ? 1 : (want < (1u << 20) ? want : (1u << 20)));
+  a.run = mothers ? 1u : run_groups(ngroups);
+  const unsigned g = grid_for_runs((ngroups + a.run - 1) / a.run);
   if (mothers)
     hipLaunchKernelGGL((k_evaluate_f64<N, E, true>), dim3(g), dim3(kBlock), 0, s, a);
   else

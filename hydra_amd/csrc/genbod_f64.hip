@@ -324,13 +324,9 @@ constexpr int kNumE = 6;
 template <int N, int E>
 static void gen_one(const GenArgs& in, hipStream_t s) {
   GenArgs a = in;
-  // runs of 4 groups once there are groups for 8 waves per SIMD even so;
-  // smaller launches keep one group per wave (latency)
   const uint64_t ngroups = (a.n + kWave - 1) / kWave;
-  a.run = ngroups >= kRunMinGroups ? kRunGroups : 1u;
-  const uint64_t nruns = (ngroups + a.run - 1) / a.run;
-  const uint64_t want = (nruns + kBlock / kWave - 1) / (kBlock / kWave);
-  const unsigned g = (unsigned)(want == 0 ? 1 : (want < (1u << 20) ? want : (1u << 20)));
+  a.run = run_groups(ngroups);
+  const unsigned g = grid_for_runs((ngroups + a.run - 1) / a.run);
   if (a.mothers)
     hipLaunchKernelGGL((k_generate_f64<N, E, true>), dim3(g), dim3(kBlock), 0, s, a);
   else

@@ -12,8 +12,8 @@ namespace hpsp {
 constexpr int kAvgChunk = 4096;  // events per AverageOn reduction chunk
 constexpr int kUwTile = 1024;    // events per Unweight tile (256 threads x 4)
 constexpr int kXcds = 8;         // MI355X: workgroups are dealt to 8 XCDs in turn
-constexpr uint32_t kRunGroups = 4;         // groups of 64 events per wave in a generation run
-constexpr uint64_t kRunMinGroups = 32768;  // 8 runs per SIMD of 256 CUs x 4 SIMDs
+constexpr uint64_t kRunMax = 16;        // longest generation run, in groups of 64 events
+constexpr uint64_t kRunMinRuns = 8192;  // 8 runs per SIMD of 256 CUs x 4 SIMDs
 
 struct GenArgs {
   DecayK d;
@@ -200,6 +200,22 @@ int launch_rng_words(int engine, const uint64_t* key, uint64_t discard, uint64_t
                      uint64_t* out, hipStream_t s);
 
 // Blocks for a grid-stride launch over `items` work items of kBlock each.
+// Groups of 64 events per wave in a generation run (k_generate_*,
+// k_evaluate_f64 over one mother): a longer run shares more engine words
+// between consecutive groups, but the runs must still give every SIMD several
+// waves; the longest power of two up to kRunMax that leaves kRunMinRuns runs.
+inline uint32_t run_groups(uint64_t ngroups) {
+  uint64_t r = 1;
+  while (r < kRunMax && ngroups >= 2 * r * kRunMinRuns) r *= 2;
+  return (uint32_t)r;
+}
+
+// Blocks of four waves for nruns runs, one run per wave (grid-stride beyond).
+inline unsigned grid_for_runs(uint64_t nruns) {
+  const uint64_t want = (nruns + kBlock / kWave - 1) / (kBlock / kWave);
+  return (unsigned)(want == 0 ? 1 : (want < (1u << 20) ? want : (1u << 20)));
+}
+
 inline unsigned grid_for(uint64_t items, unsigned cap = 65536u) {
   const uint64_t b = (items + kBlock - 1) / kBlock;
   return (unsigned)(b == 0 ? 1 : (b < cap ? b : cap));
