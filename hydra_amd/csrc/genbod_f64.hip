@@ -183,9 +183,10 @@ void k_uw_regen(const UnweightArgs a) {
 }
 
 // AverageOn: chunks of kAvgChunk consecutive events, one partial per chunk.
-// Within a chunk thread t folds events t, t+256, ... in order; the waves and
-// the four wave results merge in a fixed tree, so a partial depends only on
-// its event range.
+// Within a chunk wave w folds its own run of consecutive groups of 64 events
+// (groups 16w .. 16w+15), lane l event l of each, in order, so the squares
+// words carry from group to group; the four wave results merge in a fixed
+// tree, so a partial depends only on its event range.
 template <int N, int ENGINE, bool MOTHERS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
 void k_average_f64(const AvgArgs a) {
@@ -195,12 +196,17 @@ void k_average_f64(const AvgArgs a) {
   const uint64_t nchunks = (a.n + kAvgChunk - 1) / kAvgChunk;
   for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     Stats acc = {0.0, 0.0, 0.0};
-    for (int it = 0; it < kAvgChunk / kBlock; ++it) {
-      const uint64_t k0 = c * kAvgChunk + (uint64_t)it * kBlock + (uint64_t)wave * kWave;
+    constexpr int kGroupsPerWave = kAvgChunk / kBlock;
+    EventWords<N, ENGINE> words;
+    for (int it = 0; it < kGroupsPerWave; ++it) {
+      const uint64_t k0 =
+          c * kAvgChunk + ((uint64_t)wave * kGroupsPerWave + (uint64_t)it) * kWave;
       const uint64_t k = k0 + (uint64_t)lane;
       const bool live = k < a.n;
-      EventWords<N, ENGINE> words;
-      words.init(a.d, a.first + k0, lane);
+      if (it == 0)
+        words.init(a.d, a.first + k0, lane);
+      else
+        words.advance(a.d, a.first + k0, lane);
       double p[N + (MOTHERS ? 1 : 0)][4];
       double f, wt;
       if constexpr (MOTHERS) {

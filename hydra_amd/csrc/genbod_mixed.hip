@@ -284,12 +284,19 @@ __global__ __launch_bounds__(kBlock) void k_average_mixed(const AvgArgs a) {
   const uint64_t nchunks = (a.n + kAvgChunk - 1) / kAvgChunk;
   for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     Stats acc = {0.0, 0.0, 0.0};
-    for (int it = 0; it < kAvgChunk / kBlock; ++it) {
-      const uint64_t k0 = c * kAvgChunk + (uint64_t)it * kBlock + (uint64_t)wave * kWave;
+    // wave w folds groups 16w .. 16w+15 of the chunk in order (as the FP64
+    // kernel), carrying the squares words from group to group
+    constexpr int kGroupsPerWave = kAvgChunk / kBlock;
+    EventWordsM<N, ENGINE> words;
+    for (int it = 0; it < kGroupsPerWave; ++it) {
+      const uint64_t k0 =
+          c * kAvgChunk + ((uint64_t)wave * kGroupsPerWave + (uint64_t)it) * kWave;
       const uint64_t k = k0 + (uint64_t)lane;
       const bool live = k < a.n;
-      EventWordsM<N, ENGINE> words;
-      words.init(a.d, a.first + k0, lane);
+      if (it == 0)
+        words.init(a.d, a.first + k0, lane);
+      else
+        words.advance(a.d, a.first + k0, lane);
       float p[N + (MOTHERS ? 1 : 0)][4];
       double wt;
       if constexpr (MOTHERS) {
