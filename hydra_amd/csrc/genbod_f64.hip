@@ -23,7 +23,7 @@ void k_generate_f64(const GenArgs a) {
   // store is (scalar base) + lane * 8
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const uint64_t ngroups = (a.n + kWave - 1) / kWave;
-  const uint64_t run_len = a.run;
+  const uint64_t run_len = a.run ? a.run : 1;  // the launcher sets it; 0 reads as 1
   const uint64_t nruns = (ngroups + run_len - 1) / run_len;
   const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
   for (uint64_t run = (uint64_t)blockIdx.x * (kBlock / kWave) + (uint64_t)wave; run < nruns;
