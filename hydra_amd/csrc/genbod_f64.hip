@@ -331,8 +331,9 @@ template <int N, int E>
 static void gen_one(const GenArgs& in, hipStream_t s) {
   GenArgs a = in;
   const uint64_t ngroups = (a.n + kWave - 1) / kWave;
-  // momentum columns without PhaseSpaceWeight: bound by the stores
-  a.run = run_groups(ngroups, (a.cols && !a.evw) ? kRunMaxStores : kRunMax);
+  // momentum columns of at most 3 daughters without PhaseSpaceWeight: bound
+  // by the stores (more daughters bring arithmetic enough to gain from 16)
+  a.run = run_groups(ngroups, (a.cols && !a.evw && N <= 3) ? kRunMaxStores : kRunMax);
   const unsigned g = grid_for_runs((ngroups + a.run - 1) / a.run);
   if (a.mothers)
     hipLaunchKernelGGL((k_generate_f64<N, E, true>), dim3(g), dim3(kBlock), 0, s, a);

@@ -4,7 +4,8 @@ launches take one group per wave.  Events are a function of (decay, seed,
 index) alone, so one large launch and the same range generated in small
 pieces (one group per wave) must agree bit for bit, including a partial last
 run, every engine family and each output form (momenta and weights: runs up
-to 16 groups; momenta alone: up to 4; FP32 columns of the mixed form).
+to 16 groups; momenta of at most 3 daughters alone: up to 4; FP32 columns of
+the mixed form).
 """
 import numpy as np
 import pytest
@@ -35,9 +36,10 @@ def pieces_match(fulls, piece_fn, n):
 
 
 CASES = [
-    # (N, engine, n, want_evw, prec, out_f32): runs of 16, 4, 4, 4, 4 groups
+    # (N, engine, n, want_evw, prec, out_f32): runs of 16, 4, 16, 4, 4, 4 groups
     (3, "squares3", (1 << 23) + 4321, True, capi.FP64, False),
     (3, "squares3", (1 << 21) + 12345, False, capi.FP64, False),
+    (7, "squares3", (1 << 23) + 4321, False, capi.FP64, False),
     (3, "philox", (1 << 21) + 12345, True, capi.FP64, False),
     (7, "squares4", (1 << 21) + 12345, True, capi.FP64, False),
     (3, "squares3", (1 << 21) + 12345, True, capi.MIXED, True),
