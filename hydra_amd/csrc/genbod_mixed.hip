@@ -341,7 +341,9 @@ static void gen_one(const GenArgs& in, hipStream_t s) {
   const uint64_t ngroups = (a.n + kWave - 1) / kWave;
   // momentum columns of at most 3 daughters without PhaseSpaceWeight: bound
   // by the stores (more daughters bring arithmetic enough to gain from 16)
-  a.run = run_groups(ngroups, (a.cols && !a.evw && N <= 3) ? kRunMaxStores : kRunMax);
+  uint64_t cap = (a.cols && !a.evw && N <= 3) ? kRunMaxStores : kRunMax;
+  if (a.run_cap && a.run_cap < cap) cap = a.run_cap;
+  a.run = run_groups(ngroups, cap);
   const unsigned g = grid_for_runs((ngroups + a.run - 1) / a.run);
   if (a.mothers)
     hipLaunchKernelGGL((k_generate_mixed<N, E, true>), dim3(g), dim3(kBlock), 0, s, a);

@@ -26,7 +26,8 @@ struct GenArgs {
   double* evw;
   const double* mothers;  // FP64 mother columns (E,px,py,pz) or null
   uint64_t mld;
-  uint32_t run;  // consecutive groups of 64 events per wave (>= 1; set by the launcher)
+  uint32_t run;      // consecutive groups of 64 events per wave (>= 1; set by the launcher)
+  uint32_t run_cap;  // 0: the launcher's policy; else the longest run allowed
 };
 
 struct AvgArgs {
