@@ -69,8 +69,9 @@ __device__ __forceinline__ gptr<T> glob_u(T* p) {
 __device__ __forceinline__ uint64_t rot32(uint64_t x) { return (x >> 32) | (x << 32); }
 
 // squares3 word for counter ctr (squares3.h:105-119)
-__device__ __forceinline__ uint32_t squares3(uint64_t key, uint64_t ctr) {
-  uint64_t y = key * ctr;
+// The same from y = key * ctr, for callers that step the counter by a fixed
+// amount and y with it (an add instead of a 64-bit multiply).
+__device__ __forceinline__ uint32_t squares3_y(uint64_t key, uint64_t y) {
   uint64_t x = y;
   uint64_t z = y + key;
   x = x * x + y;
@@ -78,6 +79,9 @@ __device__ __forceinline__ uint32_t squares3(uint64_t key, uint64_t ctr) {
   x = x * x + z;
   x = rot32(x);
   return (uint32_t)((x * x + y) >> 32);
+}
+__device__ __forceinline__ uint32_t squares3(uint64_t key, uint64_t ctr) {
+  return squares3_y(key, key * ctr);
 }
 
 // squares4 word (squares4.h:106-122)

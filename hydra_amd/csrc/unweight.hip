@@ -59,8 +59,12 @@ __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
     }
   }
   uint32_t c = 0;
+  // key * (jump + k) of the wave's first group; group j + 1 is kBlock events
+  // on, so the product steps by key * kBlock (an add, not a 64-bit multiply)
+  uint64_t ykey = a.uw_key * (a.jump + (tile * (kUwTile / kWave) + wave) * kWave + lane);
+  const uint64_t ystep = a.uw_key * (uint64_t)kBlock;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
+  for (int j = 0; j < kItems; ++j, ykey += ystep) {
     const uint64_t g = tile * (kUwTile / kWave) + (uint64_t)j * (kBlock / kWave) + wave;
     const uint64_t k = g * kWave + (uint64_t)lane;
     bool acc = false;
@@ -74,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
         w = a.w_in ? a.w_in[k] : event_weight_f64<N>(p, a.d.m, a.d.maxw);
         if (a.use_f) w = w * functor_eval<double>(a.f, p);
       }
-      const double u = (double)squares3(a.uw_key, a.jump + k) * 0x1p-32;
+      const double u = (double)squares3_y(a.uw_key, ykey) * 0x1p-32;  // ctr = jump + k
       // Decays::Unweight (FlagDaugthers, Decays.inl:75-89): w / max > u;
       // hydra::unweight (RndFlag, RandomUtils.h:254-262): u * max <= w
       acc = a.rule == 0 ? (w / maxv) > u : (u * maxv) <= w;
