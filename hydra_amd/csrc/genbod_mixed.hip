@@ -167,13 +167,18 @@ template <int N, int ENGINE>
 struct EventWordsM {
   static constexpr int W = 3 * N - 4;
   std::conditional_t<is_squares<ENGINE>(), WaveWords, char> ww;
+  uint64_t y_hi;  // squares: key * counter of this lane's second word
   uint64_t w[is_squares<ENGINE>() ? 1 : W];
   __device__ __forceinline__ void init(const DecayK& d, uint64_t wave_first_evt, int lane) {
     if constexpr (is_squares<ENGINE>()) {
+      // key * counter of this lane's two words, 64 counters apart: the
+      // second product is the first plus key * 64
       const uint64_t c0 = wave_first_evt + 3 * (uint64_t)N;
       ww.lane = lane;
-      ww.lo = squares_word<ENGINE>(d.key[0], c0 + (uint64_t)lane);
-      ww.hi = squares_word<ENGINE>(d.key[0], c0 + kWave + (uint64_t)lane);
+      const uint64_t y_lo = d.key[0] * (c0 + (uint64_t)lane);
+      y_hi = y_lo + (d.key[0] << 6);
+      ww.lo = squares_word_y<ENGINE>(d.key[0], y_lo);
+      ww.hi = squares_word_y<ENGINE>(d.key[0], y_hi);
     } else {
       r123_words<ENGINE, W>(d.key, wave_first_evt + (uint64_t)lane + 3 * (uint64_t)N, w);
     }
@@ -183,8 +188,8 @@ struct EventWordsM {
   __device__ __forceinline__ void advance(const DecayK& d, uint64_t wave_first_evt, int lane) {
     if constexpr (is_squares<ENGINE>()) {
       ww.lo = ww.hi;
-      ww.hi = squares_word<ENGINE>(d.key[0],
-                                   wave_first_evt + 3 * (uint64_t)N + kWave + (uint64_t)lane);
+      y_hi += d.key[0] << 6;  // counter + 64
+      ww.hi = squares_word_y<ENGINE>(d.key[0], y_hi);
     } else {
       init(d, wave_first_evt, lane);
     }

@@ -85,8 +85,7 @@ __device__ __forceinline__ uint32_t squares3(uint64_t key, uint64_t ctr) {
 }
 
 // squares4 word (squares4.h:106-122)
-__device__ __forceinline__ uint32_t squares4(uint64_t key, uint64_t ctr) {
-  uint64_t y = key * ctr;
+__device__ __forceinline__ uint32_t squares4_y(uint64_t key, uint64_t y) {
   uint64_t x = y;
   uint64_t z = y + key;
   x = x * x + y;
@@ -96,6 +95,9 @@ __device__ __forceinline__ uint32_t squares4(uint64_t key, uint64_t ctr) {
   x = x * x + y;
   x = rot32(x);
   return (uint32_t)((x * x + z) >> 32);
+}
+__device__ __forceinline__ uint32_t squares4(uint64_t key, uint64_t ctr) {
+  return squares4_y(key, key * ctr);
 }
 
 __device__ __forceinline__ uint64_t rotl64(uint64_t x, unsigned n) {
@@ -240,6 +242,14 @@ __device__ __forceinline__ uint32_t squares_word(uint64_t key, uint64_t ctr) {
     return squares3(key, ctr);
   else
     return squares4(key, ctr);
+}
+// The same from y = key * ctr.
+template <int ENGINE>
+__device__ __forceinline__ uint32_t squares_word_y(uint64_t key, uint64_t y) {
+  if constexpr (ENGINE == HPSP_SQUARES3)
+    return squares3_y(key, y);
+  else
+    return squares4_y(key, y);
 }
 
 // SplitMix64 step (splitmix.h:54-61).
