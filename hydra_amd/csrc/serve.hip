@@ -56,10 +56,18 @@ constexpr int kStatSlots = 8192;
 
 size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Bytes x rounded up to a multiple of a; false if that does not fit size_t.
+bool round_up_checked(size_t x, size_t a, size_t* out) {
+  size_t t;
+  if (__builtin_add_overflow(x, a - 1, &t)) return false;
+  *out = t / a * a;
+  return true;
+}
+
 class Arena {
  public:
   int init(size_t bytes) {
-    size_ = round_up(bytes, kArenaAlign);
+    if (!round_up_checked(bytes, kArenaAlign, &size_)) return HPSP_ERR_INVALID_ARGUMENT;
     void* p = nullptr;
     if (hipMalloc(&p, size_) != hipSuccess) return HPSP_ERR_HIP;
     base_ = static_cast<char*>(p);
@@ -143,15 +151,31 @@ struct Batch {
   hipEvent_t done = nullptr;
 };
 
-size_t footprint(const hpsp_request& r) {
+// Arena bytes of a request: its columns (+ weights) for GENERATE, its
+// partials and result for AVERAGE.  False when the size does not fit size_t
+// (a huge or wrapped-around event count), which the caller rejects.
+bool footprint(const hpsp_request& r, size_t* out) {
   if (r.kind == HPSP_REQ_GENERATE) {
     const size_t esz = r.out_f32 ? 4 : 8;
-    size_t b = round_up(r.n * 4 * (size_t)r.decay.n * esz, kArenaAlign);
-    if (r.want_weights) b += round_up(r.n * 8, kArenaAlign);
-    return b;
+    size_t cols, b;
+    if (__builtin_mul_overflow((size_t)r.n, 4 * (size_t)r.decay.n * esz, &cols)) return false;
+    if (!round_up_checked(cols, kArenaAlign, &b)) return false;
+    if (r.want_weights) {
+      size_t w, wr;
+      if (__builtin_mul_overflow((size_t)r.n, (size_t)8, &w)) return false;
+      if (!round_up_checked(w, kArenaAlign, &wr)) return false;
+      if (__builtin_add_overflow(b, wr, &b)) return false;
+    }
+    *out = b;
+    return true;
   }
-  const size_t m = (r.n + kAvgChunk - 1) / kAvgChunk;
-  return round_up((m ? m : 1) * sizeof(Stats), kArenaAlign) + kArenaAlign;
+  const size_t m = r.n / kAvgChunk + (r.n % kAvgChunk ? 1 : 0);
+  size_t p, b;
+  if (__builtin_mul_overflow(m ? m : (size_t)1, sizeof(Stats), &p)) return false;
+  if (!round_up_checked(p, kArenaAlign, &b)) return false;
+  if (__builtin_add_overflow(b, kArenaAlign, &b)) return false;
+  *out = b;
+  return true;
 }
 
 }  // namespace
@@ -182,6 +206,42 @@ struct hpsp_server {
   bool stop = false;
   hpsp_server_info info{};
   std::thread worker;
+
+  // Streams, events and pinned result slots; on failure the caller releases
+  // whatever was created.
+  int create_resources(int n_streams) {
+    streams.assign(n_streams, nullptr);
+    for (auto& st : streams)
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return HPSP_ERR_HIP;
+    if (hipStreamCreateWithFlags(&main, hipStreamNonBlocking) != hipSuccess) return HPSP_ERR_HIP;
+    joins.assign(n_streams, nullptr);
+    for (auto& e : joins)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return HPSP_ERR_HIP;
+    if (hipEventCreateWithFlags(&last_done, hipEventDisableTiming) != hipSuccess)
+      return HPSP_ERR_HIP;
+    if (hipEventRecord(last_done, main) != hipSuccess) return HPSP_ERR_HIP;
+    void* p = nullptr;
+    if (hipHostMalloc(&p, kStatSlots * sizeof(hpsp_stats), hipHostMallocDefault) != hipSuccess)
+      return HPSP_ERR_HIP;
+    pinned = static_cast<hpsp_stats*>(p);
+    return HPSP_OK;
+  }
+
+  void release_resources() {
+    for (auto& e : joins)
+      if (e) (void)hipEventDestroy(e);
+    joins.clear();
+    if (last_done) (void)hipEventDestroy(last_done);
+    last_done = nullptr;
+    for (auto& st : streams)
+      if (st) (void)hipStreamDestroy(st);
+    streams.clear();
+    if (main) (void)hipStreamDestroy(main);
+    main = nullptr;
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    arena.destroy();
+  }
 
   // ---- worker side ------------------------------------------------------
   void admit_locked() {
@@ -404,24 +464,19 @@ int hpsp_server_create(hpsp_server** out, size_t hbm_bytes, int n_streams,
     return HPSP_ERR_INVALID_ARGUMENT;
   auto s = std::make_unique<hpsp_server>();
   if (hipGetDevice(&s->device) != hipSuccess) return HPSP_ERR_NO_DEVICE;
-  if (chunk_events) s->chunk = round_up(chunk_events, kAvgChunk);
+  if (chunk_events) {
+    size_t c = 0;
+    if (!round_up_checked(chunk_events, kAvgChunk, &c) || c == 0)
+      return HPSP_ERR_INVALID_ARGUMENT;
+    s->chunk = c;
+  }
   int rc = s->arena.init(hbm_bytes);
   if (rc) return rc;
-  s->streams.resize(n_streams);
-  for (int k = 0; k < n_streams; ++k)
-    if (hipStreamCreateWithFlags(&s->streams[k], hipStreamNonBlocking) != hipSuccess)
-      return HPSP_ERR_HIP;
-  if (hipStreamCreateWithFlags(&s->main, hipStreamNonBlocking) != hipSuccess) return HPSP_ERR_HIP;
-  s->joins.resize(n_streams);
-  for (auto& e : s->joins)
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return HPSP_ERR_HIP;
-  if (hipEventCreateWithFlags(&s->last_done, hipEventDisableTiming) != hipSuccess)
-    return HPSP_ERR_HIP;
-  (void)hipEventRecord(s->last_done, s->main);
-  void* p = nullptr;
-  if (hipHostMalloc(&p, kStatSlots * sizeof(hpsp_stats), hipHostMallocDefault) != hipSuccess)
-    return HPSP_ERR_HIP;
-  s->pinned = static_cast<hpsp_stats*>(p);
+  rc = s->create_resources(n_streams);
+  if (rc) {
+    s->release_resources();
+    return rc;
+  }
   for (int i = kStatSlots - 1; i >= 0; --i) s->free_slots.push_back(i);
   s->info.arena_bytes = s->arena.size();
   hpsp_server* raw = s.release();
@@ -443,8 +498,7 @@ int hpsp_server_submit(hpsp_server* s, const hpsp_request* r, uint64_t* ticket) 
     if (rc) return rc;
   }
   q->dk = make_decayk(&r->decay, r->seed);
-  q->bytes = footprint(*r);
-  if (q->bytes > s->arena.size()) {
+  if (!footprint(*r, &q->bytes) || q->bytes > s->arena.size()) {
     std::lock_guard<std::mutex> g(s->mu);
     ++s->info.rejected;
     return HPSP_ERR_WORKSPACE;
@@ -507,12 +561,7 @@ int hpsp_server_destroy(hpsp_server* s) {
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
   (void)hipDeviceSynchronize();
-  for (auto& e : s->joins) (void)hipEventDestroy(e);
-  (void)hipEventDestroy(s->last_done);
-  for (auto& st : s->streams) (void)hipStreamDestroy(st);
-  (void)hipStreamDestroy(s->main);
-  if (s->pinned) (void)hipHostFree(s->pinned);
-  s->arena.destroy();
+  s->release_resources();
   delete s;
   return HPSP_OK;
 }

@@ -479,7 +479,7 @@ static void uw_one(const UnweightArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_uw_scan_top, dim3(1), dim3(kBlock), 0, s, a.run_sum, nruns, a.n_acc);
   hipLaunchKernelGGL(k_uw_scan_tiles, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
                      ntiles, a.run_sum, a.tile_off);
-  if ((a.cols_out || a.idx_out) && !a.regen) {
+  if ((a.cols_out || a.idx_out || a.w_out) && !a.regen) {
     if constexpr (N <= 6) {
       const uint64_t nblocks = (a.n + kBlock - 1) / kBlock;
       const uint64_t grid = (nblocks + kXcds - 1) / kXcds * kXcds;

@@ -338,9 +338,12 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   double GetMotherMass() const { return fMotherMass; }
 
   // ---- raw device access (SoA)
+  // Non-const access may write the momenta: the container then no longer
+  // knows which Generate's events it holds, nor their PhaseSpaceWeight.
   double* column(size_t particle, int component) {
     materialize();
     fGenValid = false;
+    fWValid = false;
     return fCols.data() + (4 * particle + component) * fCap;
   }
   const double* column(size_t particle, int component) const {
@@ -350,6 +353,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   double* data() {
     materialize();
     fGenValid = false;
+    fWValid = false;
     return fCols.data();
   }
   const double* data() const {
@@ -359,12 +363,17 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   size_t ld() const { return fCap; }
 
   // For a writer that replaces every column: no materialisation, and the
-  // container forgets which events it held.
+  // container forgets which events it held and their weights (the writer
+  // states them again with set_generated / set_weights_cache_valid).
   double* columns_for_overwrite() {
     fLazy = false;
     fGenValid = false;
+    fWValid = false;
     return fCols.data();
   }
+  // Materialise deferred momenta without forgetting anything: for a writer
+  // that replaces only some rows and must keep the others.
+  void materialize_rows() const { materialize(); }
 
   // PhaseSpace::Generate(mother, events) with this container's decay: only
   // the PhaseSpaceWeight column has been written; the momenta of events

@@ -187,19 +187,9 @@ class PhaseSpace {
       hydra::device::vector<Vector4R> const& mothers,
       Decays<hydra::tuple<P...>, hydra::device::sys_t>& daughters) {
     static_assert(sizeof...(P) == N, "container arity differs from PhaseSpace<N>");
-    daughters.columns_for_overwrite();
-    if (daughters.size() < mothers.size()) daughters.resize(mothers.size());
     const hpsp_decay d = descriptor(Vector4R(fMotherMass, 0, 0, 0));
-    const bool same = same_decay(daughters.GetMotherMass(), daughters.GetMasses()) &&
-                      daughters.GetMaxWeight() == fMaxWeight;
-    detail::hpsp_check(
-        hpsp_generate_mothers(&d, fSeed, 0, mothers.size(), mothers.data(), mothers.ld(),
-                              HPSP_FP64, daughters.columns_for_overwrite(), daughters.ld(), 0, nullptr,
-                              same ? daughters.weights_cache_for_write() : nullptr, nullptr),
-        "PhaseSpace::Generate");
-    detail::hpsp_sync("PhaseSpace::Generate");
-    daughters.set_weights_cache_valid(same && mothers.size() > 0);
-    return make_range(daughters.begin(), daughters.begin() + (std::ptrdiff_t)mothers.size());
+    generate_over(d, mothers.size(), mothers.data(), mothers.ld(), daughters);
+    return make_range(daughters.begin(), daughters.end());
   }
 
   // Generate(mothers, daughters) with the mothers one particle of another
@@ -210,20 +200,9 @@ class PhaseSpace {
   Range<typename Decays<hydra::tuple<P...>, hydra::device::sys_t>::iterator> Generate(
       MotherRange const& mothers, Decays<hydra::tuple<P...>, hydra::device::sys_t>& daughters) {
     static_assert(sizeof...(P) == N, "container arity differs from PhaseSpace<N>");
-    const size_t n = mothers.size();
-    daughters.columns_for_overwrite();
-    if (daughters.size() < n) daughters.resize(n);
     const hpsp_decay d = descriptor(Vector4R(fMotherMass, 0, 0, 0));
-    const bool same = same_decay(daughters.GetMotherMass(), daughters.GetMasses()) &&
-                      daughters.GetMaxWeight() == fMaxWeight;
-    detail::hpsp_check(
-        hpsp_generate_mothers(&d, fSeed, 0, n, mothers.data(0), mothers.ld(), HPSP_FP64,
-                              daughters.columns_for_overwrite(), daughters.ld(), 0, nullptr,
-                              same ? daughters.weights_cache_for_write() : nullptr, nullptr),
-        "PhaseSpace::Generate");
-    detail::hpsp_sync("PhaseSpace::Generate");
-    daughters.set_weights_cache_valid(same && n > 0);
-    return make_range(daughters.begin(), daughters.begin() + (std::ptrdiff_t)n);
+    generate_over(d, mothers.size(), mothers.data(0), mothers.ld(), daughters);
+    return make_range(daughters.begin(), daughters.end());
   }
 
   // ---- Evaluate (PhaseSpace.inl:289-348) --------------------------------
@@ -328,6 +307,35 @@ class PhaseSpace {
                   "Evaluate takes 1..HPSP_MAX_EVAL_FUNCTORS functors");
     static_assert(std::conjunction<is_phase_space_functor<F>...>::value,
                   "functors must be device-described phase-space functors");
+  }
+
+  // DecayMothers over rows 0..n-1 of daughters (launch_decayers.inl:153-182):
+  // rows from n on keep their events, as the reference's transform leaves
+  // them, and the container is grown to n rows if shorter.  The
+  // PhaseSpaceWeight column is kept when the container's decay is this one
+  // and every row's weight is then known.
+  template <typename Container>
+  void generate_over(hpsp_decay const& d, size_t n, const double* mothers, size_t mld,
+                     Container& daughters) {
+    const bool same = same_decay(daughters.GetMotherMass(), daughters.GetMasses()) &&
+                      daughters.GetMaxWeight() == fMaxWeight;
+    const bool partial = daughters.size() > n;
+    const bool rest_known = partial && daughters.weights_cache() != nullptr;
+    // rows n.. stay: form their deferred momenta before the columns are taken
+    if (partial) daughters.materialize_rows();
+    double* cols = daughters.columns_for_overwrite();
+    if (daughters.size() < n) {
+      daughters.resize(n);
+      cols = daughters.columns_for_overwrite();
+    }
+    const bool keep_w = same && n > 0 && (!partial || rest_known);
+    detail::hpsp_check(hpsp_generate_mothers(&d, fSeed, 0, n, mothers, mld, HPSP_FP64, cols,
+                                             daughters.ld(), 0, nullptr,
+                                             keep_w ? daughters.weights_cache_for_write() : nullptr,
+                                             nullptr),
+                       "PhaseSpace::Generate");
+    detail::hpsp_sync("PhaseSpace::Generate");
+    daughters.set_weights_cache_valid(keep_w);
   }
 
   bool same_decay(double M, std::array<double, N> const& m) const {

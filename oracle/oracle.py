@@ -99,6 +99,10 @@ class Oracle:
         L.po_average_on.argtypes = [_I, _I, _P, _D, _D, _P, _U64, _U64, _U64, _I, _P, _I, _P]
         L.po_average_on_mothers.argtypes = [_I, _I, _P, _D, _D, _U64, _U64, _P, _I, _P, _I, _P]
         L.po_stats_merge.argtypes = [_P, _P, _P]
+        L.po_permutation_digest.argtypes = [_P, _U64]
+        L.po_permutation_digest.restype = _U64
+        L.po_unweight_digest.argtypes = [_I, _I, _P, _D, _D, _P, _U64, _U64, _U64, _U64, _P, _I]
+        L.po_unweight_digest.restype = _U64
         L.po_evaluate.argtypes = [_I, _I, _P, _D, _D, _P, _U64, _U64, _U64, _I, _P, _P, _P, _I]
         L.po_evaluate_mothers.argtypes = [_I, _I, _P, _U64, _U64, _U64, _P, _I, _P, _P, _P, _I]
         L.po_pair_hash.argtypes = [_U64, _U64]
@@ -236,6 +240,26 @@ class Oracle:
         nacc = self.L.po_unweight(N, _ptr(m), mw, fid, _ptr(fp), max_value, _ptr(c), n, seed,
                                   _ptr(out), _ptr(idx), _ptr(flags), nthreads)
         return nacc, out, idx, flags
+
+    def permutation_digest(self, idx):
+        """Order-sensitive digest of an output-row -> container-index map."""
+        a = np.ascontiguousarray(idx, dtype=np.uint64)
+        return int(self.L.po_permutation_digest(_ptr(a), len(a)))
+
+    def unweight_digest(self, M, masses, gen_seed, n, uw_seed, first=0, engine=0, nthreads=0):
+        """Generate events first..first+n-1 and Unweight(uw_seed) them without
+        storing anything: (accepted count, permutation_digest of the stable
+        partition's row -> index map)."""
+        m = _f64(masses)
+        N = len(m)
+        rc, ecm, mw = self.params(M, m)
+        assert rc == 0 and n < (1 << 32)
+        beta = self.mother_beta([M, 0, 0, 0])
+        d = _U64(0)
+        nacc = self.L.po_unweight_digest(N, ENGINES.get(engine, engine), _ptr(m), ecm, mw,
+                                         _ptr(beta), gen_seed, first, n, uw_seed,
+                                         ctypes.byref(d), nthreads)
+        return int(nacc), int(d.value)
 
     def average_on(self, M, masses, seed, n, fid=0, fpar=None, first=0, engine=0,
                    mother4=None, intervals=1):

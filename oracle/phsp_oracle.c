@@ -739,6 +739,79 @@ uint64_t po_unweight(int N, const double* masses, double max_weight, int fid,
   return nacc;
 }
 
+/* Order-sensitive digest of a permutation: the sum over output positions
+ * pos of mix(idx[pos] << 32 | pos), mix the SplitMix64 output function (a
+ * bijection of 64-bit words).  Positions and indices are below 2^32.        */
+static inline uint64_t digest_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+uint64_t po_permutation_digest(const uint64_t* idx, uint64_t n) {
+  uint64_t d = 0;
+  long long nn = (long long)n;
+#pragma omp parallel for schedule(static) reduction(+ : d)
+  for (long long pos = 0; pos < nn; ++pos) d += digest_mix((idx[pos] << 32) | (uint64_t)pos);
+  return d;
+}
+
+/* Generate (events first..first+n-1, single mother) followed by
+ * Decays::Unweight(uw_seed) on the generated container (Decays.inl:415-455),
+ * without storing the events: each event is generated, its PhaseSpaceWeight
+ * recomputed from its momenta (Decays.inl:215-231), flagged against
+ * squares3(uw_seed) word k (Decays.inl:75-89), and the stable partition's
+ * permutation (accepted first, then rejected, both in order) is reduced to
+ * po_permutation_digest of the output-row -> container-index map.  Returns
+ * the accepted count; n must be below 2^32.  One byte of flags per event is
+ * kept between the two passes.                                             */
+uint64_t po_unweight_digest(int N, int engine, const double* masses, double ecm,
+                            double max_weight, const double* beta3, uint64_t gen_seed,
+                            uint64_t first, uint64_t n, uint64_t uw_seed, uint64_t* digest,
+                            int nthreads) {
+  *digest = 0;
+  if (n == 0 || n >= (1ULL << 32)) return 0;
+  set_threads(nthreads);
+  uint8_t* f = (uint8_t*)malloc(n);
+  uint64_t key = uw_seed;
+  key = po_splitmix64(&key);
+  long long nn = (long long)n;
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < nn; ++k) {
+    double p[PO_MAXN][4];
+    decay_one(N, engine, masses, ecm, max_weight, 0, beta3, gen_seed, first + (uint64_t)k, p);
+    const double w = event_weight_row(N, masses, max_weight, &p[0][0]);
+    const double u = po_uniform(0, po_squares3(key, (uint64_t)k));
+    f[k] = (w / max_weight > u) ? 1 : 0;
+  }
+  /* accepted count per block, then each block's destinations */
+  const long long B = 1024;
+  const long long nb = (nn + B - 1) / B;
+  uint64_t* acc_before = (uint64_t*)calloc((size_t)nb + 1, sizeof(uint64_t));
+#pragma omp parallel for schedule(static)
+  for (long long b = 0; b < nb; ++b) {
+    uint64_t c = 0;
+    for (long long k = b * B; k < nn && k < (b + 1) * B; ++k) c += f[k];
+    acc_before[b + 1] = c;
+  }
+  for (long long b = 0; b < nb; ++b) acc_before[b + 1] += acc_before[b];
+  const uint64_t nacc = acc_before[nb];
+  uint64_t d = 0;
+#pragma omp parallel for schedule(static) reduction(+ : d)
+  for (long long b = 0; b < nb; ++b) {
+    uint64_t a = acc_before[b];
+    uint64_t r = nacc + (uint64_t)(b * B) - acc_before[b];
+    for (long long k = b * B; k < nn && k < (b + 1) * B; ++k) {
+      const uint64_t dst = f[k] ? a++ : r++;
+      d += digest_mix(((uint64_t)k << 32) | dst);
+    }
+  }
+  free(acc_before);
+  free(f);
+  *digest = d;
+  return nacc;
+}
+
 /* AddStatsPHSP (StatsPHSP.h:62-86); stats are {W, mean, M2}.                */
 void po_stats_merge(const double* x, const double* y, double* out) {
   double w = x[0] + y[0];

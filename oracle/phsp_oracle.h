@@ -81,6 +81,11 @@ void po_average_on_mothers(int N, int engine, const double* masses, double ecm,
                            const double* mother_cols, int fid, const double* fpar,
                            int intervals, double* out3);
 void po_stats_merge(const double* x3, const double* y3, double* out3);
+uint64_t po_permutation_digest(const uint64_t* idx, uint64_t n);
+uint64_t po_unweight_digest(int N, int engine, const double* masses, double ecm,
+                            double max_weight, const double* beta3, uint64_t gen_seed,
+                            uint64_t first, uint64_t n, uint64_t uw_seed, uint64_t* digest,
+                            int nthreads);
 
 void po_evaluate(int N, int engine, const double* masses, double ecm, double max_weight,
                  const double* beta3, uint64_t seed, uint64_t first, uint64_t n, int nf,

@@ -45,6 +45,12 @@ static void emit_array(const char* key, const std::vector<double>& v, bool comma
   std::printf("]%s\n", comma ? "," : "");
 }
 
+// device-to-device copy of n doubles, synchronous
+static void detail_copy(double* dst, const double* src, size_t n) {
+  hydra::detail::hpsp_check(hpsp_memcpy_device(dst, src, n * sizeof(double), nullptr), "copy");
+  hydra::detail::hpsp_sync("copy");
+}
+
 static void emit_words(const char* key, const std::vector<unsigned long long>& v) {
   std::printf("\"%s\": [", key);
   for (size_t i = 0; i < v.size(); ++i) std::printf("%s\"%llu\"", i ? ", " : "", v[i]);
@@ -178,6 +184,37 @@ int main(int argc, char** argv) {
         }
     std::printf("\"unweight_paths_same\": %d,\n\"gen_flags\": [%d, %d],\n", paths_same, gen_before,
                 gen_after);
+    // momenta written through a column pointer: Unweight() flags the events
+    // with the PhaseSpaceWeight of the new momenta (Decays.inl:428-431), as
+    // hpsp_unweight without a weight column does, not with the weights the
+    // container kept from Generate
+    {
+      auto E6 = Events;
+      double* px_a = E6.column(0, 1);
+      detail_copy(px_a, E6.column(1, 1), n);
+      hpsp_decay d6;
+      hydra::detail::hpsp_check(hpsp_decay_init(&d6, 3, P_mass, masses, HPSP_SQUARES3), "init");
+      hydra::detail::device_buffer<double> out6(12 * n);
+      const size_t wsb = hpsp_unweight_workspace_size(n);
+      hydra::detail::device_buffer<unsigned char> ws6(wsb);
+      hydra::detail::device_buffer<uint64_t> nacc6(1);
+      hydra::detail::hpsp_check(
+          hpsp_unweight(&d6, nullptr, -1.0, 0x180ec6d33cfd0abaULL, E6.data(), E6.ld(), n, nullptr,
+                        out6.data(), n, nullptr, nullptr, ws6.data(), wsb, nacc6.data(), nullptr),
+          "hpsp_unweight");
+      std::vector<double> want(12 * n);
+      uint64_t want_n = 0;
+      hydra::detail::hpsp_check(hpsp_memcpy_to_host(want.data(), out6.data(), 12 * n * 8), "copy");
+      hydra::detail::hpsp_check(hpsp_memcpy_to_host(&want_n, nacc6.data(), 8), "copy");
+      const int cache_before = E6.weights_cache() ? 1 : 0;
+      auto acc6 = E6.Unweight();
+      hydra::Decays<hydra::tuple<A, B, C>, hydra::host::sys_t> h6(E6);
+      int same6 = (acc6.size() == want_n && want_n != accepted.size()) ? 1 : 0;
+      for (size_t c = 0; c < 12 && same6; ++c)
+        if (std::memcmp(h6.data() + c * h6.ld(), want.data() + c * n, n * sizeof(double)) != 0)
+          same6 = 0;
+      std::printf("\"unweight_after_write\": [%d, %d],\n", cache_before, same6);
+    }
     // Events was read after its deferred Generate: the next Generate into it
     // writes momenta and weights in one pass, with the same bits, and its
     // Unweight() still regenerates (the result of E2's above)

@@ -228,6 +228,40 @@ int main(int argc, char** argv) {
     printf("admission: %zu requests through a %zu-byte arena ok\n", cases.size(), arena);
   }
 
+  // 3b. sizes that do not fit size_t are refused, never admitted: an event
+  //     count whose footprint wraps around (a negative count cast to
+  //     uint64), and chunk sizes that round up past 2^64
+  {
+    hpsp_server* s = nullptr;
+    CHECK(hpsp_server_create(&s, 8u << 20, 2, 4096) == HPSP_OK, "create");
+    std::vector<Case> cases = mix(53, 4, 2, 7000);
+    const uint64_t wraps[] = {(uint64_t)-1, (uint64_t)-96, 1ULL << 61, (1ULL << 64) / 96 + 1,
+                              ((uint64_t)-1) / 8 + 1};
+    for (int kind = 0; kind < 2; ++kind) {
+      for (uint64_t n : wraps) {
+        Case c = cases[0];
+        c.r.kind = kind ? HPSP_REQ_AVERAGE : HPSP_REQ_GENERATE;
+        c.r.want_weights = 1;
+        c.r.n = n;
+        uint64_t t = 0;
+        const int rc = hpsp_server_submit(s, &c.r, &t);
+        CHECK(rc == HPSP_ERR_WORKSPACE, "kind %d n %llu admitted (rc %d)", kind,
+              (unsigned long long)n, rc);
+      }
+    }
+    drive(s, cases, 0);  // the server still works afterwards
+    CHECK(hpsp_server_destroy(s) == HPSP_OK, "destroy");
+    hpsp_server* bad = nullptr;
+    CHECK(hpsp_server_create(&bad, 8u << 20, 2, (uint64_t)-1) == HPSP_ERR_INVALID_ARGUMENT &&
+              bad == nullptr,
+          "chunk size rounding to 0 accepted");
+    CHECK(hpsp_server_create(&bad, 8u << 20, 2, (uint64_t)-4095) == HPSP_ERR_INVALID_ARGUMENT,
+          "chunk size rounding past 2^64 accepted");
+    CHECK(hpsp_server_create(&bad, (size_t)-1, 2, 4096) == HPSP_ERR_INVALID_ARGUMENT,
+          "arena size rounding past 2^64 accepted");
+    printf("oversized and wrapping sizes refused ok\n");
+  }
+
   // 4. shutdown with requests pending, admitted and held: destroy returns,
   //    frees everything (the leak checker sees the rest)
   {

@@ -68,6 +68,9 @@ def test_gpu_generate_unweight_average(golden, oracle):
     # Unweight() regenerates a freshly generated container and moves one whose
     # columns were handed out for writing: the same container either way
     assert r["gen_flags"] == [1, 0] and r["unweight_paths_same"] == 1
+    # momenta written through a column pointer: the kept weights are dropped
+    # and Unweight() flags with the weights of the new momenta
+    assert r["unweight_after_write"] == [0, 1]
     # Generate writes the weights and defers the momenta to their first read
     assert r["lazy_after_generate"] == 1 and r["lazy_after_read"] == 0
     assert r["lazy_same"] == 1

@@ -242,6 +242,22 @@ def test_unweight_vs_oracle(oracle, key, n):
             assert torch.equal(w_out[:n], evw[:n][idx[:n]])
 
 
+@pytest.mark.parametrize("n", [1, 1000, 5000])
+def test_unweight_weights_only_output(oracle, n):
+    """w_out requested alone (no columns, no indices) is still written: the
+    PhaseSpaceWeight column permuted by the stable partition."""
+    M, m = DECAYS["c1_3body"]
+    dec = make_decay(M, m)
+    cols, _, evw = generate(dec, 17, n)
+    onacc, _, oidx, _ = oracle.unweight(M, m, cols.cpu().numpy(), UW_SEED)
+    nacc, out, idx, w_out = unweight(dec, cols, UW_SEED, w_in=evw, want_out=False,
+                                     want_idx=False)
+    assert out is None and idx is None
+    assert nacc == onacc
+    expect = evw.cpu().numpy()[:n][oidx.astype(np.int64)]
+    assert np.array_equal(bits(w_out.cpu().numpy()[:n]), bits(expect))
+
+
 def test_unweight_empty_and_extremes(oracle):
     M, m = DECAYS["c1_3body"]
     dec = make_decay(M, m)

@@ -234,6 +234,45 @@ def test_unweight_is_stable_partition(oracle):
     assert np.array_equal(out, cols[:, idx.astype(np.int64)])
 
 
+def test_unweight_digest_equals_stored_partition(oracle, golden):
+    """The streaming Generate + Unweight digest equals the digest of the
+    stored partition's index map (po_unweight, pinned to the reference's
+    golden index set) at the golden size and at ragged larger sizes."""
+    M, m = DECAYS["c1_3body"]
+    d_golden = oracle.permutation_digest(golden["uw_c1_idx"])
+    assert oracle.unweight_digest(M, m, SEED, 4096, UW_SEED) == \
+        (int(golden["uw_c1_nacc"][0]), d_golden)
+    for n, seed in ((1, 5), (1023, 9), (100_003, 11)):
+        cols, _ = oracle.generate(M, m, seed, n)
+        nacc, _, idx, _ = oracle.unweight(M, m, cols, UW_SEED)
+        assert oracle.unweight_digest(M, m, seed, n, UW_SEED) == \
+            (nacc, oracle.permutation_digest(idx))
+
+
+def test_permutation_digest_is_order_sensitive(oracle):
+    """Swapping two rows, shifting a run by one position, or dropping the
+    rejected half all change the digest (the mistakes a scatter could make)."""
+    rng = np.random.default_rng(3)
+    idx = rng.permutation(50_000).astype(np.uint64)
+    d = oracle.permutation_digest(idx)
+    sw = idx.copy()
+    sw[[10, 20]] = sw[[20, 10]]
+    assert oracle.permutation_digest(sw) != d
+    sh = np.concatenate([idx[1:], idx[:1]])
+    assert oracle.permutation_digest(sh) != d
+    assert oracle.permutation_digest(idx[:25_000]) != d
+    assert oracle.permutation_digest(idx) == d
+
+
+def test_unweight_digest_equals_reference(oracle, reference):
+    """Against the reference's own Generate + Unweight() (OMP build) at 2^20."""
+    M, m = DECAYS["c1_3body"]
+    n = 1 << 20
+    nacc, _, idx, _ = reference.unweight(M, m, SEED, n, UW_SEED)
+    assert oracle.unweight_digest(M, m, SEED, n, UW_SEED) == \
+        (nacc, oracle.permutation_digest(idx))
+
+
 # ---------------------------------------------------------------- AverageOn
 @pytest.mark.parametrize("tag,fid", [("one", 0), ("m2", 2), ("bw3", 1)])
 def test_average_on_golden(oracle, golden, tag, fid):

@@ -35,7 +35,7 @@ def test_server_host_logic_asan_ubsan(binaries):
               {"ASAN_OPTIONS": "detect_leaks=1:abort_on_error=0",
                "UBSAN_OPTIONS": "halt_on_error=1:print_stacktrace=1"})
     for line in ("paced: 600 requests ok", "burst: 600 requests ok", "admission:",
-                 "shutdown with work pending ok"):
+                 "oversized and wrapping sizes refused ok", "shutdown with work pending ok"):
         assert line in out, out
 
 
