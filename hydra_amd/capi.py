@@ -145,6 +145,9 @@ _SIGS = {
                            _P, _U64, _P]),
     "hpsp_evaluate_mothers": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _I, _U64, _P,
                                    _U64, _U64, _U64, _P, _U64, _P]),
+    "hpsp_unweight_values": (_I, [_I, _P, _D, _U64, _P, _U64, _U64, _P, _U64, _P, _P, _P, _SZ,
+                                  _P, _P]),
+    "hpsp_stats_reduce": (_I, [_P, _U64, _P, _P]),
     "hpsp_unweight_regenerate": (_I, [ctypes.POINTER(Decay), _U64, _U64, _U64, _P, _U64, _P,
                                       _U64, _P, _P, _P, _SZ, _P, _P]),
     "hpsp_unweight_sample_workspace_size": (_SZ, [_U64]),

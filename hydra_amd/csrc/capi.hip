@@ -182,10 +182,10 @@ int evaluate_common(const hpsp_decay* d, const hpsp_functor* fs, int nf, uint64_
   a.d = make_decayk(d, seed);
   const int nvec = d->n + (mothers ? 1 : 0);
   for (int j = 0; j < nf; ++j) {
-    rc = make_functork(&fs[j], nvec, &a.f[j], mothers ? 1 : 0);
+    rc = make_functork(&fs[j], nvec, &a.fs.f[j], mothers ? 1 : 0);
     if (rc) return rc;
   }
-  a.nf = nf;
+  a.fs.nf = nf;
   a.seed = seed;
   a.first = first;
   a.n = n;
@@ -529,6 +529,48 @@ int hpsp_unweight(const hpsp_decay* d, const hpsp_functor* f, double max_value, 
   return hip_status(launch_unweight(d->n, a, s));
 }
 
+int hpsp_unweight_values(int n_particles, const double* values, double max_value, uint64_t seed,
+                         const double* cols_in, uint64_t ld_in, uint64_t n, double* cols_out,
+                         uint64_t ld_out, uint64_t* idx_out, double* values_out, void* ws,
+                         size_t ws_bytes, uint64_t* n_acc_dev, void* stream) {
+  if (n_particles < 2 || n_particles > HPSP_MAX_N || !n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (n > 0) {
+    if (!values) return HPSP_ERR_INVALID_ARGUMENT;
+    if (cols_out && (!cols_in || ld_in < n || ld_out < n)) return HPSP_ERR_INVALID_ARGUMENT;
+  }
+  const UwLayout L = uw_layout(n);
+  if (!ws || ws_bytes < L.total) return HPSP_ERR_WORKSPACE;
+  char* base = static_cast<char*>(ws);
+  hipStream_t s = as_stream(stream);
+  UnweightArgs a;
+  memset(&a, 0, sizeof(a));
+  a.f.kind = HPSP_F_ONE;
+  a.maxv = max_value;
+  if (!(max_value > 0.0) && n > 0) {
+    double* mv = reinterpret_cast<double*>(base + L.maxv);
+    const int e = launch_max_column(values, n, reinterpret_cast<double*>(base + L.block_max),
+                                    grid_for(n), mv, s);
+    if (e) return HPSP_ERR_HIP;
+    a.maxv_dev = mv;
+  }
+  uint64_t s0 = seed;
+  a.uw_key = splitmix64_host(&s0);
+  a.cols_in = cols_in;
+  a.ld_in = ld_in;
+  a.n = n;
+  a.w_in = values;
+  a.cols_out = cols_out;
+  a.ld_out = ld_out;
+  a.idx_out = idx_out;
+  a.w_out = values_out;
+  a.flags = reinterpret_cast<uint64_t*>(base + L.flags);
+  a.tile_count = reinterpret_cast<uint32_t*>(base + L.tile_count);
+  a.tile_off = reinterpret_cast<uint64_t*>(base + L.tile_off);
+  a.run_sum = reinterpret_cast<uint64_t*>(base + L.run_sum);
+  a.n_acc = n_acc_dev;
+  return hip_status(launch_unweight(n_particles, a, s));
+}
+
 int hpsp_unweight_regenerate(const hpsp_decay* d, uint64_t gen_seed, uint64_t first,
                              uint64_t n, const double* w, uint64_t seed, double* cols_out,
                              uint64_t ld_out, uint64_t* idx_out, double* w_out, void* ws,
@@ -655,6 +697,12 @@ int hpsp_average_on(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed, u
                     uint64_t n, int prec, void* ws, size_t ws_bytes, hpsp_stats* out_dev,
                     void* stream) {
   return average_common(d, f, seed, first, n, nullptr, 0, prec, ws, ws_bytes, out_dev, stream);
+}
+
+int hpsp_stats_reduce(const hpsp_stats* partials, uint64_t m, hpsp_stats* out_dev, void* stream) {
+  if (!out_dev || (m > 0 && !partials)) return HPSP_ERR_INVALID_ARGUMENT;
+  return hip_status(launch_stats_finalize(reinterpret_cast<const Stats*>(partials), m,
+                                          reinterpret_cast<Stats*>(out_dev), as_stream(stream)));
 }
 
 int hpsp_average_on_mothers(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed,

@@ -182,68 +182,6 @@ void k_uw_regen(const UnweightArgs a) {
   }
 }
 
-// AverageOn: chunks of kAvgChunk consecutive events, one partial per chunk.
-// Within a chunk wave w folds its own run of consecutive groups of 64 events
-// (groups 16w .. 16w+15), lane l event l of each, in order, so the squares
-// words carry from group to group; the four wave results merge in a fixed
-// tree, so a partial depends only on its event range.
-template <int N, int ENGINE, bool MOTHERS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
-void k_average_f64(const AvgArgs a) {
-  __shared__ Stats sh[kBlock / kWave];
-  const int lane = __lane_id();
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const uint64_t nchunks = (a.n + kAvgChunk - 1) / kAvgChunk;
-  for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    Stats acc = {0.0, 0.0, 0.0};
-    constexpr int kGroupsPerWave = kAvgChunk / kBlock;
-    EventWords<N, ENGINE> words;
-    for (int it = 0; it < kGroupsPerWave; ++it) {
-      const uint64_t k0 =
-          c * kAvgChunk + ((uint64_t)wave * kGroupsPerWave + (uint64_t)it) * kWave;
-      const uint64_t k = k0 + (uint64_t)lane;
-      const bool live = k < a.n;
-      if (it == 0)
-        words.init(a.d, a.first + k0, lane);
-      else
-        words.advance(a.d, a.first + k0, lane);
-      double p[N + (MOTHERS ? 1 : 0)][4];
-      double f, wt;
-      if constexpr (MOTHERS) {
-        double q[N][4];
-        wt = genbod_f64<N, ENGINE>(a.d, words, q);
-        const uint64_t km = live ? k : 0;
-        double m4[4];
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc) m4[cc] = a.mothers[(uint64_t)cc * a.mld + km];
-        const double bx = m4[1] / m4[0], by = m4[2] / m4[0], bz = m4[3] / m4[0];
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-          boost_general(q[i], bx, by, bz);
-#pragma unroll
-          for (int cc = 0; cc < 4; ++cc) p[i + 1][cc] = q[i][cc];
-        }
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc) p[0][cc] = m4[cc];
-      } else {
-        wt = genbod_f64<N, ENGINE>(a.d, words, p);
-        if (a.d.do_boost) {
-#pragma unroll
-          for (int i = 0; i < N; ++i) boost_folded(p[i], a.d);
-        }
-      }
-      f = functor_eval<double, false>(a.f, p);
-      if (live) stats_push(acc, f, wt);
-    }
-    acc = wave_reduce_stats(acc);
-    if (lane == 0) sh[wave] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0)
-      a.partials[c] = stats_merge(stats_merge(sh[0], sh[1]), stats_merge(sh[2], sh[3]));
-    __syncthreads();
-  }
-}
-
 // Merge m partials in a fixed order: thread t folds a contiguous slice, then
 // a pairwise tree over the 256 slice results.
 __global__ __launch_bounds__(kBlock) void k_stats_finalize(const Stats* part, uint64_t m,

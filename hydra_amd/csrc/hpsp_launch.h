@@ -30,19 +30,31 @@ struct GenArgs {
   uint32_t run_cap;  // 0: the launcher's policy; else the longest run allowed
 };
 
-struct AvgArgs {
+// AverageOn arguments; FK is the functor: a FunctorK description here, a
+// user's callable in a translation unit that compiles its own kernels
+// (hydra_amd/include/hydra/detail/UserKernels.h).
+template <typename FK>
+struct AvgArgsT {
   DecayK d;
-  FunctorK f;
+  FK f;
   uint64_t first, n;
   const double* mothers;
   uint64_t mld;
   Stats* partials;  // one per kAvgChunk events
 };
+using AvgArgs = AvgArgsT<FunctorK>;
 
-struct EvalArgs {
-  DecayK d;
+// Up to HPSP_MAX_EVAL_FUNCTORS described functors, evaluated one by one.
+struct DescSet {
   FunctorK f[HPSP_MAX_EVAL_FUNCTORS];
   int nf;
+};
+
+// Evaluate arguments; FS is the set of functors whose values are written.
+template <typename FS>
+struct EvalArgsT {
+  DecayK d;
+  FS fs;
   uint64_t seed;            // Evaluate(mothers): the seed paired with each event index
   uint64_t first, n;
   const double* mothers;    // Evaluate(mothers): SoA mother columns
@@ -51,6 +63,7 @@ struct EvalArgs {
   uint64_t ld;
   uint32_t run;             // consecutive groups of 64 events per wave (set by the launcher)
 };
+using EvalArgs = EvalArgsT<DescSet>;
 
 // Four-vectors of stored events: particle i, component c, event k at
 // cols[i][c * ld[i] + k].
@@ -192,6 +205,9 @@ int launch_event_weight(int N, const WeightArgs& a, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s);
+// largest of n values (max_element's value) into *out, through per-block maxima
+int launch_max_column(const double* w, uint64_t n, double* block_max, uint64_t nblocks,
+                      double* out, hipStream_t s);
 // after launch_unweight with regen set: the stable partition of events
 // gen_first .. gen_first + n - 1 of the generation stream, generated again
 int launch_unweight_regen(int N, int engine, const UnweightArgs& a, hipStream_t s);

@@ -545,6 +545,13 @@ static void launch_flags(int N, const UnweightArgs& a, uint64_t ntiles, hipStrea
   kFlags[N == 0 ? 0 : N - 1](a, grid, s);
 }
 
+int launch_max_column(const double* w, uint64_t n, double* block_max, uint64_t nblocks,
+                      double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_max_column, dim3((unsigned)nblocks), dim3(kBlock), 0, s, w, n, block_max);
+  hipLaunchKernelGGL(k_max_of_blocks, dim3(1), dim3(kBlock), 0, s, block_max, nblocks, out);
+  return (int)hipGetLastError();
+}
+
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s) {
   if (a.n == 0) {
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, a.n_acc, (uint64_t)0);

@@ -26,8 +26,10 @@
 #include <hydra/Types.h>
 #include <hydra/Vector4R.h>
 #include <hydra/detail/Runtime.h>
+#include <hydra/detail/UserFunctors.h>
 #include <hydra/device/System.h>
 #include <hydra/host/System.h>
+#include <hydra/multivector.h>
 
 #include "hpsp.h"
 
@@ -122,6 +124,25 @@ class PhaseSpaceReweight {
   hpsp_functor f_;
 };
 
+// PhaseSpaceReweight<F, P...> with a user's callable F (compiled by hipcc into
+// the caller's kernels, hydra/detail/UserKernels.h): `events | reweight`
+// gives PhaseSpaceWeight * F of each event, F receiving the event's particles
+// by their declarg types.
+template <typename Functor, typename... Particles>
+class UserReweight {
+ public:
+  static constexpr size_t N = sizeof...(Particles);
+  UserReweight(Functor const& f, double motherMass, std::array<double, N> const& masses)
+      : d_(detail::make_decay<N>(motherMass, masses, "PhaseSpaceReweight")), f_(f) {}
+  double GetMaxWeight() const { return d_.max_weight; }
+  const hpsp_decay& descriptor() const { return d_; }
+  const Functor& functor() const { return f_; }
+
+ private:
+  hpsp_decay d_;
+  Functor f_;
+};
+
 // Decays::Meld(others...) (Decays.h:286-293): the particles of one container
 // followed by those of the others, row by row, as the decay chain of
 // examples/phase_space/phsp_chain.inl:244 joins B0 -> J/psi K pi with
@@ -180,6 +201,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     value_type operator*() const { return (*c_)[i_]; }
     iterator& operator++() { ++i_; return *this; }
     iterator operator+(difference_type d) const { return iterator(c_, i_ + d); }
+    const Decays* container() const { return c_; }
     iterator operator-(difference_type d) const { return iterator(c_, i_ - d); }
     difference_type operator-(const iterator& o) const { return (difference_type)i_ - (difference_type)o.i_; }
     bool operator==(const iterator& o) const { return i_ == o.i_ && c_ == o.c_; }
@@ -309,11 +331,14 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   PhaseSpaceWeight<Particles...> GetEventWeightFunctor() const {
     return PhaseSpaceWeight<Particles...>(fMotherMass, fMasses);
   }
+  // a described functor (the library's kernels) or a user's callable
+  // (compiled by hipcc in the caller's translation unit)
   template <typename Functor>
-  PhaseSpaceReweight<Functor, Particles...> GetEventWeightFunctor(Functor const& f) const {
-    static_assert(is_phase_space_functor<Functor>::value,
-                  "functor must be one of the device-described phase-space functors");
-    return PhaseSpaceReweight<Functor, Particles...>(f, fMotherMass, fMasses);
+  auto GetEventWeightFunctor(Functor const& f) const {
+    if constexpr (is_phase_space_functor<Functor>::value)
+      return PhaseSpaceReweight<Functor, Particles...>(f, fMotherMass, fMasses);
+    else
+      return UserReweight<Functor, Particles...>(f, fMotherMass, fMasses);
   }
 
   // ---- unweighting (Decays.inl:415-511)
@@ -326,11 +351,13 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   template <typename Functor>
   Range<iterator> Unweight(Functor const& functor, double weight = -1.0,
                            size_t seed = 0x39abdc4529b1661c) {
-    static_assert(is_phase_space_functor<Functor>::value,
-                  "functor must be one of the device-described phase-space functors");
     const hpsp_decay d = detail::make_decay<N>(fMotherMass, fMasses, "Decays::Unweight");
-    const hpsp_functor f = detail::to_hpsp(functor);
-    return unweight(d, &f, weight, seed);
+    if constexpr (is_phase_space_functor<Functor>::value) {
+      const hpsp_functor f = detail::to_hpsp(functor);
+      return unweight(d, &f, weight, seed);
+    } else {
+      return unweight_user(d, functor, weight, seed);
+    }
   }
 
   std::array<double, N> GetMasses() const { return fMasses; }
@@ -524,6 +551,34 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     return Range<iterator>(begin(), begin() + (std::ptrdiff_t)acc);
   }
 
+  // Unweight(functor) with a user's callable: its PhaseSpaceReweight values
+  // from a kernel compiled here, the partition by the library.  The kept
+  // PhaseSpaceWeight column is not carried through (it is recomputed when
+  // next needed).
+  template <typename Functor>
+  Range<iterator> unweight_user(const hpsp_decay& d, Functor const& functor, double max_value,
+                                size_t seed) {
+    materialize();
+    const size_t n = fSize;
+    static_assert(detail::user_width<hydra::tuple<Particles...>, Functor>() == 1,
+                  "Unweight(functor): the functor returns one value per event");
+    detail::device_buffer<double> vals(n ? n : 1);
+    detail::user_event_values((hydra::tuple<Particles...>*)nullptr, d, fCols.data(), fCap, n,
+                              true, vals.data(), n ? n : 1, functor);
+    detail::device_buffer<double> out(4 * N * (n ? n : 1));
+    detail::device_buffer<double> wout(n ? n : 1);
+    const size_t wsb = hpsp_unweight_workspace_size(n);
+    detail::device_buffer<unsigned char> ws(wsb);
+    detail::device_buffer<uint64_t> nacc(1);
+    detail::hpsp_check(hpsp_unweight_values((int)N, vals.data(), max_value, (uint64_t)seed,
+                                            fCols.data(), fCap, n, out.data(), n ? n : 1,
+                                            nullptr, nullptr, ws.data(), wsb, nacc.data(),
+                                            nullptr),
+                       "Decays::Unweight");
+    fWValid = false;
+    return adopt(out, wout, nacc);
+  }
+
   Range<iterator> unweight(const hpsp_decay& d, const hpsp_functor* f, double max_value,
                            size_t seed) {
     materialize();
@@ -668,6 +723,67 @@ EventWeights operator|(const Decays<hydra::tuple<Particles...>, hydra::device::s
     detail::hpsp_sync("PhaseSpaceReweight");
   }
   return out;
+}
+
+// events | events.GetEventWeightFunctor(f) with a user's callable f.
+template <typename Functor, typename... Particles>
+EventWeights operator|(const Decays<hydra::tuple<Particles...>, hydra::device::sys_t>& ev,
+                       const UserReweight<Functor, Particles...>& w) {
+  static_assert(detail::user_width<hydra::tuple<Particles...>, Functor>() == 1,
+                "GetEventWeightFunctor(functor): the functor returns one value per event");
+  EventWeights out(ev.size());
+  detail::user_event_values((hydra::tuple<Particles...>*)nullptr, w.descriptor(), ev.data(),
+                            ev.ld(), ev.size(), true, out.data(), ev.size() ? ev.size() : 1,
+                            w.functor());
+  return out;
+}
+
+namespace detail {
+
+template <size_t K>
+struct value_columns {
+  template <size_t... I>
+  static auto make(std::index_sequence<I...>) -> hydra::tuple<decltype((void)I, double())...>;
+  using type = multivector<decltype(make(std::make_index_sequence<K>{})), hydra::device::sys_t>;
+};
+
+// rows [first, first + n) of a container through a wrapped lambda
+template <typename L, typename... Particles>
+auto lambda_rows(const Decays<hydra::tuple<Particles...>, hydra::device::sys_t>& ev, size_t first,
+                 size_t n, hydra::Lambda<L, 0> const& f) {
+  constexpr int K = user_width<hydra::tuple<Particles...>, hydra::Lambda<L, 0>>();
+  const hpsp_decay d = make_decay<sizeof...(Particles)>(ev.GetMotherMass(), ev.GetMasses(),
+                                                        "events | functor");
+  if constexpr (K == 1) {
+    EventWeights out(n);
+    user_event_values((hydra::tuple<Particles...>*)nullptr, d, ev.data() + first, ev.ld(), n,
+                      false, out.data(), n ? n : 1, f);
+    return out;
+  } else {
+    typename value_columns<(size_t)K>::type out(n);
+    user_event_values((hydra::tuple<Particles...>*)nullptr, d, ev.data() + first, ev.ld(), n,
+                      false, out.data(), out.ld(), f);
+    return out;
+  }
+}
+
+}  // namespace detail
+
+// events | lambda (Range2.inl:213-238) materialised on the device: a lambda
+// of one value gives a column (EventWeights), of a tuple of K values a
+// multivector of K columns (the Dalitz variables of
+// examples/phase_space/phsp_basic.inl:159-163 and 194).
+template <typename L, typename... Particles>
+auto operator|(const Decays<hydra::tuple<Particles...>, hydra::device::sys_t>& ev,
+               hydra::Lambda<L, 0> const& f) {
+  return detail::lambda_rows(ev, 0, ev.size(), f);
+}
+
+// A range of a container's rows (what Unweight returns) through a lambda
+// (examples/phase_space/phsp_unweighting_functor.inl:209).
+template <typename It, typename L, typename = decltype(std::declval<It const&>().container())>
+auto operator|(Range<It> const& r, hydra::Lambda<L, 0> const& f) {
+  return detail::lambda_rows(*r.begin().container(), r.begin().index(), r.size(), f);
 }
 
 // hydra::unweight(device::sys, events, functor[, max_pdf, rng_seed, rng_jump])

@@ -150,6 +150,19 @@ class DenseHistogram<double, N, hydra::device::sys_t> {
     return fill_columns(data, weights.data());
   }
 
+  // Fill(data) / Fill(data, weights) from one value column (a one-value
+  // lambda over the events, `events | lambda`)
+  template <size_t M = N, typename = typename std::enable_if<M == 1>::type>
+  DenseHistogram& Fill(EventWeights const& data) {
+    return fill_column(data, nullptr);
+  }
+  template <size_t M = N, typename = typename std::enable_if<M == 1>::type>
+  DenseHistogram& Fill(EventWeights const& data, EventWeights const& weights) {
+    if (weights.size() < data.size())
+      throw std::invalid_argument("[hydra::DenseHistogram::Fill]: fewer weights than data");
+    return fill_column(data, weights.data());
+  }
+
   // Fused fills from the events
   template <typename EventsT>
   DenseHistogram& Fill(EventAxes<N, EventsT> const& axes) {
@@ -238,6 +251,14 @@ class DenseHistogram<double, N, hydra::device::sys_t> {
     static_assert(MV::ncols == N, "data needs one column per axis");
     detail::hpsp_check(hpsp_histogram_fill(&h_, data.data(), data.ld(), data.size(), w,
                                            contents_.data(), nullptr),
+                       "DenseHistogram::Fill");
+    detail::hpsp_sync("DenseHistogram::Fill");
+    return *this;
+  }
+
+  DenseHistogram& fill_column(EventWeights const& data, const double* w) {
+    detail::hpsp_check(hpsp_histogram_fill(&h_, data.data(), data.size() ? data.size() : 1,
+                                           data.size(), w, contents_.data(), nullptr),
                        "DenseHistogram::Fill");
     detail::hpsp_sync("DenseHistogram::Fill");
     return *this;

@@ -18,17 +18,19 @@ struct named_arg;
 template <typename T>
 struct named_arg<T, true> : public T {
   named_arg() = default;
-  named_arg(T const& x) : T(x) {}
-  T Value() const { return *this; }
+  __hydra_dual__ named_arg(T const& x) : T(x) {}
+  __hydra_dual__ T Value() const { return *this; }
+  __hydra_dual__ T const& operator()() const { return *this; }
 };
 
 // arithmetic types: the argument holds a T and converts to it
 template <typename T>
 struct named_arg<T, false> {
   named_arg() = default;
-  named_arg(T x) : value(x) {}
-  operator T() const { return value; }
-  T Value() const { return value; }
+  __hydra_dual__ named_arg(T x) : value(x) {}
+  __hydra_dual__ operator T() const { return value; }
+  __hydra_dual__ T Value() const { return value; }
+  __hydra_dual__ T operator()() const { return value; }
   T value{};
 };
 
@@ -41,7 +43,7 @@ struct named_arg<T, false> {
   struct NAME : public hydra::detail::named_arg<TYPE> {       \
     using value_type = TYPE;                                  \
     NAME() = default;                                         \
-    NAME(TYPE const& x) : hydra::detail::named_arg<TYPE>(x) {} \
+    __hydra_dual__ NAME(TYPE const& x) : hydra::detail::named_arg<TYPE>(x) {} \
   };                                                          \
   }                                                           \
   }

@@ -4,7 +4,11 @@
 // Same constructor checks, seed handling and member functions as the
 // reference; Generate, AverageOn and the container's Unweight run the gfx950
 // kernels of libhpsp.so through the C ABI (include/hpsp.h).  Every call is
-// synchronous to the caller, as in the reference.
+// synchronous to the caller, as in the reference.  AverageOn and Evaluate take
+// the described functors of PhaseSpaceFunctors.h (run by the library's
+// kernels) or, in a translation unit built with hipcc, any callable of the
+// event's four-vectors (compiled into the library's kernel templates there,
+// hydra/detail/UserKernels.h).
 #pragma once
 
 #include <array>
@@ -22,6 +26,7 @@
 #include <hydra/Types.h>
 #include <hydra/Vector4R.h>
 #include <hydra/detail/Runtime.h>
+#include <hydra/detail/UserFunctors.h>
 #include <hydra/detail/random/engines.h>
 #include <hydra/device/System.h>
 #include <hydra/device/vector.h>
@@ -103,19 +108,24 @@ class PhaseSpace {
   template <typename FUNCTOR>
   std::pair<GReal_t, GReal_t> AverageOn(hydra::device::vector<Vector4R> const& mothers,
                                         FUNCTOR const& functor) {
-    static_assert(is_phase_space_functor<FUNCTOR>::value,
-                  "functor must be one of the device-described phase-space functors");
     const hpsp_decay d = descriptor(Vector4R(fMotherMass, 0, 0, 0));
-    const hpsp_functor f = detail::to_hpsp(functor);
     const size_t n = mothers.size();
-    const size_t wsb = hpsp_average_on_workspace_size(n);
-    detail::device_buffer<unsigned char> ws(wsb);
-    detail::device_buffer<hpsp_stats> out(1);
-    detail::hpsp_check(hpsp_average_on_mothers(&d, &f, fSeed, mothers.data(), mothers.ld(), n,
-                                               HPSP_FP64, ws.data(), wsb, out.data(), nullptr),
-                       "PhaseSpace::AverageOn");
     hpsp_stats st;
-    detail::hpsp_check(hpsp_memcpy_to_host(&st, out.data(), sizeof(st)), "PhaseSpace::AverageOn");
+    if constexpr (!is_phase_space_functor<FUNCTOR>::value) {
+      st = detail::user_average_on<N, GRND::engine_id, true>(d, fSeed, functor, 0, n,
+                                                             mothers.data(), mothers.ld(),
+                                                             HPSP_FP64);
+    } else {
+      const hpsp_functor f = detail::to_hpsp(functor);
+      const size_t wsb = hpsp_average_on_workspace_size(n);
+      detail::device_buffer<unsigned char> ws(wsb);
+      detail::device_buffer<hpsp_stats> out(1);
+      detail::hpsp_check(hpsp_average_on_mothers(&d, &f, fSeed, mothers.data(), mothers.ld(), n,
+                                                 HPSP_FP64, ws.data(), wsb, out.data(), nullptr),
+                         "PhaseSpace::AverageOn");
+      detail::hpsp_check(hpsp_memcpy_to_host(&st, out.data(), sizeof(st)),
+                         "PhaseSpace::AverageOn");
+    }
     return std::make_pair(st.mean, ::sqrt(st.m2) / st.w);
   }
 
@@ -125,19 +135,23 @@ class PhaseSpace {
   template <typename FUNCTOR>
   hpsp_stats AverageOnStats(Vector4R const& mother, FUNCTOR const& functor, size_t first,
                             size_t n, int precision = HPSP_FP64) {
-    static_assert(is_phase_space_functor<FUNCTOR>::value,
-                  "functor must be one of the device-described phase-space functors");
     const hpsp_decay d = descriptor(mother);
-    const hpsp_functor f = detail::to_hpsp(functor);
-    const size_t wsb = hpsp_average_on_workspace_size(n);
-    detail::device_buffer<unsigned char> ws(wsb);
-    detail::device_buffer<hpsp_stats> out(1);
-    detail::hpsp_check(hpsp_average_on(&d, &f, fSeed, first, n, precision, ws.data(), wsb,
-                                       out.data(), nullptr),
-                       "PhaseSpace::AverageOn");
-    hpsp_stats st;
-    detail::hpsp_check(hpsp_memcpy_to_host(&st, out.data(), sizeof(st)), "PhaseSpace::AverageOn");
-    return st;
+    if constexpr (!is_phase_space_functor<FUNCTOR>::value) {
+      return detail::user_average_on<N, GRND::engine_id, false>(d, fSeed, functor, first, n,
+                                                                nullptr, 0, precision);
+    } else {
+      const hpsp_functor f = detail::to_hpsp(functor);
+      const size_t wsb = hpsp_average_on_workspace_size(n);
+      detail::device_buffer<unsigned char> ws(wsb);
+      detail::device_buffer<hpsp_stats> out(1);
+      detail::hpsp_check(hpsp_average_on(&d, &f, fSeed, first, n, precision, ws.data(), wsb,
+                                         out.data(), nullptr),
+                         "PhaseSpace::AverageOn");
+      hpsp_stats st;
+      detail::hpsp_check(hpsp_memcpy_to_host(&st, out.data(), sizeof(st)),
+                         "PhaseSpace::AverageOn");
+      return st;
+    }
   }
 
   // ---- Generate (PhaseSpace.inl:351-444) --------------------------------
@@ -213,13 +227,17 @@ class PhaseSpace {
       FUNCTORS const&... functors) {
     static_assert(sizeof...(T) == sizeof...(FUNCTORS) + 1,
                   "result needs one column for the weight and one per functor");
-    check_functors<FUNCTORS...>();
     const hpsp_decay d = descriptor(mother);
-    const hpsp_functor fs[] = {detail::to_hpsp(functors)...};
-    detail::hpsp_check(hpsp_evaluate(&d, fs, (int)sizeof...(FUNCTORS), fSeed, 0, result.size(),
-                                     result.data(), result.ld(), nullptr),
-                       "PhaseSpace::Evaluate");
-    detail::hpsp_sync("PhaseSpace::Evaluate");
+    if constexpr (check_functors<FUNCTORS...>()) {
+      const hpsp_functor fs[] = {detail::to_hpsp(functors)...};
+      detail::hpsp_check(hpsp_evaluate(&d, fs, (int)sizeof...(FUNCTORS), fSeed, 0, result.size(),
+                                       result.data(), result.ld(), nullptr),
+                         "PhaseSpace::Evaluate");
+      detail::hpsp_sync("PhaseSpace::Evaluate");
+    } else {
+      detail::user_evaluate<N, GRND::engine_id, false>(d, fSeed, nullptr, 0, 0, result.size(),
+                                                       result.data(), result.ld(), functors...);
+    }
     return make_range(result.begin(), result.end());
   }
 
@@ -233,17 +251,22 @@ class PhaseSpace {
       FUNCTORS const&... functors) {
     static_assert(sizeof...(T) == sizeof...(FUNCTORS) + 1,
                   "result needs one column for the weight and one per functor");
-    check_functors<FUNCTORS...>();
     if (result.size() < mothers.size())
       throw std::invalid_argument("[hydra::PhaseSpace::Evaluate]: result shorter than mothers");
     const hpsp_decay d = descriptor(Vector4R(fMotherMass, 0, 0, 0));
-    const hpsp_functor fs[] = {detail::to_hpsp(functors)...};
-    detail::hpsp_check(
-        hpsp_evaluate_mothers(&d, fs, (int)sizeof...(FUNCTORS), fSeed, mothers.data(),
-                              mothers.ld(), 0, mothers.size(), result.data(), result.ld(),
-                              nullptr),
-        "PhaseSpace::Evaluate");
-    detail::hpsp_sync("PhaseSpace::Evaluate");
+    if constexpr (check_functors<FUNCTORS...>()) {
+      const hpsp_functor fs[] = {detail::to_hpsp(functors)...};
+      detail::hpsp_check(
+          hpsp_evaluate_mothers(&d, fs, (int)sizeof...(FUNCTORS), fSeed, mothers.data(),
+                                mothers.ld(), 0, mothers.size(), result.data(), result.ld(),
+                                nullptr),
+          "PhaseSpace::Evaluate");
+      detail::hpsp_sync("PhaseSpace::Evaluate");
+    } else {
+      detail::user_evaluate<N, GRND::engine_id, true>(d, fSeed, mothers.data(), mothers.ld(), 0,
+                                                      mothers.size(), result.data(), result.ld(),
+                                                      functors...);
+    }
     return make_range(result.begin(), result.begin() + (std::ptrdiff_t)mothers.size());
   }
 
@@ -301,12 +324,17 @@ class PhaseSpace {
     fMaxWeight = d.max_weight;
   }
 
+  // True when every functor is a described one (the library's kernels),
+  // false when none is (user functors, compiled here); one call takes one kind.
   template <typename... F>
-  static void check_functors() {
+  static constexpr bool check_functors() {
     static_assert(sizeof...(F) >= 1 && sizeof...(F) <= HPSP_MAX_EVAL_FUNCTORS,
                   "Evaluate takes 1..HPSP_MAX_EVAL_FUNCTORS functors");
-    static_assert(std::conjunction<is_phase_space_functor<F>...>::value,
-                  "functors must be device-described phase-space functors");
+    constexpr bool all = std::conjunction<is_phase_space_functor<F>...>::value;
+    constexpr bool any = std::disjunction<is_phase_space_functor<F>...>::value;
+    static_assert(all || !any,
+                  "Evaluate: described functors and user functors cannot be mixed in one call");
+    return all;
   }
 
   // DecayMothers over rows 0..n-1 of daughters (launch_decayers.inl:153-182):
