@@ -100,6 +100,8 @@ class ServerInfo(ctypes.Structure):
                  "arena_bytes", "arena_in_use", "arena_peak")]
 
 
+NOALLOC_MAX_EVENTS = 1 << 28  # HPSP_NOALLOC_MAX_EVENTS
+
 REQ_GENERATE, REQ_AVERAGE = 0, 1
 ERR_SHUTDOWN = 7
 
@@ -139,6 +141,8 @@ _SIGS = {
     "hpsp_average_on_workspace_size": (_SZ, [_U64]),
     "hpsp_average_on": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _U64, _U64, _U64,
                              _I, _P, _SZ, _P, _P]),
+    "hpsp_average_on_noalloc": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _U64,
+                                     _U64, _U64, _I, _P, _P]),
     "hpsp_average_on_mothers": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _U64, _P,
                                      _U64, _U64, _I, _P, _SZ, _P, _P]),
     "hpsp_evaluate": (_I, [ctypes.POINTER(Decay), ctypes.POINTER(Functor), _I, _U64, _U64, _U64,

@@ -4,6 +4,7 @@
 // (built with -ffp-contract=off).
 #include <math.h>
 #include <algorithm>
+#include <mutex>
 #include <string.h>
 
 #include "hpsp.h"
@@ -37,6 +38,33 @@ UwLayout uw_layout(uint64_t n) {
 }
 
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Partials of hpsp_average_on_noalloc: one per kAvgChunk events of a call,
+// HPSP_NOALLOC_MAX_EVENTS / kAvgChunk of them (1.5 MiB).  Each GPU holds its
+// own copy at its own address; host code uses that address (scratch_on_device),
+// never the array's name, which in host code is only a stand-in.
+constexpr uint64_t kNoAllocPartials = HPSP_NOALLOC_MAX_EVENTS / kAvgChunk;
+static __device__ __align__(256) Stats scratch[kNoAllocPartials];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device (with that device current) and kept.
+int scratch_on_device(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return HPSP_ERR_NO_DEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return HPSP_ERR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return HPSP_OK;
+}
 
 }  // namespace
 
@@ -703,6 +731,19 @@ int hpsp_stats_reduce(const hpsp_stats* partials, uint64_t m, hpsp_stats* out_de
   if (!out_dev || (m > 0 && !partials)) return HPSP_ERR_INVALID_ARGUMENT;
   return hip_status(launch_stats_finalize(reinterpret_cast<const Stats*>(partials), m,
                                           reinterpret_cast<Stats*>(out_dev), as_stream(stream)));
+}
+
+int hpsp_average_on_noalloc(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed,
+                            uint64_t first, uint64_t n, int prec, hpsp_stats* out_dev,
+                            void* stream) {
+  if (n > HPSP_NOALLOC_MAX_EVENTS) return HPSP_ERR_WORKSPACE;
+  int rc = check_decay(d);
+  if (rc) return rc;
+  void* ws = nullptr;
+  rc = scratch_on_device(&ws);
+  if (rc) return rc;
+  return average_common(d, f, seed, first, n, nullptr, 0, prec, ws, sizeof(scratch), out_dev,
+                        stream);
 }
 
 int hpsp_average_on_mothers(const hpsp_decay* d, const hpsp_functor* f, uint64_t seed,

@@ -269,3 +269,10 @@ def test_plane_angle_only_over_stored_particles():
                                         None) == 1
     r = capi.make_range([5.279, 0, 0, 0], [3.0969, 0.493677, 0.13957])
     assert L.hpsp_histogram_fill_range(h, ctypes.byref(r), 1, f, 0, 10, 0, out, None) == 1
+
+
+def test_noalloc_limit_matches_header():
+    import re
+    text = open(capi.HEADER_PATH).read()
+    m = re.search(r"#define HPSP_NOALLOC_MAX_EVENTS \(1ULL << (\d+)\)", text)
+    assert m and capi.NOALLOC_MAX_EVENTS == 1 << int(m.group(1))
