@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--extras", type=int, default=1, help="also time the other workloads")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--only", default="",
-                    help="run one workload: c2, c3, c4, evaluate, histogram, range, c5 (serving), or copy (a plain 1 GiB device copy "
+                    help="run one workload: c2, c2_mixed, c3, c4, evaluate, histogram, range, c5 (serving), or copy (a plain 1 GiB device copy "
                          "that calibrates the profiler's byte counters)")
     return ap.parse_args()
 
@@ -229,6 +229,83 @@ def compare_cols(a, b, N, sample=1 << 20):
     return float(d.max()), float(torch.sqrt((d * d).mean()))
 
 
+def generate_both(torch, capi, M, masses, n, first=0):
+    """FP64 and mixed Generate of the same events: (cols64, wt64, evw64),
+    (cols32, wt_mixed, evw_mixed) as tensors."""
+    lib = capi.lib()
+    dec = capi.make_decay(M, masses)
+    N = len(masses)
+    sh = capi.stream_handle()
+    out = []
+    for prec, dt in ((capi.FP64, torch.float64), (capi.MIXED, torch.float32)):
+        cols = torch.empty((4 * N, n), dtype=dt, device="cuda:0")
+        wt = torch.empty(n, dtype=torch.float64, device="cuda:0")
+        evw = torch.empty(n, dtype=torch.float64, device="cuda:0")
+        capi.check(lib.hpsp_generate(ctypes.byref(dec), SEED, first, n, prec, capi.ptr(cols), n,
+                                     int(dt == torch.float32), capi.ptr(wt), capi.ptr(evw), sh),
+                   "generate")
+        out.append((cols, wt, evw))
+    torch.cuda.synchronize()
+    return out
+
+
+def rel_err(a, b):
+    """max and RMS of |a - b| / |b| over the entries where b != 0."""
+    import torch
+    d = (a.double() - b.double()).abs()
+    m = b.double().abs() > 0
+    r = d[m] / b.double().abs()[m]
+    return float(r.max()), float(torch.sqrt((r * r).mean()))
+
+
+def mixed_errors(torch, capi, M, masses, n):
+    """Errors of the mixed form against FP64 on the same n events: momenta
+    (|dp|/E per component), generation weight, PhaseSpaceWeight, and the
+    mean weight (the phase-space integral)."""
+    N = len(masses)
+    (c64, w64, e64), (c32, w32, e32) = generate_both(torch, capi, M, masses, n)
+    pmax, prms = compare_cols(c32, c64, N, sample=n)
+    wmax, wrms = rel_err(w32, w64)
+    emax, erms = rel_err(e32, e64)
+    i64, i32 = float(w64.double().mean()), float(w32.double().mean())
+    return {"sample_events": n, "momentum_err_max": pmax, "momentum_err_rms": prms,
+            "gen_weight_rel_err_max": wmax, "gen_weight_rel_err_rms": wrms,
+            "phsp_weight_rel_err_max": emax, "phsp_weight_rel_err_rms": erms,
+            "integral_rel_err": abs(i32 - i64) / abs(i64)}
+
+
+def run_c2_mixed(torch, capi, n=1 << 27):
+    """The headline 3-body decay, FP64 against mixed (FP32 columns): Generate
+    with the PhaseSpaceWeight column, and the momenta alone."""
+    N = 3
+    res = {"events": n}
+    for tag, with_w in (("generate", True), ("momenta_only", False)):
+        for prec, f32, name in ((capi.FP64, False, "fp64"), (capi.MIXED, True, "mixed")):
+            lib = capi.lib()
+            dec = capi.make_decay(M3, MASSES3)
+            cols = torch.empty((4 * N, n), dtype=torch.float32 if f32 else torch.float64,
+                               device="cuda:0")
+            evw = torch.empty(n, dtype=torch.float64, device="cuda:0") if with_w else None
+            sh = capi.stream_handle()
+            t = Timer(torch)
+            for i in range(6):
+                if i:
+                    t.start()
+                capi.check(lib.hpsp_generate(ctypes.byref(dec), SEED, 0, n, prec, capi.ptr(cols),
+                                             n, int(f32), None, capi.ptr(evw), sh), "generate")
+                if i:
+                    t.stop()
+            res[f"{tag}_{name}_ms"] = t.avg_ms
+            b = n * ((16 if f32 else 32) * N + (8 if with_w else 0))
+            res[f"{tag}_{name}_GBs"] = b / (t.avg_ms * 1e6)
+            del cols, evw
+            torch.cuda.empty_cache()
+        res[f"{tag}_mixed_speedup"] = res[f"{tag}_fp64_ms"] / res[f"{tag}_mixed_ms"]
+    res["errors"] = mixed_errors(torch, capi, M3, MASSES3, 1 << 22)
+    torch.cuda.empty_cache()
+    return res
+
+
 def run_c3(torch, capi):
     res = {}
     n = 1 << 26
@@ -248,6 +325,8 @@ def run_c3(torch, capi):
             "flops_per_event": fl,
         }
         del c64, cmx
+        torch.cuda.empty_cache()
+        res[tag]["errors"] = mixed_errors(torch, capi, 10.0, masses, 1 << 22)
         torch.cuda.empty_cache()
     return res
 
@@ -517,6 +596,48 @@ def serving_mix(count, seed=5):
     return out
 
 
+def served_mixed_errors(torch, capi, nreq=6000, seed=5, count=24, nmax=1 << 20):
+    """Errors of the served mix's mixed-precision requests against FP64 (the
+    first `count` of each kind, events capped at nmax).  A served request's
+    result is bit for bit the direct call's (tests/test_gpu_serve.py), so the
+    direct calls are compared here."""
+    import numpy as np
+    lib = capi.lib()
+    sh = capi.stream_handle()
+    gen, avg = [], []
+    for r in serving_mix(nreq, seed):
+        if not r["mixed"]:
+            continue
+        n = min(r["n"], nmax)
+        if r["avg"] and len(avg) < count:
+            dec = capi.make_decay(r["M"], r["m"])
+            terms = BW2 if r["N"] >= 3 else [(0.892, 0.05, 1.0, 0.3, 0, 1)]
+            f = capi.make_functor(capi.F_BW_SUM, terms)
+            ws_bytes = lib.hpsp_average_on_workspace_size(n)
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda:0")
+            out = torch.empty((2, 3), dtype=torch.float64, device="cuda:0")
+            for k, prec in enumerate((capi.FP64, capi.MIXED)):
+                o = ctypes.c_void_p(out.data_ptr() + 24 * k)
+                capi.check(lib.hpsp_average_on(ctypes.byref(dec), ctypes.byref(f), r["seed"], 0, n,
+                                               prec, capi.ptr(ws), ws_bytes, o, sh), "average_on")
+            st = out.cpu().numpy()
+            s64 = math.sqrt(st[0, 2]) / st[0, 0]
+            s32 = math.sqrt(st[1, 2]) / st[1, 0]
+            avg.append((abs(st[1, 1] - st[0, 1]) / abs(st[0, 1]), abs(s32 - s64) / s64))
+        elif not r["avg"] and len(gen) < count:
+            e = mixed_errors(torch, capi, r["M"], r["m"], n)
+            gen.append(e)
+        if len(gen) >= count and len(avg) >= count:
+            break
+    torch.cuda.empty_cache()
+    g = {k: float(max(e[k] for e in gen)) for k in gen[0] if k != "sample_events"}
+    return {"generate_requests": len(gen), "generate_max_over_requests": g,
+            "average_requests": len(avg),
+            "average_mean_rel_err_max": float(max(a[0] for a in avg)),
+            "average_sigma_rel_err_max": float(max(a[1] for a in avg)),
+            "events_cap": nmax}
+
+
 def run_c5(torch, capi, nreq=6000, rate=3000.0, seed=5):
     """Poisson arrivals into the continuous-batching server."""
     import numpy as np
@@ -714,9 +835,14 @@ def main():
             extras["range"]["cpu_baseline"] = cpu_baseline_range()
         print(json.dumps(extras), flush=True)
         return
+    if args.only == "c2_mixed":
+        extras["c2_mixed"] = run_c2_mixed(torch, capi)
+        print(json.dumps(extras), flush=True)
+        return
     if args.only == "c5":
         extras["c5"] = run_c5(torch, capi)
         extras["c5_saturated"] = run_c5(torch, capi, rate=50000.0)
+        extras["c5_mixed_errors"] = served_mixed_errors(torch, capi)
         print(json.dumps(extras), flush=True)
         return
     if args.only == "copy":
@@ -736,6 +862,8 @@ def main():
     c2 = run_c2(torch, capi, args.n, args.steps, args.warmup)
     print(f"# c2 done: {c2['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     if args.extras and not args.only:
+        extras["c2_mixed"] = run_c2_mixed(torch, capi)
+        print("# c2 mixed done", file=sys.stderr, flush=True)
         extras["c3"] = run_c3(torch, capi)
         print("# c3 done", file=sys.stderr, flush=True)
         extras["c4"] = run_c4(torch, capi)
@@ -747,6 +875,7 @@ def main():
         extras["range"] = run_range(torch, capi)
         print("# range done", file=sys.stderr, flush=True)
         extras["c5"] = run_c5(torch, capi)
+        extras["c5"]["mixed_errors"] = served_mixed_errors(torch, capi)
         print("# c5 done", file=sys.stderr, flush=True)
     extras["c2"] = c2
 

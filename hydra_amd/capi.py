@@ -169,6 +169,7 @@ _SIGS = {
     "hpsp_range_generate": (_I, [ctypes.POINTER(Decay), _U64, _U64, _U64, _P, _U64, _P, _P]),
     "hpsp_histogram_fill_range": (_I, [ctypes.POINTER(Histogram), ctypes.POINTER(Decay), _U64,
                                        ctypes.POINTER(Functor), _U64, _U64, _I, _P, _P]),
+    "hpsp_checksum": (_I, [_P, _U64, _P, _P]),
     "hpsp_malloc": (_I, [ctypes.POINTER(_P), _SZ]),
     "hpsp_free": (_I, [_P]),
     "hpsp_memcpy_to_device": (_I, [_P, _P, _SZ]),

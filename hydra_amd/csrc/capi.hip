@@ -753,6 +753,12 @@ int hpsp_average_on_mothers(const hpsp_decay* d, const hpsp_functor* f, uint64_t
   return average_common(d, f, seed, 0, n, mothers, mld, prec, ws, ws_bytes, out_dev, stream);
 }
 
+int hpsp_checksum(const void* data, uint64_t bytes, uint64_t* sum_dev, void* stream) {
+  if (!sum_dev || (bytes % 8) != 0 || (bytes > 0 && !data)) return HPSP_ERR_INVALID_ARGUMENT;
+  return hip_status(launch_checksum(static_cast<const uint64_t*>(data), bytes / 8, sum_dev,
+                                    as_stream(stream)));
+}
+
 int hpsp_malloc(void** ptr, size_t bytes) {
   if (!ptr) return HPSP_ERR_INVALID_ARGUMENT;
   return hip_status((int)hipMalloc(ptr, bytes ? bytes : 1));

@@ -9,6 +9,7 @@
 #include <type_traits>
 
 #include "hpsp.h"
+#include "sincos_table.h"
 
 namespace hpsp {
 
@@ -388,42 +389,33 @@ struct WaveWords {
 };
 
 // ---------------------------------------------------------------------------
-// sin and cos of 2*pi*u for u in [0, 1].  The quadrant is taken in u itself
-// (r = u - q/4 is exact because u and q/4 are multiples of one power of two
-// and within a factor two of each other), the remaining angle x = 2*pi*r with
-// |x| <= pi/4 is formed with a two-term 2*pi, and sin/cos of x come from
-// their Taylor series (terms to x^17 / x^16; the first omitted terms are
-// below 1e-19 on |x| <= pi/4), evaluated by Horner's rule with FMA.
-// The reference evaluates sin/cos of the rounded angle fl(2*pi*u); the two
-// agree to a few 1e-16 absolute.
+// sin and cos of 2*pi*u for u in [0, 1].  u = (j + d) / 256 with j = floor(256 u)
+// and d in [0, 1) (both exact: 256 u is u scaled by a power of two); the
+// angle is 2*pi*j/256 + x with x = d * pi/128 in [0, pi/128) (a two-term
+// pi/128), so sin and cos are the table's correctly rounded values at
+// 2*pi*j/256 turned by x: s = sa*cx + ca*sx, c = ca*cx - sa*sx, with sin x
+// and cos x from their Taylor series to x^7 / x^6 (the first omitted terms
+// are below 1e-20 on [0, pi/128)).  The result is within a few 1e-16 of
+// sin/cos(2*pi*u); the reference evaluates sin/cos of the rounded angle
+// fl(2*pi*u), which differs from 2*pi*u by at most half an ulp.
 __device__ __forceinline__ void sincos_2pi(double u, double& s, double& c) {
-  const double qf = __builtin_rint(4.0 * u);
-  const double r = __builtin_fma(-0.25, qf, u);  // exact
-  const double x = __builtin_fma(r, 6.283185307179586, r * 2.4492935982947064e-16);
+#pragma clang fp contract(off)
+  const double t = u * 256.0;
+  const double jf = __builtin_floor(t);
+  const double d = t - jf;
+  const double x = __builtin_fma(d, 0.02454369260617026, d * 9.567553118338697e-19);
+  const int j = (int)jf;  // 0..256
+  const double sa = kSinCos256[j][0];
+  const double ca = kSinCos256[j][1];
   const double x2 = x * x;
-  double ps = 2.8114572543455206e-15;
-  ps = __builtin_fma(ps, x2, -7.647163731819816e-13);
-  ps = __builtin_fma(ps, x2, 1.6059043836821613e-10);
-  ps = __builtin_fma(ps, x2, -2.505210838544172e-08);
-  ps = __builtin_fma(ps, x2, 2.7557319223985893e-06);
-  ps = __builtin_fma(ps, x2, -0.0001984126984126984);
-  ps = __builtin_fma(ps, x2, 0.008333333333333333);
-  ps = __builtin_fma(ps, x2, -0.16666666666666666);
+  double ps = __builtin_fma(x2, -1.984126984126984e-04, 8.333333333333333e-03);
+  ps = __builtin_fma(x2, ps, -0.16666666666666666);
   const double sx = __builtin_fma(x * x2, ps, x);
-  double pc = 4.779477332387385e-14;
-  pc = __builtin_fma(pc, x2, -1.1470745597729725e-11);
-  pc = __builtin_fma(pc, x2, 2.08767569878681e-09);
-  pc = __builtin_fma(pc, x2, -2.755731922398589e-07);
-  pc = __builtin_fma(pc, x2, 2.48015873015873e-05);
-  pc = __builtin_fma(pc, x2, -0.001388888888888889);
-  pc = __builtin_fma(pc, x2, 0.041666666666666664);
-  pc = __builtin_fma(pc, x2, -0.5);
+  double pc = __builtin_fma(x2, -1.388888888888889e-03, 4.1666666666666664e-02);
+  pc = __builtin_fma(x2, pc, -0.5);
   const double cx = __builtin_fma(x2, pc, 1.0);
-  const int q = ((int)qf) & 3;
-  const double a = (q & 1) ? cx : sx;   // |sin| candidate
-  const double b = (q & 1) ? sx : cx;   // |cos| candidate
-  s = (q & 2) ? -a : a;
-  c = ((q + 1) & 2) ? -b : b;
+  s = __builtin_fma(sa, cx, ca * sx);
+  c = __builtin_fma(ca, cx, -(sa * sx));
 }
 
 // ---------------------------------------------------------------------------
@@ -575,10 +567,52 @@ __device__ __forceinline__ Real dot4(const Real (&a)[4], const Real (&b)[4]) {
   return a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
 }
 
+// One of K scalars for a run-time (wave-uniform) index, through bit masks.
+template <typename Real, int K>
+__device__ __forceinline__ Real pick1(const Real (&v)[K], int i) {
+  using U = std::conditional_t<sizeof(Real) == 8, uint64_t, uint32_t>;
+  U x = __builtin_bit_cast(U, v[0]);
+#pragma unroll
+  for (int j = 1; j < K; ++j) {
+    const U m = (U)0 - (U)(i == j);
+    x = (x & ~m) | (__builtin_bit_cast(U, v[j]) & m);
+  }
+  return __builtin_bit_cast(Real, x);
+}
+
+// (p[a] + p[b])^2, the pair's invariant mass squared (Vector4R.inl:56-60),
+// with the sum taken component by component (a + b and b + a round alike).
+template <typename Real>
+__device__ __forceinline__ Real pair_mass2(const Real (&u)[4], const Real (&w)[4]) {
+#pragma clang fp contract(off)
+  Real v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = u[c] + w[c];
+  return v[0] * v[0] - v[1] * v[1] - v[2] * v[2] - v[3] * v[3];
+}
+
 // Four-vectors of one event in registers (the generation kernels).
 template <typename Real, int P>
 struct RegAcc {
   const Real (&p)[P][4];
+  // With at most four four-vectors, the functors of particle pairs take the
+  // pair masses from a table of all P(P-1)/2 pairs formed once per event
+  // (the pair named by wave-uniform indices is then one of 3 or 6 scalars)
+  // instead of selecting two four-vectors per term.
+  static constexpr bool kPairTable = P >= 2 && P <= 4;
+  static constexpr int kPairs = P * (P - 1) / 2;
+  __device__ __forceinline__ void pair_table(Real (&m2)[kPairs > 0 ? kPairs : 1]) const {
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+#pragma unroll
+      for (int b = a + 1; b < P; ++b) m2[k++] = pair_mass2(p[a], p[b]);
+  }
+  // index in pair_table of the pair (a, b), a != b
+  __device__ __forceinline__ static int pair_index(int a, int b) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return lo * (2 * P - lo - 1) / 2 + (hi - lo - 1);
+  }
   __device__ __forceinline__ void get(int i, Real (&v)[4]) const { pick4(p, i, v); }
   // p[first] + ... + p[last - 1], left to right, through the unrolled array
   __device__ __forceinline__ void sum(int first, int last, Real (&Pv)[4]) const {
@@ -621,6 +655,27 @@ __device__ __forceinline__ void unit3(Real (&v)[4]) {
   for (int c = 0; c < 4; ++c) v[c] *= inv;
 }
 
+// Whether an accessor offers the table of pair masses (RegAcc with P <= 4).
+template <typename Acc, typename = void>
+struct has_pair_table : std::false_type {};
+template <typename Acc>
+struct has_pair_table<Acc, std::void_t<decltype(Acc::kPairTable)>>
+    : std::integral_constant<bool, Acc::kPairTable> {};
+template <typename Acc, bool = has_pair_table<Acc>::value>
+struct pair_table_size : std::integral_constant<int, 1> {};
+template <typename Acc>
+struct pair_table_size<Acc, true> : std::integral_constant<int, Acc::kPairs> {};
+template <typename Real, typename Acc>
+__device__ __forceinline__ Real pair_from_table(const Acc& acc, int a, int b) {
+  if constexpr (has_pair_table<Acc>::value) {
+    Real tab[Acc::kPairs];
+    acc.pair_table(tab);
+    return pick1(tab, Acc::pair_index(a, b));
+  } else {
+    return Real(0);
+  }
+}
+
 // Functor value of one event; acc.get(i, v) yields four-vector i and
 // acc.sum(first, last, P) the left-to-right sum of four-vectors [first, last).
 // The decay-plane angle (atan2 and two normalisations) is compiled only where
@@ -631,12 +686,17 @@ __device__ __forceinline__ double functor_eval_acc(const FunctorK& f, const Acc&
 #pragma clang fp contract(off)
   if (f.kind == HPSP_F_ONE) return 1.0;
   if (f.kind == HPSP_F_MASS2_01 || f.kind == HPSP_F_MASS2 || f.kind == HPSP_F_MASS) {
-    Real u[4], v[4], s[4];
-    acc.get(f.kind == HPSP_F_MASS2_01 ? 0 : f.a[0], u);
-    acc.get(f.kind == HPSP_F_MASS2_01 ? 1 : f.b[0], v);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) s[c] = u[c] + v[c];
-    const Real m2 = dot4(s, s);
+    const int ia = f.kind == HPSP_F_MASS2_01 ? 0 : f.a[0];
+    const int ib = f.kind == HPSP_F_MASS2_01 ? 1 : f.b[0];
+    Real m2;
+    if (has_pair_table<Acc>::value && ia != ib) {
+      m2 = pair_from_table<Real>(acc, ia, ib);
+    } else {
+      Real u[4], v[4];
+      acc.get(ia, u);
+      acc.get(ib, v);
+      m2 = pair_mass2(u, v);
+    }
     if (f.kind != HPSP_F_MASS) return (double)m2;
     // Vector4R::mass: 0 unless the square is positive
     if constexpr (sizeof(Real) == 4)
@@ -704,13 +764,23 @@ __device__ __forceinline__ double functor_eval_acc(const FunctorK& f, const Acc&
     return chi;
   }
   Real Ar = 0, Ai = 0;
+  Real tab[pair_table_size<Acc>::value];
+  if constexpr (has_pair_table<Acc>::value) acc.pair_table(tab);
   for (int k = 0; k < f.nterms; ++k) {
-    Real u[4], w[4], v[4];
-    acc.get(f.a[k], u);
-    acc.get(f.b[k], w);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) v[c] = u[c] + w[c];
-    const Real s = v[0] * v[0] - v[1] * v[1] - v[2] * v[2] - v[3] * v[3];
+    Real s = 0;
+    bool tabled = false;
+    if constexpr (has_pair_table<Acc>::value) {
+      if (f.a[k] != f.b[k]) {
+        s = pick1(tab, Acc::pair_index(f.a[k], f.b[k]));
+        tabled = true;
+      }
+    }
+    if (!tabled) {
+      Real u[4], w[4];
+      acc.get(f.a[k], u);
+      acc.get(f.b[k], w);
+      s = pair_mass2(u, w);
+    }
     const Real M = (Real)f.m[k], G = (Real)f.g[k];
     const Real dr = M * M - s;
     const Real di = -(M * G);

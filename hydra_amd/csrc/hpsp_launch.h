@@ -205,6 +205,8 @@ int launch_event_weight(int N, const WeightArgs& a, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s);
+// order-independent checksum of n 64-bit words into *sum (device)
+int launch_checksum(const uint64_t* w, uint64_t n, uint64_t* sum, hipStream_t s);
 // largest of n values (max_element's value) into *out, through per-block maxima
 int launch_max_column(const double* w, uint64_t n, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);

@@ -462,6 +462,32 @@ __global__ __launch_bounds__(kBlock) void k_max_of_blocks(const double* block_ma
 
 __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
 
+// Order-independent checksum of n 64-bit words: the sum of a mix of each word
+// with its index (a write that changes any word changes it with probability
+// 1 - 2^-64).  One partial per block, added atomically.
+__global__ __launch_bounds__(kBlock) void k_checksum(const uint64_t* w, uint64_t n, uint64_t* sum) {
+  __shared__ uint64_t sh[kBlock / kWave];
+  uint64_t acc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    uint64_t z = w[i] ^ (i * 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    acc += z ^ (z >> 31);
+  }
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) acc += __shfl_xor(acc, off, kWave);
+  if (__lane_id() == 0) sh[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd((unsigned long long*)sum, (unsigned long long)(sh[0] + sh[1] + sh[2] + sh[3]));
+}
+
+int launch_checksum(const uint64_t* w, uint64_t n, uint64_t* sum, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, sum, (uint64_t)0);
+  if (n) hipLaunchKernelGGL(k_checksum, dim3(grid_for(n, 8192)), dim3(kBlock), 0, s, w, n, sum);
+  return (int)hipGetLastError();
+}
+
 constexpr int kNumN = HPSP_MAX_N - 1;
 
 static void launch_flags(int N, const UnweightArgs& a, uint64_t ntiles, hipStream_t s);

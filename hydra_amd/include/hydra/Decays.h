@@ -13,6 +13,7 @@
 #pragma once
 
 #include <array>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <tuple>
@@ -343,7 +344,10 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
 
   // ---- unweighting (Decays.inl:415-511)
   Range<iterator> Unweight(size_t seed = 0x180ec6d33cfd0aba) {
-    if (fGenValid && fWValid) return unweight_regenerate(seed);
+    if (fGenValid && fWValid) {
+      if (fCheck) guard_verify();
+      return unweight_regenerate(seed);
+    }
     const hpsp_decay d = detail::make_decay<N>(fMotherMass, fMasses, "Decays::Unweight");
     return unweight(d, nullptr, -1.0, seed);
   }
@@ -427,7 +431,22 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     fGen = d;
     fGenSeed = seed;
     fGenValid = true;
+    if (fCheck) guard_record();
   }
+
+  // Opt-in check of the deferred path (also on when the environment variable
+  // HYDRA_DEFERRED_CHECKS is set to 1 when the container is made): a
+  // checksum of the columns taken when Generate marks them as its events
+  // (and after deferred momenta are formed) is compared before Unweight()
+  // regenerates them; a write through a column pointer taken before the
+  // Generate, which the regeneration would silently discard, throws
+  // std::logic_error instead.  Costs two reads of the columns per
+  // Generate/Unweight pair.
+  void SetDeferredChecks(bool on) {
+    fCheck = on;
+    if (on && fGenValid) guard_record();
+  }
+  bool DeferredChecks() const { return fCheck; }
   bool generated() const { return fGenValid; }
 
   // PhaseSpaceWeight column kept alongside the momenta (null when unknown).
@@ -440,6 +459,29 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   static value_type make_row(const std::array<double, 4 * N>& v, std::index_sequence<I...>) {
     // column (4*i + c) -> row entry i
     return value_type(Vector4R(v[4 * I], v[4 * I + 1], v[4 * I + 2], v[4 * I + 3])...);
+  }
+
+  static bool checks_from_env() {
+    const char* e = std::getenv("HYDRA_DEFERRED_CHECKS");
+    return e && e[0] == '1';
+  }
+
+  uint64_t columns_checksum() const {
+    detail::device_buffer<uint64_t> sum(1);
+    uint64_t h = 0;
+    detail::hpsp_check(hpsp_checksum(fCols.data(), (uint64_t)(4 * N * fCap * sizeof(double)),
+                                     sum.data(), nullptr),
+                       "Decays (deferred checks)");
+    detail::hpsp_check(hpsp_memcpy_to_host(&h, sum.data(), sizeof(h)), "Decays (deferred checks)");
+    return h;
+  }
+  void guard_record() const { fGuard = columns_checksum(); }
+  void guard_verify() const {
+    if (columns_checksum() != fGuard)
+      throw std::logic_error(
+          "[hydra::Decays::Unweight]: the columns of Generate's events were written through a "
+          "pointer taken before that Generate; Unweight() regenerates those events and would "
+          "discard the writes (take column()/data() after Generate to write momenta)");
   }
 
   void init(size_t n) {
@@ -462,6 +504,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
       fReadAfterGen = true;
     }
     fLazy = false;
+    if (fCheck && fGenValid) guard_record();
   }
 
   void regrow(size_t cap) {
@@ -610,6 +653,8 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   bool fGenValid = false;
   mutable bool fLazy = false;
   mutable bool fReadAfterGen = false;
+  bool fCheck = checks_from_env();
+  mutable uint64_t fGuard = 0;
 };
 
 // Decays in host memory (the reference's Decays<tuple<P...>, host::sys_t>):

@@ -215,6 +215,34 @@ int main(int argc, char** argv) {
           same6 = 0;
       std::printf("\"unweight_after_write\": [%d, %d],\n", cache_before, same6);
     }
+    // the opt-in deferred-path check: a write through a column pointer taken
+    // before Generate is caught at Unweight(); an untouched container passes
+    {
+      hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> G(P_mass, masses, n);
+      double* stale = G.column(0, 1);
+      G.SetDeferredChecks(true);
+      phsp.Generate(Parent, G);
+      detail_copy(stale + 5, stale + 7, 1);
+      int caught = 0;
+      try {
+        G.Unweight();
+      } catch (std::logic_error const&) {
+        caught = 1;
+      }
+      hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> H(P_mass, masses, n);
+      H.SetDeferredChecks(true);
+      phsp.Generate(Parent, H);
+      (void)H[3];  // forms the deferred momenta: the checksum follows
+      int passed = 1;
+      size_t nacc_h = 0;
+      try {
+        nacc_h = H.Unweight().size();
+      } catch (std::logic_error const&) {
+        passed = 0;
+      }
+      std::printf("\"deferred_checks\": [%d, %d, %d],\n", caught, passed,
+                  nacc_h == accepted.size() ? 1 : 0);
+    }
     // Events was read after its deferred Generate: the next Generate into it
     // writes momenta and weights in one pass, with the same bits, and its
     // Unweight() still regenerates (the result of E2's above)
