@@ -24,6 +24,7 @@ import ctypes
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -68,7 +69,7 @@ def parse():
     ap.add_argument("--extras", type=int, default=1, help="also time the other workloads")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--only", default="",
-                    help="run one workload: c2, c2_mixed, c3, c4, evaluate, histogram, range, c5 (serving), or copy (a plain 1 GiB device copy "
+                    help="run one workload: c2, c2_cpp (the step through the C++ API), c2_mixed, c3, c4, evaluate, histogram, range, c5 (serving), or copy (a plain 1 GiB device copy "
                          "that calibrates the profiler's byte counters)")
     return ap.parse_args()
 
@@ -272,6 +273,22 @@ def mixed_errors(torch, capi, M, masses, n):
             "gen_weight_rel_err_max": wmax, "gen_weight_rel_err_rms": wrms,
             "phsp_weight_rel_err_max": emax, "phsp_weight_rel_err_rms": erms,
             "integral_rel_err": abs(i32 - i64) / abs(i64)}
+
+
+def run_c2_cpp(n, steps, warmup):
+    """The headline step through the C++ drop-in (tests/cpp/bench_decays:
+    `phsp.Generate(B0, Events); Events.Unweight()` on one device Decays, timed
+    on the host clock around each step, the accepted count copied back as the
+    reference's Range size is).  Reported beside the C ABI's line: the same
+    kernels plus the C++ API's host work and synchronisation."""
+    exe = os.path.join(ROOT, "tests", "cpp", "bin", "bench_decays")
+    if not os.path.exists(exe):
+        return {"skipped": "tests/cpp/bin/bench_decays not built"}
+    out = subprocess.run([exe, str(n), str(steps), str(warmup)], check=True,
+                         capture_output=True, text=True, timeout=600)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    r["events_per_s"] = n / (r["ms_per_step"] * 1e-3)
+    return r
 
 
 def run_c2_mixed(torch, capi, n=1 << 27):
@@ -835,6 +852,10 @@ def main():
             extras["range"]["cpu_baseline"] = cpu_baseline_range()
         print(json.dumps(extras), flush=True)
         return
+    if args.only == "c2_cpp":
+        extras["c2_cpp"] = run_c2_cpp(args.n, args.steps, args.warmup)
+        print(json.dumps(extras), flush=True)
+        return
     if args.only == "c2_mixed":
         extras["c2_mixed"] = run_c2_mixed(torch, capi)
         print(json.dumps(extras), flush=True)
@@ -862,6 +883,8 @@ def main():
     c2 = run_c2(torch, capi, args.n, args.steps, args.warmup)
     print(f"# c2 done: {c2['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     if args.extras and not args.only:
+        extras["c2_cpp"] = run_c2_cpp(args.n, args.steps, args.warmup)
+        print("# c2 through the C++ API done", file=sys.stderr, flush=True)
         extras["c2_mixed"] = run_c2_mixed(torch, capi)
         print("# c2 mixed done", file=sys.stderr, flush=True)
         extras["c3"] = run_c3(torch, capi)

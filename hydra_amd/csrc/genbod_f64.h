@@ -74,7 +74,10 @@ __device__ __forceinline__ double genbod_f64(const DecayK& d, const WordFn& word
     const double sZ = sqrt_fast(1 - cZ * cZ);
     // sin/cos of angY = 2*pi*u
     double sY, cY;
-    sincos_2pi(to_unit<ENGINE>(word(N - 2 + 2 * (i - 1) + 1)), sY, cY);
+    if constexpr (is_squares<ENGINE>())
+      sincos_2pi_w32((uint32_t)word(N - 2 + 2 * (i - 1) + 1), sY, cY);
+    else
+      sincos_2pi(to_unit<ENGINE>(word(N - 2 + 2 * (i - 1) + 1)), sY, cY);
 
 #pragma unroll
     for (int j = 0; j <= i; ++j) {

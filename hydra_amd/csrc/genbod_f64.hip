@@ -93,7 +93,9 @@ void k_generate_f64(const GenArgs a) {
 // never read back: 32N bytes written per event (+16 with the weight column)
 // instead of 64N moved.
 //
-// One group of 64 events per wave, as in k_generate_f64.  The block's 256
+// One group of 64 events per wave (with the squares engines the block's
+// 256 + W - 1 engine words are evaluated once each and shared through LDS, so
+// an event costs one word, as in k_generate_f64's runs).  The block's 256
 // events fall into two contiguous runs of the output, its accepted events and
 // its rejected ones; for N <= 6 they are staged in LDS in partition order and
 // every column is written by all 256 threads as those two runs, so that only
@@ -103,6 +105,14 @@ void k_generate_f64(const GenArgs a) {
 // neighbouring blocks, which write neighbouring pieces of both runs, meet in
 // one L2.  A block's offset comes from the tile offset and a 16-lane scan of
 // the popcounts of its tile's flag words (block_part).
+// Squares words of one event from the block's words in LDS: thread t's own
+// word (counter c0 + t) and words j >= 1 at w[j], w = sw + t.
+struct BlockWords {
+  const uint32_t* w;
+  uint32_t own;
+  __device__ __forceinline__ uint64_t operator()(int j) const { return j == 0 ? own : w[j]; }
+};
+
 template <int N, int ENGINE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
 void k_uw_regen(const UnweightArgs a) {
@@ -112,6 +122,9 @@ void k_uw_regen(const UnweightArgs a) {
   // LDS (12 KB at N = 3) does not cap the waves in flight below the registers
   constexpr int CS = !kStage ? 1 : (C <= 8 ? C : C / 2);
   __shared__ double sh[CS][kStage ? kBlock : 1];
+  // squares engines: the block's engine words (see BlockWords)
+  constexpr int W = 3 * N - 4;
+  __shared__ uint32_t sw[is_squares<ENGINE>() ? kBlock + W - 1 : 1];
   constexpr int kWavesPerBlock = kBlock / kWave;
   const uint64_t ngroups = (a.n + kWave - 1) / kWave;
   const uint64_t nblocks = (ngroups + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -130,13 +143,29 @@ void k_uw_regen(const UnweightArgs a) {
   const uint64_t k0 = g * kWave;
   const uint64_t k = k0 + (uint64_t)lane;
   const bool live_wave = g < ngroups;  // wave-uniform
+  uint32_t own = 0;
+  if constexpr (is_squares<ENGINE>()) {
+    // event kb + t reads the words at counters c0 + t + j, j < W: thread t
+    // evaluates the one at c0 + t, threads t < W - 1 also the one at c0 + 256 + t
+    const uint64_t c0 = a.gen_first + kb + 3 * (uint64_t)N;
+    const uint32_t t = threadIdx.x;
+    own = squares_word_y<ENGINE>(a.d.key[0], a.d.key[0] * (c0 + t));
+    sw[t] = own;
+    if (t < (uint32_t)(W - 1))
+      sw[kBlock + t] = squares_word_y<ENGINE>(a.d.key[0], a.d.key[0] * (c0 + kBlock + t));
+    __syncthreads();
+  }
   double p[N][4];
   uint32_t r = 0;  // the event's rank in the block's partition order
   if (live_wave) {
     const uint64_t fw = bp.fw;
-    EventWords<N, ENGINE> words;
-    words.init(a.d, a.gen_first + k0, lane);
-    genbod_f64<N, ENGINE>(a.d, words, p);
+    if constexpr (is_squares<ENGINE>()) {
+      genbod_f64<N, ENGINE>(a.d, BlockWords{sw + threadIdx.x, own}, p);
+    } else {
+      EventWords<N, ENGINE> words;
+      words.init(a.d, a.gen_first + k0, lane);
+      genbod_f64<N, ENGINE>(a.d, words, p);
+    }
     if (a.d.do_boost) {
 #pragma unroll
       for (int i = 0; i < N; ++i) boost_folded(p[i], a.d);

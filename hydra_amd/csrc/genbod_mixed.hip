@@ -5,7 +5,8 @@
 // agree with the FP64 ones to a few ulp.  The kinematics after pdk
 // (energies, rotations, boosts, functor) run in FP32 with FMA contraction,
 // hardware sin/cos in revolutions, square roots and reciprocals.
-// Accumulation of AverageOn stays FP64.
+// Accumulation of AverageOn stays FP64: shifted sums per lane, then the
+// StatsPHSP merges of the FP64 kernel.
 #include <array>
 #include <type_traits>
 #include <utility>
@@ -26,12 +27,14 @@ __device__ __forceinline__ uint32_t word32(uint64_t w) {
 
 // cos and sin of the polar angle from u = w / 2^32 without cancellation:
 // cZ = 2u - 1 = (2w - 2^32) / 2^32 and sZ = sqrt(1 - cZ^2) = 2 sqrt(w (2^32 - w)) / 2^32,
-// both formed from exact integers and rounded once to FP32.
+// both formed from exact integers and rounded once to FP32.  Only 32-bit
+// conversions: w - 2^31 is w ^ 2^31 read as signed, and 2^32 - w is 0 - w in
+// 32 bits except at w = 0, where a = 0 makes the product 0 either way (the
+// same bits as the 64-bit forms for every w).
 __device__ __forceinline__ void polar_f32(uint32_t w, float& cZ, float& sZ) {
-  const int64_t t = 2 * (int64_t)w - (int64_t)(1ULL << 32);
-  cZ = (float)t * 0x1p-32f;
+  cZ = (float)(int32_t)(w ^ 0x80000000u) * 0x1p-31f;
   const float a = (float)w;
-  const float b = (float)((1ULL << 32) - (uint64_t)w);
+  const float b = (float)(0u - w);
   sZ = __builtin_amdgcn_sqrtf(a * b) * 0x1p-31f;
 }
 
@@ -281,6 +284,32 @@ __global__ __launch_bounds__(kBlock) void k_generate_mixed(const GenArgs a) {
   }
 }
 
+// One lane's events of the mixed AverageOn in FP64: the sums of w, w d and
+// w d^2 with d = f - K, K the lane's first value (the shifted-data form of
+// the weighted variance, exact for a constant functor), with no division per
+// event; stats() turns them into the (w, mean, m2) that the StatsPHSP merges
+// of the block and chunk stages take.
+struct ShiftedSums {
+  double k = 0.0, w = 0.0, s1 = 0.0, s2 = 0.0;
+  __device__ __forceinline__ void push(double f, double wt) {
+#pragma clang fp contract(off)
+    if (w == 0.0) k = f;
+    const double d = f - k;
+    const double wd = wt * d;
+    w += wt;
+    s1 += wd;
+    s2 = __builtin_fma(wd, d, s2);
+  }
+  __device__ __forceinline__ Stats stats() const {
+#pragma clang fp contract(off)
+    if (w == 0.0) return Stats{0.0, 0.0, 0.0};
+    const double r = s1 / w;
+    const double m2 = s2 - s1 * r;
+    // rounding can leave a tiny negative sum of squares; NaN passes on
+    return Stats{w, k + r, m2 < 0.0 ? 0.0 : m2};
+  }
+};
+
 template <int N, int ENGINE, bool MOTHERS>
 __global__ __launch_bounds__(kBlock) void k_average_mixed(const AvgArgs a) {
   __shared__ Stats sh[kBlock / kWave];
@@ -288,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_average_mixed(const AvgArgs a) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const uint64_t nchunks = (a.n + kAvgChunk - 1) / kAvgChunk;
   for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    Stats acc = {0.0, 0.0, 0.0};
+    ShiftedSums acc;
     // wave w folds groups 16w .. 16w+15 of the chunk in order (as the FP64
     // kernel), carrying the squares words from group to group
     constexpr int kGroupsPerWave = kAvgChunk / kBlock;
@@ -326,10 +355,10 @@ __global__ __launch_bounds__(kBlock) void k_average_mixed(const AvgArgs a) {
         final_boost_f32<N>(p, a.d);
       }
       const double f = functor_eval<float, false>(a.f, p);
-      if (live) stats_push(acc, f, wt);
+      if (live) acc.push(f, wt);
     }
-    acc = wave_reduce_stats(acc);
-    if (lane == 0) sh[wave] = acc;
+    const Stats st = wave_reduce_stats(acc.stats());
+    if (lane == 0) sh[wave] = st;
     __syncthreads();
     if (threadIdx.x == 0)
       a.partials[c] = stats_merge(stats_merge(sh[0], sh[1]), stats_merge(sh[2], sh[3]));

@@ -398,13 +398,8 @@ struct WaveWords {
 // are below 1e-20 on [0, pi/128)).  The result is within a few 1e-16 of
 // sin/cos(2*pi*u); the reference evaluates sin/cos of the rounded angle
 // fl(2*pi*u), which differs from 2*pi*u by at most half an ulp.
-__device__ __forceinline__ void sincos_2pi(double u, double& s, double& c) {
+__device__ __forceinline__ void sincos_turn(int j, double x, double& s, double& c) {
 #pragma clang fp contract(off)
-  const double t = u * 256.0;
-  const double jf = __builtin_floor(t);
-  const double d = t - jf;
-  const double x = __builtin_fma(d, 0.02454369260617026, d * 9.567553118338697e-19);
-  const int j = (int)jf;  // 0..256
   const double sa = kSinCos256[j][0];
   const double ca = kSinCos256[j][1];
   const double x2 = x * x;
@@ -416,6 +411,26 @@ __device__ __forceinline__ void sincos_2pi(double u, double& s, double& c) {
   const double cx = __builtin_fma(x2, pc, 1.0);
   s = __builtin_fma(sa, cx, ca * sx);
   c = __builtin_fma(ca, cx, -(sa * sx));
+}
+
+__device__ __forceinline__ void sincos_2pi(double u, double& s, double& c) {
+#pragma clang fp contract(off)
+  const double t = u * 256.0;
+  const double jf = __builtin_floor(t);
+  const double d = t - jf;
+  const double x = __builtin_fma(d, 0.02454369260617026, d * 9.567553118338697e-19);
+  sincos_turn((int)jf, x, s, c);  // j in 0..256
+}
+
+// The same for u = w / 2^32 (a squares engine's word): j = w >> 24 and
+// d = (w mod 2^24) / 2^24 taken from the word, the 2^-24 folded into the two
+// constants of x (powers of two scale exactly: the same bits as sincos_2pi).
+__device__ __forceinline__ void sincos_2pi_w32(uint32_t w, double& s, double& c) {
+#pragma clang fp contract(off)
+  const double dd = (double)(w & 0xFFFFFFu);
+  const double x = __builtin_fma(dd, 0.02454369260617026 * 0x1p-24,
+                                 dd * (9.567553118338697e-19 * 0x1p-24));
+  sincos_turn((int)(w >> 24), x, s, c);
 }
 
 // ---------------------------------------------------------------------------

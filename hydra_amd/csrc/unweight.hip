@@ -33,6 +33,26 @@ __device__ __forceinline__ void load_event_uw(const double* cols, uint64_t ld, u
     for (int q = 0; q < 4; ++q) p[i][q] = cols[(uint64_t)(4 * i + q) * ld + k];
 }
 
+// RN(w / maxv) > u, the reference's test, decided without a division where the
+// product with the reciprocal settles it.  rmax = RN(1 / maxv) and
+// q = RN(w * rmax) put q within 2^-51.3 |q| of w / maxv's rounded value
+// (relative errors 2^-53 each, rmax and q normal), so when |q - u| exceeds
+// 2^-50 |q| the rounded quotient lies strictly on q's side of u.  Otherwise
+// (and for zero, subnormal, infinite or NaN operands) the division decides:
+// the same flag as the division in every case.
+__device__ __forceinline__ bool ratio_above(double w, double maxv, double rmax, bool rmax_ok,
+                                            double u) {
+#pragma clang fp contract(off)
+  const double q = w * rmax;
+  const double gap = q - u;
+  const double aq = __builtin_fabs(q);
+  const bool settled = rmax_ok && aq >= 0x1p-1000 && aq <= 0x1p+1000 &&
+                       __builtin_fabs(gap) > aq * 0x1p-50;
+  bool r = gap > 0.0;
+  if (!settled) r = (w / maxv) > u;
+  return r;
+}
+
 // N = 0: the weights come from a.w_in alone (no functor), and the kernel
 // holds no momenta: few registers, many waves in flight for the one stream it
 // reads.  Tiles are dealt to XCDs in contiguous ranges.
@@ -46,6 +66,8 @@ __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
   const uint64_t tile = xcd_tile(ntiles);
   if (tile >= ntiles) return;  // whole block
   const double maxv = a.maxv_dev ? *a.maxv_dev : a.maxv;
+  const double rmax = 1.0 / maxv;
+  const bool rmax_ok = __builtin_fabs(rmax) >= 0x1p-1000 && __builtin_fabs(rmax) <= 0x1p+1000;
   constexpr bool w_only = (N == 0);
   // with a weight column, the wave's four loads are issued before any flag
   // word is stored
@@ -81,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_uw_flags(const UnweightArgs a) {
       const double u = (double)squares3_y(a.uw_key, ykey) * 0x1p-32;  // ctr = jump + k
       // Decays::Unweight (FlagDaugthers, Decays.inl:75-89): w / max > u;
       // hydra::unweight (RndFlag, RandomUtils.h:254-262): u * max <= w
-      acc = a.rule == 0 ? (w / maxv) > u : (u * maxv) <= w;
+      acc = a.rule == 0 ? ratio_above(w, maxv, rmax, rmax_ok, u) : (u * maxv) <= w;
     }
     const uint64_t ball = __ballot(acc);
     if (lane == 0) *(glob(a.flags) + g) = ball;

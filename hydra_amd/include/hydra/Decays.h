@@ -536,62 +536,93 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     materialize();
     const size_t n = fSize;
     if (w.size() < n) throw std::invalid_argument("[hydra::unweight]: fewer weights than events");
-    detail::device_buffer<double> out(4 * N * (n ? n : 1));
-    detail::device_buffer<double> wout(n ? n : 1);
+    auto& out = spare_cols(n);
     const size_t wsb = hpsp_unweight_sample_workspace_size(n);
-    detail::device_buffer<unsigned char> ws(wsb);
-    detail::device_buffer<uint64_t> nacc(1);
+    unsigned char* ws = spare_ws(wsb);
     const double* wc = weights_cache();
     // the container's own PhaseSpaceWeight column moves with the events
     detail::hpsp_check(hpsp_unweight_sample(4 * (int)N, fCols.data(), fCap, n, w.data(), max_pdf,
                                             (uint64_t)rng_seed, (uint64_t)rng_jump,
                                             HPSP_ORDER_SEQUENTIAL, out.data(), n ? n : 1,
-                                            nullptr, nullptr, ws.data(), wsb, nacc.data(),
-                                            nullptr),
+                                            nullptr, nullptr, ws, wsb, spare_nacc(), nullptr),
                        "hydra::unweight");
+    detail::device_buffer<double>* wout = nullptr;
     if (wc) {
       // reorder the cached weights with the same permutation (as a one-column sample)
+      wout = &spare_w(n);
       detail::hpsp_check(hpsp_unweight_sample(1, wc, n, n, w.data(), max_pdf, (uint64_t)rng_seed,
                                               (uint64_t)rng_jump, HPSP_ORDER_SEQUENTIAL,
-                                              wout.data(), n ? n : 1, nullptr, nullptr,
-                                              ws.data(), wsb, nacc.data(), nullptr),
+                                              wout->data(), n ? n : 1, nullptr, nullptr, ws, wsb,
+                                              spare_nacc(), nullptr),
                          "hydra::unweight");
     }
-    uint64_t acc = 0;
-    detail::hpsp_check(hpsp_memcpy_to_host(&acc, nacc.data(), sizeof(acc)), "hydra::unweight");
-    fCols.swap(out);
-    fW.swap(wout);
-    fWValid = fWValid && wc != nullptr;
-    fCap = n ? n : 1;
-    fGenValid = false;
-    return Range<iterator>(begin(), begin() + (std::ptrdiff_t)acc);
+    fWValid = wc != nullptr;
+    return adopt(out, wout);
   }
 
  private:
+  // The events written once, at their places (the PhaseSpaceWeight column
+  // is not moved with them, as the reference's Decays holds none: it is
+  // formed again from the momenta if it is needed again).
   Range<iterator> unweight_regenerate(size_t seed) {
     const size_t n = fSize;
-    detail::device_buffer<double> out(4 * N * (n ? n : 1));
-    detail::device_buffer<double> wout(n ? n : 1);
+    auto& out = spare_cols(n);
     const size_t wsb = hpsp_unweight_workspace_size(n);
-    detail::device_buffer<unsigned char> ws(wsb);
-    detail::device_buffer<uint64_t> nacc(1);
     detail::hpsp_check(hpsp_unweight_regenerate(&fGen, fGenSeed, 0, n, fW.data(), (uint64_t)seed,
-                                                out.data(), n ? n : 1, nullptr, wout.data(),
-                                                ws.data(), wsb, nacc.data(), nullptr),
+                                                out.data(), n ? n : 1, nullptr, nullptr,
+                                                spare_ws(wsb), wsb, spare_nacc(), nullptr),
                        "Decays::Unweight");
-    return adopt(out, wout, nacc);
+    fWValid = false;
+    return adopt(out, nullptr);
   }
 
-  Range<iterator> adopt(detail::device_buffer<double>& out, detail::device_buffer<double>& wout,
-                        detail::device_buffer<uint64_t>& nacc) {
+  // Takes the unweighted columns (and, when given, the weight column moved
+  // with them); the container's previous buffers become the spares.
+  Range<iterator> adopt(detail::device_buffer<double>& out, detail::device_buffer<double>* wout) {
     uint64_t acc = 0;
-    detail::hpsp_check(hpsp_memcpy_to_host(&acc, nacc.data(), sizeof(acc)), "Decays::Unweight");
+    detail::hpsp_check(hpsp_memcpy_to_host(&acc, fSpare.nacc.data(), sizeof(acc)),
+                       "Decays::Unweight");
     fCols.swap(out);
-    fW.swap(wout);
+    if (wout) fW.swap(*wout);
     fCap = fSize ? fSize : 1;
     fGenValid = false;
     fLazy = false;
     return Range<iterator>(begin(), begin() + (std::ptrdiff_t)acc);
+  }
+
+  // Buffers that the Unweight calls fill and then swap with the container's
+  // own (output columns, weight column) or use as workspace: kept from call to
+  // call, so that repeated Generate/Unweight cycles of one size allocate
+  // nothing after the first.  A copy of the container starts without them.
+  struct Spare {
+    detail::device_buffer<double> cols, w;
+    detail::device_buffer<unsigned char> ws;
+    detail::device_buffer<uint64_t> nacc;
+    Spare() = default;
+    Spare(const Spare&) {}
+    Spare(Spare&&) noexcept = default;
+    Spare& operator=(const Spare&) { return *this; }
+    Spare& operator=(Spare&&) noexcept = default;
+  };
+  // sized for n events: reallocated when too small or more than twice too large
+  static void fit(detail::device_buffer<double>& b, size_t need) {
+    if (b.size() < need || b.size() / 2 > need) b.allocate(need);
+  }
+  detail::device_buffer<double>& spare_cols(size_t n) {
+    fit(fSpare.cols, 4 * N * (n ? n : 1));
+    return fSpare.cols;
+  }
+  detail::device_buffer<double>& spare_w(size_t n) {
+    fit(fSpare.w, n ? n : 1);
+    return fSpare.w;
+  }
+  unsigned char* spare_ws(size_t bytes) {
+    if (fSpare.ws.size() < bytes) fSpare.ws.allocate(bytes);
+    return fSpare.ws.data();
+  }
+  uint64_t* spare_nacc() {
+    if (fSpare.nacc.size() < 1) fSpare.nacc.allocate(1);
+    return fSpare.nacc.data();
   }
 
   // Unweight(functor) with a user's callable: its PhaseSpaceReweight values
@@ -608,36 +639,31 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     detail::device_buffer<double> vals(n ? n : 1);
     detail::user_event_values((hydra::tuple<Particles...>*)nullptr, d, fCols.data(), fCap, n,
                               true, vals.data(), n ? n : 1, functor);
-    detail::device_buffer<double> out(4 * N * (n ? n : 1));
-    detail::device_buffer<double> wout(n ? n : 1);
+    auto& out = spare_cols(n);
     const size_t wsb = hpsp_unweight_workspace_size(n);
-    detail::device_buffer<unsigned char> ws(wsb);
-    detail::device_buffer<uint64_t> nacc(1);
     detail::hpsp_check(hpsp_unweight_values((int)N, vals.data(), max_value, (uint64_t)seed,
                                             fCols.data(), fCap, n, out.data(), n ? n : 1,
-                                            nullptr, nullptr, ws.data(), wsb, nacc.data(),
+                                            nullptr, nullptr, spare_ws(wsb), wsb, spare_nacc(),
                                             nullptr),
                        "Decays::Unweight");
     fWValid = false;
-    return adopt(out, wout, nacc);
+    return adopt(out, nullptr);
   }
 
   Range<iterator> unweight(const hpsp_decay& d, const hpsp_functor* f, double max_value,
                            size_t seed) {
     materialize();
     const size_t n = fSize;
-    detail::device_buffer<double> out(4 * N * (n ? n : 1));
-    detail::device_buffer<double> wout(n ? n : 1);
+    auto& out = spare_cols(n);
     const size_t wsb = hpsp_unweight_workspace_size(n);
-    detail::device_buffer<unsigned char> ws(wsb);
-    detail::device_buffer<uint64_t> nacc(1);
     const double* win = weights_cache();
+    detail::device_buffer<double>* wout = win ? &spare_w(n) : nullptr;
     detail::hpsp_check(hpsp_unweight(&d, f, max_value, (uint64_t)seed, fCols.data(), fCap, n,
                                      win, out.data(), n ? n : 1, nullptr,
-                                     win ? wout.data() : nullptr, ws.data(), wsb, nacc.data(),
-                                     nullptr),
+                                     wout ? wout->data() : nullptr, spare_ws(wsb), wsb,
+                                     spare_nacc(), nullptr),
                        "Decays::Unweight");
-    return adopt(out, wout, nacc);
+    return adopt(out, wout);
   }
 
   double fMotherMass = 0.0;
@@ -655,6 +681,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   mutable bool fReadAfterGen = false;
   bool fCheck = checks_from_env();
   mutable uint64_t fGuard = 0;
+  Spare fSpare;
 };
 
 // Decays in host memory (the reference's Decays<tuple<P...>, host::sys_t>):
