@@ -640,7 +640,11 @@ def served_mixed_errors(torch, capi, nreq=6000, seed=5, count=24, nmax=1 << 20):
             st = out.cpu().numpy()
             s64 = math.sqrt(st[0, 2]) / st[0, 0]
             s32 = math.sqrt(st[1, 2]) / st[1, 0]
-            avg.append((abs(st[1, 1] - st[0, 1]) / abs(st[0, 1]), abs(s32 - s64) / s64))
+            # a 2-body request integrates a constant (its one pair mass is the
+            # mother's): both sigmas are rounding noise there, so only the mean
+            # is compared
+            sig = abs(s32 - s64) / s64 if r["N"] >= 3 else None
+            avg.append((abs(st[1, 1] - st[0, 1]) / abs(st[0, 1]), sig))
         elif not r["avg"] and len(gen) < count:
             e = mixed_errors(torch, capi, r["M"], r["m"], n)
             gen.append(e)
@@ -651,7 +655,8 @@ def served_mixed_errors(torch, capi, nreq=6000, seed=5, count=24, nmax=1 << 20):
     return {"generate_requests": len(gen), "generate_max_over_requests": g,
             "average_requests": len(avg),
             "average_mean_rel_err_max": float(max(a[0] for a in avg)),
-            "average_sigma_rel_err_max": float(max(a[1] for a in avg)),
+            "average_sigma_requests": sum(1 for a in avg if a[1] is not None),
+            "average_sigma_rel_err_max": float(max(a[1] for a in avg if a[1] is not None)),
             "events_cap": nmax}
 
 
@@ -918,7 +923,7 @@ def main():
             "unit": "GB/s", "frac": c2["regen_scatter_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "k_uw_regen<3, 0>", "ms": c2["regen_scatter_ms"],
             "algorithmic_bytes_per_launch": c2["regen_scatter_bytes_per_launch"],
-            "weights_pass": {"kernel": "k_generate_f64<3, 0, false> (no columns)",
+            "weights_pass": {"kernel": "k_generate_f64<3, 0, false, false> (no columns)",
                              "bound": "fp64 issue", "ms": c2["generate_weights_ms"],
                              "flops_per_event": w_flops, "achieved": w_tflops,
                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

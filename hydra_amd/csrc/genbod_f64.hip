@@ -14,8 +14,10 @@ namespace hpsp {
 // Each wave takes runs of a.run consecutive groups of 64 events (grid-stride
 // over runs): with the squares engines a group's second 64 words are the next
 // group's first, so within a run each lane evaluates one word per group
-// instead of two.
-template <int N, int ENGINE, bool MOTHERS>
+// instead of two.  COLS = false is the weights pass of a deferred Generate
+// (no momentum columns): without the column stores the kernel holds its
+// decay constants in scalar registers with none spilled.
+template <int N, int ENGINE, bool MOTHERS, bool COLS = true>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
 void k_generate_f64(const GenArgs a) {
   const int lane = __lane_id();
@@ -60,7 +62,7 @@ void k_generate_f64(const GenArgs a) {
       }
 
       if (live) {
-        if (!a.cols) {
+        if (!COLS || !a.cols) {
           // weights only: the momenta are formed for them and not stored
         } else if (a.out_f32) {
           float* c = static_cast<float*>(a.cols) + k0;
@@ -306,6 +308,8 @@ static void gen_one(const GenArgs& in, hipStream_t s) {
   const unsigned g = grid_for_runs((ngroups + a.run - 1) / a.run);
   if (a.mothers)
     hipLaunchKernelGGL((k_generate_f64<N, E, true>), dim3(g), dim3(kBlock), 0, s, a);
+  else if (!a.cols)
+    hipLaunchKernelGGL((k_generate_f64<N, E, false, false>), dim3(g), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL((k_generate_f64<N, E, false>), dim3(g), dim3(kBlock), 0, s, a);
 }

@@ -11,7 +11,18 @@
 //   stream until the event's record as of the wait call has completed;
 //   using a destroyed event or stream is an error;
 // * hipFree, hipHostFree and hipStreamDestroy wait for the work they could
-//   still be touching, as the HIP runtime does.
+//   still be touching, as the HIP runtime does;
+// * stream capture: operations on a capturing stream become nodes of the
+//   capture's graph instead of running; a stream joins a capture by waiting
+//   on an event recorded in it and must rejoin the origin stream (the origin
+//   waits on an event recorded after its last operation) before
+//   hipStreamEndCapture, else hipErrorStreamCaptureUnjoined; capturing from
+//   another thread, a capturing stream waiting on an event from outside the
+//   capture, waiting on or querying an event last recorded in a capture, and
+//   destroying a graph exec with a launch pending are errors; hipGraphLaunch
+//   runs the nodes in capture order as one stream operation.  (The serving
+//   loop launches directly; the model of capture replays the graph-capturing
+//   loop of the project's history, see the Makefile's graph_replay.)
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -25,6 +36,7 @@
 #include <functional>
 #include <memory>
 #include <mutex>
+#include <map>
 #include <random>
 #include <thread>
 #include <vector>
@@ -49,14 +61,42 @@ struct Op {
 
 }  // namespace
 
+struct Capture;
+
 struct ihipEvent_t {
   uint64_t recorded = 0, completed = 0;
   bool destroyed = false;
+  Capture* cap = nullptr;  // recorded inside this capture (a capture event)
+  uint64_t cap_pos = 0;    // position of that record in the capture
 };
 
 struct ihipStream_t {
   std::deque<Op> q;
   bool destroyed = false;
+  Capture* cap = nullptr;  // capturing into this capture
+  uint64_t last_pos = 0;   // position of this stream's last captured operation
+};
+
+struct ihipGraph {
+  std::vector<std::function<void()>> nodes;
+  bool destroyed = false;
+};
+
+struct hipGraphExec {
+  std::vector<std::function<void()>> nodes;
+  bool destroyed = false;
+  uint64_t pending = 0;
+};
+
+struct Capture {
+  ihipStream_t* origin = nullptr;
+  ihipGraph* g = nullptr;
+  std::vector<ihipStream_t*> joined;  // besides the origin
+  std::map<ihipStream_t*, uint64_t> joined_back;  // stream -> position of its rejoin
+  uint64_t pos = 0;
+  std::thread::id tid;
+  bool invalid = false;
+  bool ended = false;
 };
 
 namespace {
@@ -137,6 +177,32 @@ void enqueue(ihipStream_t* s, Op op) {
   Device& d = dev();
   if (!s) model_fail("operation on the null stream");
   if (s->destroyed) model_fail("operation on a destroyed stream");
+  if (s->cap) {
+    Capture* c = s->cap;
+    if (std::this_thread::get_id() != c->tid) model_fail("capture from another thread");
+    ++c->pos;
+    s->last_pos = c->pos;
+    switch (op.kind) {
+      case Op::FN:
+        c->g->nodes.push_back(std::move(op.fn));
+        break;
+      case Op::RECORD:
+        op.ev->cap = c;
+        op.ev->cap_pos = c->pos;
+        break;
+      case Op::WAIT:
+        if (op.ev->cap != c) model_fail("capturing stream waits on an event from outside the capture");
+        // the origin waiting on an event recorded by a joined stream after
+        // that stream's last operation rejoins it
+        if (s == c->origin)
+          for (ihipStream_t* j : c->joined)
+            if (j->last_pos <= op.ev->cap_pos) c->joined_back[j] = c->pos;
+        break;
+    }
+    return;
+  }
+  if (op.kind == Op::WAIT && op.ev->cap) model_fail("wait outside a capture on a capture event");
+  if (op.kind == Op::RECORD) op.ev->cap = nullptr;
   s->q.push_back(std::move(op));
   d.cv_dev.notify_all();
 }
@@ -242,6 +308,7 @@ hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
 hipError_t hipEventQuery(hipEvent_t e) {
   std::lock_guard<std::mutex> g(dev().mu);
   if (!e || e->destroyed) model_fail("hipEventQuery of a destroyed event");
+  if (e->cap) model_fail("hipEventQuery of a capture event");
   return e->completed >= e->recorded ? hipSuccess : hipErrorNotReady;
 }
 
@@ -249,6 +316,15 @@ hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned int) {
   std::lock_guard<std::mutex> g(dev().mu);
   if (!e || e->destroyed) model_fail("hipStreamWaitEvent on a destroyed event");
   if (!s) model_fail("hipStreamWaitEvent on the null stream");
+  if (!s->cap && e->cap) {
+    // joins the capture of the event's record
+    Capture* c = e->cap;
+    if (c->ended) model_fail("wait on an event last recorded in an ended capture");
+    s->cap = c;
+    c->joined.push_back(s);
+    s->last_pos = ++c->pos;
+    return hipSuccess;
+  }
   Op op;
   op.kind = Op::WAIT;
   op.ev = e;
@@ -263,6 +339,83 @@ hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKin
   Op op;
   op.fn = [dst, src, bytes] { memcpy(dst, src, bytes); };
   enqueue(s, std::move(op));
+  return hipSuccess;
+}
+
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
+  std::lock_guard<std::mutex> g(dev().mu);
+  if (!s || s->destroyed) model_fail("capture on an invalid stream");
+  if (s->cap) model_fail("capture on a capturing stream");
+  Capture* c = new Capture;
+  c->origin = s;
+  c->g = new ihipGraph;
+  c->tid = std::this_thread::get_id();
+  s->cap = c;
+  return hipSuccess;
+}
+
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* out) {
+  std::lock_guard<std::mutex> g(dev().mu);
+  if (!s || !s->cap || s->cap->origin != s) model_fail("end capture on a stream not capturing");
+  Capture* c = s->cap;
+  if (std::this_thread::get_id() != c->tid) model_fail("end capture from another thread");
+  bool unjoined = false;
+  for (ihipStream_t* j : c->joined) {
+    auto it = c->joined_back.find(j);
+    if (it == c->joined_back.end() || j->last_pos > it->second) unjoined = true;
+    j->cap = nullptr;
+  }
+  s->cap = nullptr;
+  c->ended = true;
+  if (unjoined) {
+    delete c->g;
+    delete c;
+    *out = nullptr;
+    return hipErrorStreamCaptureUnjoined;
+  }
+  *out = c->g;
+  // capture events keep pointing at the capture: they stay capture events
+  // until recorded again outside one (a model of what may be waited on)
+  static std::vector<std::unique_ptr<Capture>> ended;  // reachable, as events may refer to it
+  ended.emplace_back(c);
+  return hipSuccess;
+}
+
+hipError_t hipGraphInstantiate(hipGraphExec_t* out, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
+  std::lock_guard<std::mutex> lk(dev().mu);
+  if (!g || g->destroyed) model_fail("instantiate of an invalid graph");
+  hipGraphExec* x = new hipGraphExec;
+  x->nodes = g->nodes;
+  *out = x;
+  return hipSuccess;
+}
+
+hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(dev().mu);
+  if (!x || x->destroyed) model_fail("launch of an invalid graph exec");
+  ++x->pending;
+  Op op;
+  op.fn = [x] {
+    for (auto& f : x->nodes) f();
+    --x->pending;
+  };
+  enqueue(s, std::move(op));
+  return hipSuccess;
+}
+
+hipError_t hipGraphExecDestroy(hipGraphExec_t x) {
+  std::lock_guard<std::mutex> lk(dev().mu);
+  if (!x || x->destroyed) model_fail("destroy of an invalid graph exec");
+  if (x->pending) model_fail("graph exec destroyed with a launch pending");
+  x->destroyed = true;
+  delete x;
+  return hipSuccess;
+}
+
+hipError_t hipGraphDestroy(hipGraph_t gr) {
+  std::lock_guard<std::mutex> lk(dev().mu);
+  if (!gr || gr->destroyed) model_fail("destroy of an invalid graph");
+  delete gr;
   return hipSuccess;
 }
 
