@@ -904,6 +904,7 @@ def main():
         print("# range done", file=sys.stderr, flush=True)
         extras["c5"] = run_c5(torch, capi)
         extras["c5"]["mixed_errors"] = served_mixed_errors(torch, capi)
+        extras["c5_saturated"] = run_c5(torch, capi, rate=50000.0)
         print("# c5 done", file=sys.stderr, flush=True)
     extras["c2"] = c2
 

@@ -106,22 +106,21 @@ __device__ __forceinline__ double genbod_f64(const DecayK& d, const WordFn& word
     // gamma = 1/sqrt(1 - beta^2), gamma*beta and (gamma-1)/beta^2 * beta^2 are
     // E*/M, pd/M and gamma - 1; the momenta are formed as
     // E' = gamma E + gamma beta py and py' = gamma beta E + gamma py.
-    // For pd = 0 (beta = 0) the reference's boost does nothing: the boosted
-    // values are formed regardless and the old ones selected (a branch here
-    // costs register moves on every event).
+    // For pd = 0 (beta = 0) the reference's boost does nothing; here gamma = 1
+    // and gamma beta = 0 are selected instead, which leave E and py as they
+    // are (up to the sign of a zero py): a branch costs register moves on
+    // every event, selects of the boosted components 4(i+1) instructions.
     {
       const bool boost = pd[i] > 0.0;
       const double inv_m = rcp_fast(invMas[i]);
       const double estar = sqrt_fast(pd[i] * pd[i] + invMas[i] * invMas[i]);
-      const double gamma = estar * inv_m;
-      const double gby = pd[i] * inv_m;
+      const double gamma = boost ? estar * inv_m : 1.0;
+      const double gby = boost ? pd[i] * inv_m : 0.0;
 #pragma unroll
       for (int j = 0; j <= i; ++j) {
         const double e2 = p[j][0], py2 = p[j][2];
-        const double e3 = gamma * e2 + gby * py2;
-        const double py3 = gby * e2 + gamma * py2;
-        p[j][0] = boost ? e3 : e2;
-        p[j][2] = boost ? py3 : py2;
+        p[j][0] = gamma * e2 + gby * py2;
+        p[j][2] = gby * e2 + gamma * py2;
       }
     }
   }

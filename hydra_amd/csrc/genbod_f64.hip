@@ -15,8 +15,9 @@ namespace hpsp {
 // over runs): with the squares engines a group's second 64 words are the next
 // group's first, so within a run each lane evaluates one word per group
 // instead of two.  COLS = false is the weights pass of a deferred Generate
-// (no momentum columns): without the column stores the kernel holds its
-// decay constants in scalar registers with none spilled.
+// (no momentum columns): without the column stores' addresses and store
+// paths in its loop, the scalar registers it spills are reloaded outside the
+// loop only.
 template <int N, int ENGINE, bool MOTHERS, bool COLS = true>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
 void k_generate_f64(const GenArgs a) {

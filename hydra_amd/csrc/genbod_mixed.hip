@@ -270,15 +270,11 @@ __global__ __launch_bounds__(kBlock) void k_generate_mixed(const GenArgs a) {
                                           uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane);
         }
         if (a.wt) __builtin_nontemporal_store(wt, uniform_ptr(a.wt + k0) + lane);
-        if (a.evw) {
-          double pp[N][4];
-#pragma unroll
-          for (int i = 0; i < N; ++i)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) pp[i][q] = (double)p[i][q];
-          __builtin_nontemporal_store(event_weight_f64<N>(pp, a.d.m, a.d.maxw),
-                                      uniform_ptr(a.evw + k0) + lane);
-        }
+        // PhaseSpaceWeight of the mixed form: the FP64 generation weight,
+        // which it equals in exact arithmetic.  Recomputed from the FP32
+        // momenta (masses of running sums, then pdk) it would lose up to all
+        // of its digits where a - b - c cancels near threshold.
+        if (a.evw) __builtin_nontemporal_store(wt, uniform_ptr(a.evw + k0) + lane);
       }
     }
   }

@@ -360,10 +360,13 @@ def test_mixed_generate_error_bounds(oracle, key):
     N = len(m)
     dec = make_decay(M, m)
     n = 20000
-    cols, wt, _ = generate(dec, SEED, n, prec=MIXED, out_f32=True)
+    cols, wt, evw = generate(dec, SEED, n, prec=MIXED, out_f32=True)
     ocols, owt = oracle.generate(M, m, SEED, n)
     # the weight chain stays FP64 (one-ulp sqrt and reciprocal in pdk)
     assert np.max(np.abs(wt.cpu().numpy() - owt) / owt) <= 4e-15
+    # the mixed PhaseSpaceWeight column is that generation weight (equal to
+    # PhaseSpaceWeight in exact arithmetic, not recomputed from FP32 momenta)
+    assert torch.equal(evw, wt)
     err, rms = momentum_error(cols.cpu().numpy(), ocols, N)
     assert err <= 2e-5, (err, rms)
     assert rms <= 1e-6, (err, rms)
