@@ -243,6 +243,31 @@ int main(int argc, char** argv) {
       std::printf("\"deferred_checks\": [%d, %d, %d],\n", caught, passed,
                   nacc_h == accepted.size() ? 1 : 0);
     }
+    // Generate + Unweight() three times on one container, as a loop does (each
+    // Unweight writes into the buffers the previous one left as spares, and
+    // the weight column is formed again after each): every cycle gives the
+    // rows of a fresh container's first cycle, bit for bit
+    {
+      hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> L(P_mass, masses, n);
+      hydra::Decays<hydra::tuple<A, B, C>, hydra::host::sys_t> h_ref(E2);
+      int cycles_same = 1, weights_same = 1;
+      auto w_ref = (E2 | E2.GetEventWeightFunctor()).to_host();
+      for (int cyc = 0; cyc < 3; ++cyc) {
+        phsp.Generate(Parent, L);
+        const size_t na = L.Unweight().size();
+        hydra::Decays<hydra::tuple<A, B, C>, hydra::host::sys_t> h(L);
+        if (na != accepted.size()) cycles_same = 0;
+        for (size_t c = 0; c < 12 && cycles_same; ++c)
+          if (std::memcmp(h.data() + c * h.ld(), h_ref.data() + c * h_ref.ld(),
+                          n * sizeof(double)) != 0)
+            cycles_same = 0;
+        auto w = (L | L.GetEventWeightFunctor()).to_host();
+        if (w.size() != w_ref.size() || std::memcmp(w.data(), w_ref.data(),
+                                                    w.size() * sizeof(double)) != 0)
+          weights_same = 0;
+      }
+      std::printf("\"unweight_cycles\": [%d, %d],\n", cycles_same, weights_same);
+    }
     // Events was read after its deferred Generate: the next Generate into it
     // writes momenta and weights in one pass, with the same bits, and its
     // Unweight() still regenerates (the result of E2's above)

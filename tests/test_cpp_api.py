@@ -73,6 +73,9 @@ def test_gpu_generate_unweight_average(golden, oracle):
     assert r["unweight_after_write"] == [0, 1]
     # opt-in check of the deferred path: a stale-pointer write is caught
     assert r["deferred_checks"] == [1, 1, 1]
+    # Generate + Unweight() cycles on one container reuse its buffers and give
+    # a fresh container's rows and weights each time
+    assert r["unweight_cycles"] == [1, 1]
     # Generate writes the weights and defers the momenta to their first read
     assert r["lazy_after_generate"] == 1 and r["lazy_after_read"] == 0
     assert r["lazy_same"] == 1

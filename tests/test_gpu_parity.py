@@ -382,3 +382,36 @@ def test_mixed_average_on_error(golden, oracle, key):
     amx = average_on(dec, SEED, n, f=f, prec=MIXED)
     assert abs(amx[0] - a64[0]) <= 1e-13 * a64[0]  # FP64 weights, few-ulp pdk
     assert abs(amx[1] - a64[1]) <= 1e-6 * abs(a64[1]), (amx, a64)
+
+
+@pytest.mark.gpu
+def test_unweight_accept_test_at_the_boundary(oracle):
+    """FlagDaugthers' w / max > u (Decays.inl:75-89) on weights placed within
+    three ulp of u * max, where only the correctly rounded quotient decides
+    (the kernel settles most events with a product by the reciprocal and
+    leaves these to the division): the accepted set equals the one numpy's
+    IEEE division gives, row for row in the stable partition."""
+    M, m = DECAYS["c1_3body"]
+    dec = make_decay(M, m)
+    n = 1 << 16
+    seed = 0x180EC6D33CFD0ABA
+    u = oracle.rng_words(0, seed, 0, n).astype(np.float64) * 2.0 ** -32
+    maxw = dec.max_weight
+    rng = np.random.default_rng(7)
+    w = u * maxw
+    steps = rng.integers(-3, 4, n)
+    for k in range(1, 4):
+        up, down = steps >= k, steps <= -k
+        w[up] = np.nextafter(w[up], np.inf)
+        w[down] = np.nextafter(w[down], -np.inf)
+    far = rng.random(n) < 0.25  # and some far from the boundary
+    w[far] = rng.random(int(far.sum())) * maxw
+    w[::97] = 0.0
+    want = (w / maxw) > u
+    boundary = ~far
+    assert 0 < int(want[boundary].sum()) < int(boundary.sum())  # both outcomes occur
+    cols = torch.zeros((12, n), dtype=torch.float64, device="cuda:0")
+    nacc, _, idx, _ = unweight(dec, cols, seed, w_in=torch.as_tensor(w, device="cuda:0"))
+    assert nacc == int(want.sum())
+    got = idx[:nacc].cpu().numpy()
+    assert np.array_equal(got, np.nonzero(want)[0])
