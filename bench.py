@@ -100,13 +100,15 @@ class Timer:
 
 def run_c2(torch, capi, n, steps, warmup):
     """The headline step as the C++ drop-in runs `phsp.Generate(B0, Events);
-    Events.Unweight()`: Generate writes the PhaseSpaceWeight column and defers
-    the momenta (hpsp_generate with no columns), Unweight computes the flags
-    from that column and writes every event once, at its place in the
-    partition (hpsp_unweight_regenerate).  Timed beside it, with the same
-    final container bit for bit: Generate writing the momenta then the
-    regenerating Unweight ("eager"), and then the moving Unweight
-    (hpsp_unweight: every event read and moved)."""
+    Events.Unweight()`: Generate records (decay, seed) and forms nothing;
+    Unweight (hpsp_generate_unweight) takes each event's flag from its weight
+    chain (PhaseSpaceWeight only where the chain leaves the test in doubt),
+    scans the counts and generates every event once, at its place in the
+    partition.  Timed beside it, with the same final container bit for bit:
+    the two-pass form (Generate writing the PhaseSpaceWeight column, then
+    hpsp_unweight_regenerate), Generate writing the momenta then the same
+    Unweight ("eager"), and then the moving Unweight (hpsp_unweight: every
+    event read and moved)."""
     lib = capi.lib()
     dec = capi.make_decay(M3, MASSES3)
     N = 3
@@ -114,12 +116,20 @@ def run_c2(torch, capi, n, steps, warmup):
     cols = torch.empty((4 * N, n), dtype=torch.float64, device=dev)
     evw = torch.empty(n, dtype=torch.float64, device=dev)
     out = torch.empty((4 * N, n), dtype=torch.float64, device=dev)
-    ws_bytes = lib.hpsp_unweight_workspace_size(n)
+    out2 = torch.empty((4 * N, n), dtype=torch.float64, device=dev)
+    ws_bytes = max(lib.hpsp_unweight_workspace_size(n), lib.hpsp_generate_unweight_workspace_size(n))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     nacc = torch.zeros(1, dtype=torch.int64, device=dev)
+    nacc2 = torch.zeros(1, dtype=torch.int64, device=dev)
     sh = capi.stream_handle()
-    tg, tu, tf, tm, tl = Timer(torch), Timer(torch), Timer(torch), Timer(torch), Timer(torch)
-    tc = Timer(torch)
+    T = {k: Timer(torch) for k in ("weights", "gen", "uw", "flags", "move", "cols", "step",
+                                   "chain")}
+
+    def step(with_output=True):
+        capi.check(lib.hpsp_generate_unweight(ctypes.byref(dec), SEED, 0, n, UW_SEED,
+                                              capi.ptr(out2) if with_output else None, n, None,
+                                              None, capi.ptr(ws), ws_bytes, capi.ptr(nacc2), sh),
+                   "generate_unweight")
 
     def gen():
         capi.check(lib.hpsp_generate(ctypes.byref(dec), SEED, 0, n, capi.FP64, capi.ptr(cols), n,
@@ -145,60 +155,67 @@ def run_c2(torch, capi, n, steps, warmup):
                                      n, capi.ptr(evw), capi.ptr(out), n, None, None,
                                      capi.ptr(ws), ws_bytes, capi.ptr(nacc), sh), "unweight")
 
+    def timed(key, fn, *args):
+        T[key].start()
+        fn(*args)
+        T[key].stop()
+
     for _ in range(warmup):
+        step()
         gen_weights()
         unweight()
     torch.cuda.synchronize()
-    # kernel-level timing (separate pass; events synchronise).  The flag and
-    # scan kernels alone run when no output is requested, so the scatter
-    # kernel's time is the full Unweight minus that.
+    # kernel-level timing (separate pass; events synchronise).  With no
+    # outputs requested only the flag and scan kernels run, so the
+    # regenerating scatter's time is the full call minus that.
     for _ in range(max(3, min(steps, 10))):
-        tl.start()
-        gen_weights()
-        tl.stop()
-        tg.start()
-        gen()
-        tg.stop()
-        tu.start()
-        unweight()
-        tu.stop()
-        tf.start()
-        unweight(with_output=False)
-        tf.stop()
-        tm.start()
-        unweight_move()
-        tm.stop()
-        tc.start()
-        gen_cols()
-        tc.stop()
+        timed("step", step)
+        timed("chain", step, False)
+        timed("weights", gen_weights)
+        timed("gen", gen)
+        timed("uw", unweight)
+        timed("flags", unweight, False)
+        timed("move", unweight_move)
+        timed("cols", gen_cols)
     torch.cuda.synchronize()
+    same = bool(torch.equal(out, out2)) and int(nacc.item()) == int(nacc2.item())
     t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1000.0 / steps
+    t2 = time.perf_counter()
     for _ in range(steps):
         gen_weights()
         unweight()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1000.0 / steps
-    regen_ms = tu.avg_ms - tf.avg_ms
-    move_ms = tm.avg_ms - tf.avg_ms
-    res = {"ms_per_step": ms, "events_per_s": n / (ms / 1000.0), "n_acc": int(nacc.item()),
-           "generate_weights_ms": tl.avg_ms, "unweight_ms": tu.avg_ms,
-           "flags_scan_ms": tf.avg_ms, "regen_scatter_ms": regen_ms,
-           "generate_ms": tg.avg_ms, "eager_step_ms": tg.avg_ms + tu.avg_ms,
-           "move_unweight_ms": tm.avg_ms, "move_scatter_ms": move_ms,
-           "move_step_ms": tg.avg_ms + tm.avg_ms, "generate_momenta_only_ms": tc.avg_ms}
+    t3 = time.perf_counter()
+    ms_two = (t3 - t2) * 1000.0 / steps
+    ms_of = {k: t.avg_ms for k, t in T.items()}
+    regen_ms = ms_of["step"] - ms_of["chain"]
+    regen2_ms = ms_of["uw"] - ms_of["flags"]
+    move_ms = ms_of["move"] - ms_of["flags"]
+    res = {"ms_per_step": ms, "events_per_s": n / (ms / 1000.0), "n_acc": int(nacc2.item()),
+           "same_as_two_pass": same,
+           "chain_flags_scan_ms": ms_of["chain"], "regen_scatter_ms": regen_ms,
+           "two_pass_step_ms": ms_two, "generate_weights_ms": ms_of["weights"],
+           "flags_scan_ms": ms_of["flags"], "two_pass_regen_scatter_ms": regen2_ms,
+           "generate_ms": ms_of["gen"], "eager_step_ms": ms_of["gen"] + ms_of["uw"],
+           "move_unweight_ms": ms_of["move"], "move_scatter_ms": move_ms,
+           "move_step_ms": ms_of["gen"] + ms_of["move"],
+           "generate_momenta_only_ms": ms_of["cols"]}
     gen_bytes = n * (32 * N + 8)
-    res["generate_weights_GBs"] = n * 8 / (tl.avg_ms * 1e6)
-    res["generate_weights_events_per_s"] = n / (tl.avg_ms / 1e3)
+    res["generate_weights_GBs"] = n * 8 / (ms_of["weights"] * 1e6)
     regen_bytes = n * 32 * N
     move_bytes = n * 64 * N
-    res["generate_GBs"] = gen_bytes / (tg.avg_ms * 1e6)
+    res["generate_GBs"] = gen_bytes / (ms_of["gen"] * 1e6)
     res["regen_scatter_GBs"] = regen_bytes / (regen_ms * 1e6)
     res["move_scatter_GBs"] = move_bytes / (move_ms * 1e6)
     res["generate_bytes_per_launch"] = gen_bytes
     res["regen_scatter_bytes_per_launch"] = regen_bytes
     res["move_scatter_bytes_per_launch"] = move_bytes
-    del cols, evw, out, ws
+    del cols, evw, out, out2, ws
     torch.cuda.empty_cache()
     return res
 
@@ -908,11 +925,10 @@ def main():
         print("# c5 done", file=sys.stderr, flush=True)
     extras["c2"] = c2
 
-    # The step has two kernels of about equal time: the regenerating scatter,
-    # which writes 12.9 of the step's 14.0 GB and is bound by HBM (the
-    # roofline), and the weights pass of Generate, which stores 8 B per event
-    # and is bound by the issue of its FP64 arithmetic (reported beside it
-    # against the FP64 peak, in algorithmic operations).
+    # The step is the flags from the weight chain (the engine words and the
+    # chain's FP64 arithmetic; no HBM traffic but the flag words) and the
+    # regenerating scatter, which writes the step's 12.9 GB and is bound by HBM
+    # (the roofline).  The two-pass form's weights pass is reported beside it.
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
@@ -924,11 +940,15 @@ def main():
             "unit": "GB/s", "frac": c2["regen_scatter_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "k_uw_regen<3, 0>", "ms": c2["regen_scatter_ms"],
             "algorithmic_bytes_per_launch": c2["regen_scatter_bytes_per_launch"],
-            "weights_pass": {"kernel": "k_generate_f64<3, 0, false, false> (no columns)",
-                             "bound": "fp64 issue", "ms": c2["generate_weights_ms"],
-                             "flops_per_event": w_flops, "achieved": w_tflops,
-                             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                             "frac": w_tflops / FP64_PEAK_TFLOPS}}
+            "flags_pass": {"kernel": "k_uw_flags_chain<3, 0> + scans",
+                           "bound": "valu (engine words, weight chain)",
+                           "ms": c2["chain_flags_scan_ms"],
+                           "events_per_s": args.n / (c2["chain_flags_scan_ms"] * 1e-3)},
+            "two_pass_weights_pass": {"kernel": "k_generate_f64<3, 0, false, false> (no columns)",
+                                      "bound": "fp64 issue", "ms": c2["generate_weights_ms"],
+                                      "flops_per_event": w_flops, "achieved": w_tflops,
+                                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                      "frac": w_tflops / FP64_PEAK_TFLOPS}}
     cpu = None
     if args.cpu_baseline:
         cpu = cpu_baseline()
@@ -956,8 +976,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (counter-based RNG, seed 159753)",
         "config": {"workload": "3-body decay 5.279 -> (3.0969, 0.493677, 0.13957) GeV, FP64, "
-                               "Generate into SoA + Unweight() as Decays runs them (weights "
-                               "first, each event's momenta written once at its place in the "
+                               "Generate into SoA + Unweight() as Decays runs them (nothing "
+                               "formed at Generate; Unweight takes each event's flag from its "
+                               "weight chain and writes its momenta once, at its place in the "
                                "unweighted container)", "events_per_step": args.n,
                    "n_daughters": 3, "engine": "squares3", "parallelism": "single GPU"},
         "roofline": roof,

@@ -189,7 +189,9 @@ void k_uw_regen(const UnweightArgs a) {
               *(glob_u(a.cols_out + (uint64_t)(4 * i + q) * a.ld_out) + dst) = p[i][q];
         }
       }
-      if (a.w_out) *(glob(a.w_out) + dst) = *(glob(a.w_in) + k);
+      if (a.w_out)
+        *(glob(a.w_out) + dst) =
+            a.w_in ? *(glob(a.w_in) + k) : event_weight_f64<N>(p, a.d.m, a.d.maxw);
       if (a.idx_out) *(glob(a.idx_out) + dst) = k;
     }
   }

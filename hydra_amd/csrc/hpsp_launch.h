@@ -202,6 +202,14 @@ int launch_range_generate(int N, const RangeArgs& a, hipStream_t s);
 int launch_range_histogram(int N, const RangeArgs& a, hipStream_t s);
 int launch_stats_finalize(const Stats* partials, uint64_t m, Stats* out, hipStream_t s);
 int launch_event_weight(int N, const WeightArgs& a, hipStream_t s);
+// Exclusive scan of the tile counts into a.tile_off, the total into *a.n_acc.
+int launch_uw_scans(const UnweightArgs& a, hipStream_t s);
+// Flags of Decays::Unweight for events a.gen_first.. of a Generate with decay
+// a.d, decided from each event's generation weight where it is far enough
+// from the boundary (generate_unweight.hip); scale2 bounds E^2 + p^2 of any
+// subsystem, kc the momenta's error constant.
+int launch_flags_chain(int N, int engine, const UnweightArgs& a, double scale2, double kc,
+                       hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s);

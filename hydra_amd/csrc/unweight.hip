@@ -33,26 +33,6 @@ __device__ __forceinline__ void load_event_uw(const double* cols, uint64_t ld, u
     for (int q = 0; q < 4; ++q) p[i][q] = cols[(uint64_t)(4 * i + q) * ld + k];
 }
 
-// RN(w / maxv) > u, the reference's test, decided without a division where the
-// product with the reciprocal settles it.  rmax = RN(1 / maxv) and
-// q = RN(w * rmax) put q within 2^-51.3 |q| of w / maxv's rounded value
-// (relative errors 2^-53 each, rmax and q normal), so when |q - u| exceeds
-// 2^-50 |q| the rounded quotient lies strictly on q's side of u.  Otherwise
-// (and for zero, subnormal, infinite or NaN operands) the division decides:
-// the same flag as the division in every case.
-__device__ __forceinline__ bool ratio_above(double w, double maxv, double rmax, bool rmax_ok,
-                                            double u) {
-#pragma clang fp contract(off)
-  const double q = w * rmax;
-  const double gap = q - u;
-  const double aq = __builtin_fabs(q);
-  const bool settled = rmax_ok && aq >= 0x1p-1000 && aq <= 0x1p+1000 &&
-                       __builtin_fabs(gap) > aq * 0x1p-50;
-  bool r = gap > 0.0;
-  if (!settled) r = (w / maxv) > u;
-  return r;
-}
-
 // N = 0: the weights come from a.w_in alone (no functor), and the kernel
 // holds no momenta: few registers, many waves in flight for the one stream it
 // reads.  Tiles are dealt to XCDs in contiguous ranges.
@@ -514,6 +494,17 @@ constexpr int kNumN = HPSP_MAX_N - 1;
 
 static void launch_flags(int N, const UnweightArgs& a, uint64_t ntiles, hipStream_t s);
 
+int launch_uw_scans(const UnweightArgs& a, hipStream_t s) {
+  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
+  const uint64_t nruns = (ntiles + kScanRun - 1) / kScanRun;
+  hipLaunchKernelGGL(k_uw_scan_blocks, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
+                     ntiles, a.run_sum);
+  hipLaunchKernelGGL(k_uw_scan_top, dim3(1), dim3(kBlock), 0, s, a.run_sum, nruns, a.n_acc);
+  hipLaunchKernelGGL(k_uw_scan_tiles, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
+                     ntiles, a.run_sum, a.tile_off);
+  return (int)hipGetLastError();
+}
+
 template <int N>
 static void uw_one(const UnweightArgs& a, hipStream_t s) {
   const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
@@ -521,12 +512,7 @@ static void uw_one(const UnweightArgs& a, hipStream_t s) {
     launch_flags(0, a, ntiles, s);
   else
     launch_flags(N, a, ntiles, s);
-  const uint64_t nruns = (ntiles + kScanRun - 1) / kScanRun;
-  hipLaunchKernelGGL(k_uw_scan_blocks, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
-                     ntiles, a.run_sum);
-  hipLaunchKernelGGL(k_uw_scan_top, dim3(1), dim3(kBlock), 0, s, a.run_sum, nruns, a.n_acc);
-  hipLaunchKernelGGL(k_uw_scan_tiles, dim3((unsigned)nruns), dim3(kBlock), 0, s, a.tile_count,
-                     ntiles, a.run_sum, a.tile_off);
+  launch_uw_scans(a, s);
   if ((a.cols_out || a.idx_out || a.w_out) && !a.regen) {
     if constexpr (N <= 6) {
       const uint64_t nblocks = (a.n + kBlock - 1) / kBlock;

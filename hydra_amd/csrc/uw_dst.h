@@ -82,6 +82,26 @@ __device__ __forceinline__ BlockPart block_part(const UnweightArgs& a, uint64_t 
   return b;
 }
 
+// RN(w / maxv) > u, the reference's test, decided without a division where the
+// product with the reciprocal settles it.  rmax = RN(1 / maxv) and
+// q = RN(w * rmax) put q within 2^-51.3 |q| of w / maxv's rounded value
+// (relative errors 2^-53 each, rmax and q normal), so when |q - u| exceeds
+// 2^-50 |q| the rounded quotient lies strictly on q's side of u.  Otherwise
+// (and for zero, subnormal, infinite or NaN operands) the division decides:
+// the same flag as the division in every case.
+__device__ __forceinline__ bool ratio_above(double w, double maxv, double rmax, bool rmax_ok,
+                                            double u) {
+#pragma clang fp contract(off)
+  const double q = w * rmax;
+  const double gap = q - u;
+  const double aq = __builtin_fabs(q);
+  const bool settled = rmax_ok && aq >= 0x1p-1000 && aq <= 0x1p+1000 &&
+                       __builtin_fabs(gap) > aq * 0x1p-50;
+  bool r = gap > 0.0;
+  if (!settled) r = (w / maxv) > u;
+  return r;
+}
+
 // Tile of a block: each XCD takes a contiguous range of tiles, so that
 // neighbouring tiles, which write neighbouring pieces of the accepted and
 // rejected runs, share an L2.

@@ -344,9 +344,9 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
 
   // ---- unweighting (Decays.inl:415-511)
   Range<iterator> Unweight(size_t seed = 0x180ec6d33cfd0aba) {
-    if (fGenValid && fWValid) {
+    if (fGenValid) {
       if (fCheck) guard_verify();
-      return unweight_regenerate(seed);
+      return fWValid ? unweight_regenerate(seed) : generate_unweight(seed);
     }
     const hpsp_decay d = detail::make_decay<N>(fMotherMass, fMasses, "Decays::Unweight");
     return unweight(d, nullptr, -1.0, seed);
@@ -406,12 +406,12 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
   // that replaces only some rows and must keep the others.
   void materialize_rows() const { materialize(); }
 
-  // PhaseSpace::Generate(mother, events) with this container's decay: only
-  // the PhaseSpaceWeight column has been written; the momenta of events
-  // 0 .. size() - 1 of (d, seed) are formed when first read (any access to
-  // the columns, a row, a particle range, a copy to the host) or written
-  // straight to their places by Unweight().  A function of (d, seed, index)
-  // alone, so the deferred write gives the bits an eager one would.
+  // PhaseSpace::Generate(mother, events) with this container's decay:
+  // nothing has been written; the momenta of events 0 .. size() - 1 of
+  // (d, seed) are formed when first read (any access to the columns, a row, a
+  // particle range, a copy to the host) or written straight to their places
+  // by Unweight().  A function of (d, seed, index) alone, so the deferred
+  // write gives the bits an eager one would.
   void set_generated_lazy(hpsp_decay const& d, uint64_t seed) {
     set_generated(d, seed);
     fLazy = true;
@@ -571,6 +571,22 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     detail::hpsp_check(hpsp_unweight_regenerate(&fGen, fGenSeed, 0, n, fW.data(), (uint64_t)seed,
                                                 out.data(), n ? n : 1, nullptr, nullptr,
                                                 spare_ws(wsb), wsb, spare_nacc(), nullptr),
+                       "Decays::Unweight");
+    fWValid = false;
+    return adopt(out, nullptr);
+  }
+
+  // Events that Generate has not formed (or whose weight column is not
+  // kept): flags from each event's weight chain, PhaseSpaceWeight only where
+  // the chain leaves the test in doubt, then every event generated at its
+  // place; the same container as unweight_regenerate's.
+  Range<iterator> generate_unweight(size_t seed) {
+    const size_t n = fSize;
+    auto& out = spare_cols(n);
+    const size_t wsb = hpsp_generate_unweight_workspace_size(n);
+    detail::hpsp_check(hpsp_generate_unweight(&fGen, fGenSeed, 0, n, (uint64_t)seed, out.data(),
+                                              n ? n : 1, nullptr, nullptr, spare_ws(wsb), wsb,
+                                              spare_nacc(), nullptr),
                        "Decays::Unweight");
     fWValid = false;
     return adopt(out, nullptr);

@@ -166,14 +166,10 @@ class PhaseSpace {
                       events.GetMaxWeight() == fMaxWeight;
     double* cols = events.columns_for_overwrite();
     if (same && events.size() > 0 && !events.read_after_generate()) {
-      // the weights now, the momenta when first read or at their places in
-      // Unweight() (the container keeps (decay, seed) to form them)
-      detail::hpsp_check(hpsp_generate(&d, fSeed, 0, events.size(), HPSP_FP64, nullptr,
-                                       events.ld(), 0, nullptr, events.weights_cache_for_write(),
-                                       nullptr),
-                         "PhaseSpace::Generate");
-      detail::hpsp_sync("PhaseSpace::Generate");
-      events.set_weights_cache_valid(true);
+      // nothing formed yet: the container keeps (decay, seed); the momenta
+      // are formed when first read, or Unweight() places every event once
+      // (hpsp_generate_unweight), its flags from the weight chain
+      events.set_weights_cache_valid(false);
       events.set_generated_lazy(d, (uint64_t)fSeed);
     } else if (same && events.size() > 0) {
       // a container that is read after Generate: momenta and weights in one

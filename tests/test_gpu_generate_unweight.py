@@ -1,0 +1,119 @@
+"""Generate followed by Decays::Unweight in one call (hpsp_generate_unweight:
+the flags from each event's generation weight where a bound on its distance
+from PhaseSpaceWeight settles the test, from PhaseSpaceWeight elsewhere; then
+every event generated at its place) against the two calls it replaces
+(hpsp_generate of the PhaseSpaceWeight column, then
+hpsp_unweight_regenerate, itself bit-exact with the moving Unweight, see
+test_gpu_regen): the same container, index map, weights and accepted count,
+bit for bit, for every N and engine, ragged sizes, a moving mother, decays
+near threshold (where the bound is large) and light daughters beside a heavy
+one (where it is largest)."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import DEV, check, generate, lib, make_decay, ptr, stream_handle, \
+    unweight_regenerate
+from oracle import ENGINES
+
+pytestmark = pytest.mark.gpu
+
+UW_SEED = 0x180EC6D33CFD0ABA
+
+
+def bits(t):
+    return t.cpu().numpy().view(np.uint64)
+
+
+def generate_unweight(dec, gen_seed, first, n, seed, want_idx=True, want_w=True, ld=None):
+    ld = ld or max(n, 1)
+    ws_bytes = lib().hpsp_generate_unweight_workspace_size(n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
+    out = torch.empty((4 * dec.n, ld), dtype=torch.float64, device=DEV)
+    idx = torch.empty(max(n, 1), dtype=torch.int64, device=DEV) if want_idx else None
+    w = torch.empty(max(n, 1), dtype=torch.float64, device=DEV) if want_w else None
+    nacc = torch.full((1,), 12345, dtype=torch.int64, device=DEV)
+    check(lib().hpsp_generate_unweight(ctypes.byref(dec), gen_seed, first, n, seed, ptr(out), ld,
+                                       ptr(idx), ptr(w), ptr(ws), ws_bytes, ptr(nacc),
+                                       stream_handle()), "hpsp_generate_unweight")
+    torch.cuda.synchronize()
+    return int(nacc.item()), out, idx, w
+
+
+def same(dec, gen_seed, first, n, uw_seed):
+    _, _, evw = generate(dec, gen_seed, n, first=first, want_wt=False)
+    k1, out1, idx1, w1 = unweight_regenerate(dec, gen_seed, first, n, evw, uw_seed)
+    print(f"N={dec.n} engine={dec.engine} n={n} first={first}", flush=True)
+    k2, out2, idx2, w2 = generate_unweight(dec, gen_seed, first, n, uw_seed)
+    assert k1 == k2
+    assert np.array_equal(bits(out1[:, :n]), bits(out2[:, :n]))
+    assert np.array_equal(idx1.cpu().numpy()[:n], idx2.cpu().numpy()[:n])
+    assert np.array_equal(bits(w1[:n]), bits(w2[:n]))
+    return k1
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 6, 7, 12])
+@pytest.mark.parametrize("engine", list(ENGINES))
+def test_generate_unweight_equals_two_calls(N, engine):
+    rng = np.random.default_rng(700 + 13 * N + ENGINES[engine])
+    m = list(rng.uniform(0.1, 1.0, N))
+    M = sum(m) * rng.uniform(1.5, 3.0) + 2.0
+    dec = make_decay(M, m, engine=engine)
+    seed = int(rng.integers(0, 2**63))
+    first = int(rng.integers(0, 2**40))
+    k = same(dec, seed, first, 20000 + N, int(rng.integers(0, 2**63)))
+    assert 0 <= k <= 20000 + N
+
+
+def test_generate_unweight_sizes_and_moving_mother():
+    """Partial last blocks, one block, and a boosted (moving) mother."""
+    m = [3.0969, 0.493677, 0.13957]
+    dec = make_decay(5.279, m, mother4=[6.0, 0.3, -0.2, 0.5])
+    for n in (1, 63, 255, 256, 257, 1023, 4096 + 17):
+        same(dec, 159753, 7, n, UW_SEED)
+
+
+def test_generate_unweight_headline_decay_many_blocks():
+    """The headline decay over 2^22 events, and the optional outputs left
+    out."""
+    dec = make_decay(5.279, [3.0969, 0.493677, 0.13957])
+    n = 1 << 22
+    k = same(dec, 159753, 0, n, UW_SEED)
+    k2, out2, _, _ = generate_unweight(dec, 159753, 0, n, UW_SEED, want_idx=False, want_w=False)
+    assert k2 == k
+    k3, out3, _, _ = generate_unweight(dec, 159753, 0, n, UW_SEED)
+    assert torch.equal(out2, out3)
+
+
+@pytest.mark.parametrize("case", ["threshold3", "threshold10", "light", "philox_threshold"])
+def test_generate_unweight_ill_conditioned(case):
+    M3 = 3.0969 + 0.493677 + 0.13957
+    decays = {
+        "threshold3": (M3 + 0.001, [3.0969, 0.493677, 0.13957], "squares3"),
+        "threshold10": (10.0, [0.999] * 10, "squares3"),
+        "light": (100.0, [0.000511, 0.000511, 90.0], "squares4"),
+        "philox_threshold": (M3 + 0.01, [3.0969, 0.493677, 0.13957], "philox"),
+    }
+    M, m, engine = decays[case]
+    dec = make_decay(M, m, engine=engine)
+    same(dec, 2024, 3, 1 << 20, UW_SEED)
+
+
+def test_generate_unweight_empty_and_errors():
+    dec = make_decay(5.279, [3.0969, 0.493677, 0.13957])
+    k, _, _, _ = generate_unweight(dec, 1, 0, 0, UW_SEED)
+    assert k == 0
+    nacc = torch.zeros(1, dtype=torch.int64, device=DEV)
+    ws = torch.empty(16, dtype=torch.uint8, device=DEV)
+    out = torch.empty((12, 1000), dtype=torch.float64, device=DEV)
+    # workspace too small, stride too small, no count
+    assert lib().hpsp_generate_unweight(ctypes.byref(dec), 1, 0, 1000, 2, ptr(out), 1000, None,
+                                        None, ptr(ws), 16, ptr(nacc), stream_handle()) != 0
+    ws_bytes = lib().hpsp_generate_unweight_workspace_size(1000)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
+    assert lib().hpsp_generate_unweight(ctypes.byref(dec), 1, 0, 1000, 2, ptr(out), 999, None,
+                                        None, ptr(ws), ws_bytes, ptr(nacc), stream_handle()) != 0
+    assert lib().hpsp_generate_unweight(ctypes.byref(dec), 1, 0, 1000, 2, ptr(out), 1000, None,
+                                        None, ptr(ws), ws_bytes, None, stream_handle()) != 0
