@@ -639,7 +639,14 @@ int hpsp_unweight_regenerate(const hpsp_decay* d, uint64_t gen_seed, uint64_t fi
   return hip_status(launch_unweight_regen(d->n, d->engine, a, s));
 }
 
-size_t hpsp_generate_unweight_workspace_size(uint64_t n) { return uw_layout(n).total; }
+// hpsp_unweight's workspace and one word per group of 64 events of every tile
+// the flags kernel covers, as the flags (the events the weight chain leaves in
+// doubt)
+static size_t gu_unc_offset(uint64_t n) { return uw_layout(n).total; }
+size_t hpsp_generate_unweight_workspace_size(uint64_t n) {
+  const uint64_t ngroups = (n + kWave - 1) / kWave + kUwTile / kWave;
+  return gu_unc_offset(n) + align_up(ngroups * sizeof(uint64_t));
+}
 
 int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t first, uint64_t n,
                            uint64_t seed, double* cols_out, uint64_t ld_out, uint64_t* idx_out,
@@ -651,8 +658,9 @@ int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t firs
   if (n > 0 && cols_out && ld_out < n) return HPSP_ERR_INVALID_ARGUMENT;
   if (first > UINT64_MAX - n) return HPSP_ERR_INVALID_ARGUMENT;
   const UwLayout L = uw_layout(n);
-  if (!ws || ws_bytes < L.total) return HPSP_ERR_WORKSPACE;
+  if (!ws || ws_bytes < hpsp_generate_unweight_workspace_size(n)) return HPSP_ERR_WORKSPACE;
   char* base = static_cast<char*>(ws);
+  uint64_t* unc = reinterpret_cast<uint64_t*>(base + gu_unc_offset(n));
   hipStream_t s = as_stream(stream);
   UnweightArgs a;
   memset(&a, 0, sizeof(a));
@@ -688,7 +696,7 @@ int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t firs
   }
   const bool narrow = d->engine == HPSP_SQUARES3 || d->engine == HPSP_SQUARES4;
   const double K = (narrow ? 0x1p17 : 0x1p33) * d->n;
-  rc = hip_status(launch_flags_chain(d->n, d->engine, a, scale2, 2.0 * K + 4.0, s));
+  rc = hip_status(launch_flags_chain(d->n, d->engine, a, unc, scale2, 2.0 * K + 4.0, s));
   if (rc) return rc;
   rc = hip_status(launch_uw_scans(a, s));
   if (rc) return rc;

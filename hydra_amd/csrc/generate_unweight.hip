@@ -12,8 +12,9 @@
 // with q = wt / max wherever |q - u| > (B + 2^-48) q.  Elsewhere (where the
 // two could fall on different sides of u, where the bound is large or its
 // terms ill-defined) the event's momenta and PhaseSpaceWeight are formed and
-// the reference's test decides.  The flags are those of the test on
-// PhaseSpaceWeight, which hpsp_unweight_regenerate reads from its column.
+// the reference's test decides, in a second kernel over those events.  The
+// flags are those of the test on PhaseSpaceWeight, which
+// hpsp_unweight_regenerate reads from its column.
 //
 // The bound.  With a = invMas[i] (the mass of daughters 0..i), b = m[i] and
 // c = invMas[i-1], PhaseSpaceWeight's term is pdk(mass(p_0..p_i), b,
@@ -75,11 +76,14 @@ struct OneWord {
 // The weight chain of one event (DecayMother.h:170-212: sorted uniforms,
 // invMas, pdk, the product) with one-ulp square roots and reciprocals, and
 // the bound B of the file comment; sure is false where a term of B is not
-// defined (d1 or d2 not positive) or B is not small.
+// defined (d1 or d2 not positive) or B is not small.  The bound's
+// reciprocals come from those of the chain: 1/(d1 d2) = rsq(x)^2 with
+// x = d1 d2 under pdk's square root, 1/a^2 = (2 rcp(2a))^2, and c is the
+// previous step's a (1/m0^2 = inv_m0sq at the first step).
 template <int N, int ENGINE, typename WordFn>
 __device__ __forceinline__ double chain_weight(const DecayK& d, const WordFn& word,
-                                               double scale2, double kc, double& B,
-                                               bool& sure) {
+                                               double scale2, double kc, double inv_m0sq,
+                                               double& B, bool& sure) {
 #pragma clang fp contract(off)
   constexpr double eps = 0x1p-53;
   double rno[N];
@@ -102,23 +106,37 @@ __device__ __forceinline__ double chain_weight(const DecayK& d, const WordFn& wo
     invMas[n] = rno[n] * d.ecm + sum;
   }
   double wt = d.maxw;
-  B = 40.0 * N * eps;
-  sure = true;
+  double Bs = 0.0;
+  double inv_c2 = inv_m0sq;
+  bool ok = true;
+  const double ks = kc * eps * scale2;
 #pragma unroll
   for (int n = 0; n < N - 1; ++n) {
     const double a = invMas[n + 1], b = d.m[n + 1], c = invMas[n];
     const double spc = b + c, smc = b - c;
     const double d1 = (a - spc) * (a + spc);
     const double d2 = (a - smc) * (a + smc);
-    wt *= sqrt_fast(d1 * d2) * rcp_fast(2.0 * a);
-    sure = sure && d1 > 0.0 && d2 > 0.0;
-    const double r1 = rcp_fast(d1), r2 = rcp_fast(d2);
-    const double a2 = a * a;
-    const double ka = a2 * r1 + a2 * r2 + 1.0;
-    const double kcc = c * spc * r1 + c * __builtin_fabs(smc) * r2;
-    B += (ka + kcc) * (kc * eps * scale2 * (rcp_fast(a2) + rcp_fast(c * c)) + 4.0 * eps);
+    const double x = d1 * d2;
+    // sqrt(x) from y ~ 1/sqrt(x) with one Newton step on (g, h) and a residual
+    // correction (sqrt_fast's scheme), keeping y
+    const double y = __builtin_amdgcn_rsq(__builtin_fmax(x, 0x1p-1000));
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+    const double ra = rcp_fast(2.0 * a);
+    wt *= g * ra;
+    ok = ok && d1 > 0.0 && d2 > 0.0;
+    const double inv_x = y * y;
+    const double inv_a2 = 4.0 * ra * ra;
+    const double ka = a * a * (d1 + d2) * inv_x + 1.0;
+    const double kcc = c * (spc * d2 + __builtin_fabs(smc) * d1) * inv_x;
+    Bs += (ka + kcc) * (ks * (inv_a2 + inv_c2) + 4.0 * eps);
+    inv_c2 = inv_a2;
   }
-  sure = sure && B < 0x1p-20;  // false for NaN
+  B = Bs + 40.0 * N * eps;
+  sure = ok && B < 0x1p-20;  // false for NaN
   return wt;
 }
 
@@ -126,10 +144,12 @@ __device__ __forceinline__ double chain_weight(const DecayK& d, const WordFn& wo
 
 // One 1024-event tile per block, items as k_uw_flags: thread item j is event
 // tile * 1024 + j * 256 + wave * 64 + lane of the container (generation
-// index a.gen_first + that); flags as 64-bit ballots, counts per tile.
+// index a.gen_first + that); flags as 64-bit ballots, counts per tile.  The
+// events the chain leaves in doubt get the chain's flag for now and a bit in
+// unc[group]; k_uw_patch_chain decides them.
 template <int N, int ENGINE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
-void k_uw_flags_chain(const UnweightArgs a, double scale2, double kc) {
+void k_uw_flags_chain(const UnweightArgs a, uint64_t* unc, double scale2, double kc) {
 #pragma clang fp contract(off)
   __shared__ uint32_t cnt[kBlock / kWave];
   const int lane = __lane_id();
@@ -139,12 +159,13 @@ void k_uw_flags_chain(const UnweightArgs a, double scale2, double kc) {
   if (tile >= ntiles) return;  // whole block
   const double rmax = 1.0 / a.maxv;
   const bool rmax_ok = __builtin_fabs(rmax) >= 0x1p-1000 && __builtin_fabs(rmax) <= 0x1p+1000;
+  const double inv_m0sq = 1.0 / (a.d.m[0] * a.d.m[0]);
   const uint64_t off = (uint64_t)wave * kWave + (uint64_t)lane;
   // key * (jump + k) of item 0; item j + 1 is kBlock events on
   uint64_t ykey = a.uw_key * (a.jump + tile * kUwTile + off);
   const uint64_t ystep = a.uw_key * (uint64_t)kBlock;
   uint32_t c = 0;
-  // (not unrolled: the slow path makes the body large)
+#pragma unroll
   for (int j = 0; j < kItems; ++j, ykey += ystep) {
     const uint64_t g = tile * (kUwTile / kWave) + (uint64_t)j * (kBlock / kWave) + wave;
     const uint64_t k0 = g * kWave;
@@ -155,35 +176,24 @@ void k_uw_flags_chain(const UnweightArgs a, double scale2, double kc) {
       // the one uniform is the lane's own word: counter gen_first + k + 3N
       const OneWord w{squares_word_y<ENGINE>(
           a.d.key[0], a.d.key[0] * (a.gen_first + k + 3 * (uint64_t)N))};
-      wt = chain_weight<N, ENGINE>(a.d, w, scale2, kc, B, sure);
+      wt = chain_weight<N, ENGINE>(a.d, w, scale2, kc, inv_m0sq, B, sure);
     } else {
       EventWords<N, ENGINE> words;
       words.init(a.d, a.gen_first + k0, lane);
-      wt = chain_weight<N, ENGINE>(a.d, words, scale2, kc, B, sure);
+      wt = chain_weight<N, ENGINE>(a.d, words, scale2, kc, inv_m0sq, B, sure);
     }
-    bool acc = false;
-    if (k < a.n) {
-      const double u = (double)squares3_y(a.uw_key, ykey) * 0x1p-32;  // ctr = jump + k
-      const double q = wt * rmax;
-      const double aq = __builtin_fabs(q);
-      sure = sure && rmax_ok && aq >= 0x1p-1000 && aq <= 0x1p+1000 &&
-             __builtin_fabs(q - u) > aq * (B + 0x1p-48);
-      acc = q > u;
-      if (!sure) {
-        // this event's momenta and PhaseSpaceWeight, and the reference's test
-        LaneEventWords<N, ENGINE> lw;
-        lw.init(a.d, a.gen_first + k);
-        double p[N][4];
-        genbod_f64<N, ENGINE>(a.d, lw, p);
-        if (a.d.do_boost) {
-#pragma unroll
-          for (int i = 0; i < N; ++i) boost_folded(p[i], a.d);
-        }
-        acc = ratio_above(event_weight_f64<N>(p, a.d.m, a.d.maxw), a.maxv, rmax, rmax_ok, u);
-      }
+    const bool live = k < a.n;
+    const double u = (double)squares3_y(a.uw_key, ykey) * 0x1p-32;  // ctr = jump + k
+    const double q = wt * rmax;
+    const double aq = __builtin_fabs(q);
+    sure = sure && rmax_ok && aq >= 0x1p-1000 && aq <= 0x1p+1000 &&
+           __builtin_fabs(q - u) > aq * (B + 0x1p-48);
+    const uint64_t ball = __ballot(live && q > u);
+    const uint64_t doubt = __ballot(live && !sure);
+    if (lane == 0) {
+      *(glob(a.flags) + g) = ball;
+      *(glob(unc) + g) = doubt;
     }
-    const uint64_t ball = __ballot(acc);
-    if (lane == 0) *(glob(a.flags) + g) = ball;
     c += (uint32_t)__popcll(ball);
   }
   if (lane == 0) cnt[wave] = c;
@@ -191,14 +201,61 @@ void k_uw_flags_chain(const UnweightArgs a, double scale2, double kc) {
   if (threadIdx.x == 0) a.tile_count[tile] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
 }
 
+// The events left in doubt: one thread per group of 64 (almost all have none),
+// each doubtful event's momenta and PhaseSpaceWeight formed and the
+// reference's test applied (FlagDaugthers, Decays.inl:75-89); the flag word
+// corrected and the tile's count adjusted.
+template <int N, int ENGINE>
+__global__ __launch_bounds__(kBlock) void k_uw_patch_chain(const UnweightArgs a,
+                                                           const uint64_t* unc) {
+#pragma clang fp contract(off)
+  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
+  const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= ngroups) return;
+  uint64_t m = *(glob(unc) + g);
+  if (m == 0) return;
+  const double rmax = 1.0 / a.maxv;
+  const bool rmax_ok = __builtin_fabs(rmax) >= 0x1p-1000 && __builtin_fabs(rmax) <= 0x1p+1000;
+  uint64_t fw = *(glob(a.flags) + g);
+  int delta = 0;
+  while (m) {
+    const int l = __builtin_ctzll(m);
+    m &= m - 1;
+    const uint64_t k = g * kWave + (uint64_t)l;
+    LaneEventWords<N, ENGINE> lw;
+    lw.init(a.d, a.gen_first + k);
+    double p[N][4];
+    genbod_f64<N, ENGINE>(a.d, lw, p);
+    if (a.d.do_boost) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) boost_folded(p[i], a.d);
+    }
+    const double u = (double)squares3_y(a.uw_key, a.uw_key * (a.jump + k)) * 0x1p-32;
+    const bool acc =
+        ratio_above(event_weight_f64<N>(p, a.d.m, a.d.maxw), a.maxv, rmax, rmax_ok, u);
+    const bool was = (fw >> l) & 1ULL;
+    if (acc != was) {
+      fw ^= 1ULL << l;
+      delta += acc ? 1 : -1;
+    }
+  }
+  *(glob(a.flags) + g) = fw;
+  if (delta) atomicAdd(a.tile_count + g / (kUwTile / kWave), (uint32_t)delta);
+}
+
 constexpr int kNumN = HPSP_MAX_N - 1;
 constexpr int kNumE = 6;
 
 template <int N, int E>
-static void chain_one(const UnweightArgs& a, double scale2, double kc, hipStream_t s) {
+static void chain_one(const UnweightArgs& a, uint64_t* unc, double scale2, double kc,
+                      hipStream_t s) {
   const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
   const unsigned grid = (unsigned)((ntiles + kXcds - 1) / kXcds * kXcds);
-  hipLaunchKernelGGL((k_uw_flags_chain<N, E>), dim3(grid), dim3(kBlock), 0, s, a, scale2, kc);
+  hipLaunchKernelGGL((k_uw_flags_chain<N, E>), dim3(grid), dim3(kBlock), 0, s, a, unc, scale2,
+                     kc);
+  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
+  hipLaunchKernelGGL((k_uw_patch_chain<N, E>), dim3((unsigned)((ngroups + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, s, a, (const uint64_t*)unc);
 }
 
 template <int I>
@@ -206,15 +263,15 @@ static constexpr auto chain_at = &chain_one<2 + I / kNumE, I % kNumE>;
 
 template <int... I>
 static constexpr auto chain_table(std::integer_sequence<int, I...>) {
-  return std::array<void (*)(const UnweightArgs&, double, double, hipStream_t), sizeof...(I)>{
-      chain_at<I>...};
+  return std::array<void (*)(const UnweightArgs&, uint64_t*, double, double, hipStream_t),
+                    sizeof...(I)>{chain_at<I>...};
 }
 
 static const auto kChain = chain_table(std::make_integer_sequence<int, kNumN * kNumE>{});
 
-int launch_flags_chain(int N, int engine, const UnweightArgs& a, double scale2, double kc,
-                       hipStream_t s) {
-  kChain[(N - 2) * kNumE + engine](a, scale2, kc, s);
+int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* unc, double scale2,
+                       double kc, hipStream_t s) {
+  kChain[(N - 2) * kNumE + engine](a, unc, scale2, kc, s);
   return (int)hipGetLastError();
 }
 
