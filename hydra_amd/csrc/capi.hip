@@ -639,13 +639,12 @@ int hpsp_unweight_regenerate(const hpsp_decay* d, uint64_t gen_seed, uint64_t fi
   return hip_status(launch_unweight_regen(d->n, d->engine, a, s));
 }
 
-// hpsp_unweight's workspace and one word per group of 64 events of every tile
-// the flags kernel covers, as the flags (the events the weight chain leaves in
-// doubt)
-static size_t gu_unc_offset(uint64_t n) { return uw_layout(n).total; }
+// hpsp_unweight's workspace, a counter and a list of two words per group of
+// 64 events (the groups with events the weight chain leaves in doubt)
+static size_t gu_list_offset(uint64_t n) { return uw_layout(n).total + align_up(sizeof(uint64_t)); }
 size_t hpsp_generate_unweight_workspace_size(uint64_t n) {
-  const uint64_t ngroups = (n + kWave - 1) / kWave + kUwTile / kWave;
-  return gu_unc_offset(n) + align_up(ngroups * sizeof(uint64_t));
+  const uint64_t ngroups = (n + kWave - 1) / kWave;
+  return gu_list_offset(n) + align_up(2 * ngroups * sizeof(uint64_t));
 }
 
 int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t first, uint64_t n,
@@ -660,7 +659,8 @@ int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t firs
   const UwLayout L = uw_layout(n);
   if (!ws || ws_bytes < hpsp_generate_unweight_workspace_size(n)) return HPSP_ERR_WORKSPACE;
   char* base = static_cast<char*>(ws);
-  uint64_t* unc = reinterpret_cast<uint64_t*>(base + gu_unc_offset(n));
+  uint64_t* dl_count = reinterpret_cast<uint64_t*>(base + uw_layout(n).total);
+  uint64_t* dl = reinterpret_cast<uint64_t*>(base + gu_list_offset(n));
   hipStream_t s = as_stream(stream);
   UnweightArgs a;
   memset(&a, 0, sizeof(a));
@@ -694,9 +694,9 @@ int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t firs
     const double f = a.d.gamma * (1.0 + b);
     scale2 *= f * f;
   }
-  const bool narrow = d->engine == HPSP_SQUARES3 || d->engine == HPSP_SQUARES4;
-  const double K = (narrow ? 0x1p17 : 0x1p33) * d->n;
-  rc = hip_status(launch_flags_chain(d->n, d->engine, a, unc, scale2, 2.0 * K + 4.0, s));
+  // K of the squares engines (the Random123 ones set theirs per event)
+  const double K = 0x1p17 * d->n;
+  rc = hip_status(launch_flags_chain(d->n, d->engine, a, dl_count, dl, scale2, 2.0 * K + 4.0, s));
   if (rc) return rc;
   rc = hip_status(launch_uw_scans(a, s));
   if (rc) return rc;

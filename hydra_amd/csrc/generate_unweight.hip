@@ -9,7 +9,12 @@
 // differ by rounding in the momenta, amplified by the conditioning of each
 // pdk term.  So each event's weight chain alone (no rotations, boosts or
 // sin/cos) gives wt and a bound B on |w - wt| / wt, and the flag is q > u
-// with q = wt / max wherever |q - u| > (B + 2^-48) q.  Elsewhere (where the
+// with q = wt / max wherever |q - u| > (B + 2^-48) q.  The chain is formed
+// squared, wt^2 = maxw^2 prod_i d1 d2 / (4 a^2) (no square roots, one
+// reciprocal), and the test is taken on q^2 and u^2 (both sides are not
+// negative): with e = B + 2^-47, |q^2 - u^2| > e (2 + e) q^2 puts u below
+// q (1 - e) or above q (1 + e), the margin of the linear test with room for
+// the squares' roundings.  Elsewhere (where the
 // two could fall on different sides of u, where the bound is large or its
 // terms ill-defined) the event's momenta and PhaseSpaceWeight are formed and
 // the reference's test decides, in a second kernel over those events.  The
@@ -29,10 +34,12 @@
 // set far above the momenta's errors: each rotation and boost adds a few ulp
 // of S, sin/cos are within 1.5e-16, and sin of the polar angle,
 // sqrt(1 - cos^2), carries an error up to eps / sin, where sin >= 2^-15.5
-// for a 32-bit word and >= 2^-31.5 for a 64-bit one: K = 2^17 N for the
-// squares engines, 2^33 N for the Random123 ones.  Over 2^20 events of each
-// of nine decays (N = 2..12, near threshold, electrons beside a 90 GeV
-// particle) the measured |w - wt| / wt stays below 4% of B with K = 10.
+// for a 32-bit word: K = 2^17 N for the squares engines.  The Random123
+// engines' 64-bit words let sin fall to 2^-31.5, so there K is set per event
+// from its own polar angles, K = N (64 + 4 / sin_min), with sin_min^2 taken
+// 4 eps below its estimate (1 - cos)(1 + cos).  Over 2^20 events of each of
+// nine decays (N = 2..12, near threshold, electrons beside a 90 GeV particle)
+// the measured |w - wt| / wt stays below 4% of B with K = 10.
 #include <array>
 #include <type_traits>
 #include <utility>
@@ -67,6 +74,13 @@ struct LaneEventWords {
   }
 };
 
+// A wave-uniform 64-bit value from the first active lane (scalar registers).
+__device__ __forceinline__ uint64_t first_lane(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // The one uniform of a 3-body event with a squares engine: the lane's own word.
 struct OneWord {
   uint32_t w0;
@@ -74,18 +88,33 @@ struct OneWord {
 };
 
 // The weight chain of one event (DecayMother.h:170-212: sorted uniforms,
-// invMas, pdk, the product) with one-ulp square roots and reciprocals, and
-// the bound B of the file comment; sure is false where a term of B is not
-// defined (d1 or d2 not positive) or B is not small.  The bound's
-// reciprocals come from those of the chain: 1/(d1 d2) = rsq(x)^2 with
-// x = d1 d2 under pdk's square root, 1/a^2 = (2 rcp(2a))^2, and c is the
-// previous step's a (1/m0^2 = inv_m0sq at the first step).
+// invMas, the pdk terms), squared: returns q^2 = C prod_i d1 d2 / a^2 with
+// C = (maxw / max)^2 / 4^(N-1), and the bound B of the file comment; sure is
+// false where a term of B is not defined (d1 or d2 not positive) or B is not
+// below 2^-10 (there the terms the first-order bound leaves out are below
+// 2^-10 of it, far inside the margin K leaves).  With c the previous step's a (1/m0^2 at the first step), ka + kc
+// is (d1 (a^2 + c|b-c|) + d2 (a^2 + c(b+c))) / (d1 d2) + 1, and B needs
+// only estimates of 1/(d1 d2) and 1/a^2: it is a bound with a wide margin,
+// formed with fused multiply-adds.
 template <int N, int ENGINE, typename WordFn>
-__device__ __forceinline__ double chain_weight(const DecayK& d, const WordFn& word,
-                                               double scale2, double kc, double inv_m0sq,
-                                               double& B, bool& sure) {
+__device__ __forceinline__ double chain_q2(const DecayK& d, const WordFn& word, double C,
+                                           double ks_fixed, double ks_unit, double ic2_0,
+                                           double& B, bool& sure) {
 #pragma clang fp contract(off)
   constexpr double eps = 0x1p-53;
+  // ks = (2K + 4) eps S^2: fixed K for the squares engines, per event for
+  // the Random123 ones (polar angle words N - 2 + 2 (i - 1), as GENBOD)
+  double ks = ks_fixed;
+  if constexpr (!is_squares<ENGINE>()) {
+    double s2 = 1.0;
+#pragma unroll
+    for (int i = 1; i < N; ++i) {
+      const double cz = 2 * to_unit<ENGINE>(word(N - 2 + 2 * (i - 1))) - 1;
+      s2 = __builtin_fmin(s2, (1.0 - cz) * (1.0 + cz));
+    }
+    const double inv_s = __builtin_amdgcn_rsq(__builtin_fmax(s2 - 4.0 * eps, 0x1p-120));
+    ks = ks_unit * (2.0 * N * (64.0 + 4.0 * inv_s) + 4.0);
+  }
   double rno[N];
   rno[0] = 0.0;
   rno[N - 1] = 1.0;
@@ -105,11 +134,10 @@ __device__ __forceinline__ double chain_weight(const DecayK& d, const WordFn& wo
     sum += d.m[n];
     invMas[n] = rno[n] * d.ecm + sum;
   }
-  double wt = d.maxw;
-  double Bs = 0.0;
-  double inv_c2 = inv_m0sq;
+  double P = 1.0, A = 1.0;
+  double Bs = 40.0 * N * eps;  // the roundings of the two chains
+  double ic2 = ic2_0;
   bool ok = true;
-  const double ks = kc * eps * scale2;
 #pragma unroll
   for (int n = 0; n < N - 1; ++n) {
     const double a = invMas[n + 1], b = d.m[n + 1], c = invMas[n];
@@ -117,27 +145,20 @@ __device__ __forceinline__ double chain_weight(const DecayK& d, const WordFn& wo
     const double d1 = (a - spc) * (a + spc);
     const double d2 = (a - smc) * (a + smc);
     const double x = d1 * d2;
-    // sqrt(x) from y ~ 1/sqrt(x) with one Newton step on (g, h) and a residual
-    // correction (sqrt_fast's scheme), keeping y
-    const double y = __builtin_amdgcn_rsq(__builtin_fmax(x, 0x1p-1000));
-    double g = x * y, h = 0.5 * y;
-    const double r = __builtin_fma(-h, g, 0.5);
-    g = __builtin_fma(g, r, g);
-    h = __builtin_fma(h, r, h);
-    g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
-    const double ra = rcp_fast(2.0 * a);
-    wt *= g * ra;
-    ok = ok && d1 > 0.0 && d2 > 0.0;
-    const double inv_x = y * y;
-    const double inv_a2 = 4.0 * ra * ra;
-    const double ka = a * a * (d1 + d2) * inv_x + 1.0;
-    const double kcc = c * (spc * d2 + __builtin_fabs(smc) * d1) * inv_x;
-    Bs += (ka + kcc) * (ks * (inv_a2 + inv_c2) + 4.0 * eps);
-    inv_c2 = inv_a2;
+    const double a2 = a * a;
+    P *= x;
+    A *= a2;
+    ok = ok && __builtin_fmin(d1, d2) > 0.0;
+    const double ix = __builtin_amdgcn_rcp(x), ia2 = __builtin_amdgcn_rcp(a2);
+    const double kk = __builtin_fma(
+        __builtin_fma(d1, __builtin_fma(c, __builtin_fabs(smc), a2), d2 * __builtin_fma(c, spc, a2)),
+        ix, 1.0);
+    Bs = __builtin_fma(kk, __builtin_fma(ks, ia2 + ic2, 4.0 * eps), Bs);
+    ic2 = ia2;
   }
-  B = Bs + 40.0 * N * eps;
-  sure = ok && B < 0x1p-20;  // false for NaN
-  return wt;
+  B = Bs;
+  sure = ok && B < 0x1p-10;  // false for NaN
+  return C * P * rcp_fast(A);
 }
 
 }  // namespace
@@ -145,13 +166,17 @@ __device__ __forceinline__ double chain_weight(const DecayK& d, const WordFn& wo
 // One 1024-event tile per block, items as k_uw_flags: thread item j is event
 // tile * 1024 + j * 256 + wave * 64 + lane of the container (generation
 // index a.gen_first + that); flags as 64-bit ballots, counts per tile.  The
-// events the chain leaves in doubt get the chain's flag for now and a bit in
-// unc[group]; k_uw_patch_chain decides them.
+// events the chain leaves in doubt get the chain's flag for now, and each
+// group with any is appended to the list dl as (group, mask of those events),
+// one atomicAdd on dl_count per block that has any; k_uw_patch_chain decides
+// them.
 template <int N, int ENGINE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
-void k_uw_flags_chain(const UnweightArgs a, uint64_t* unc, double scale2, double kc) {
+void k_uw_flags_chain(const UnweightArgs a, uint64_t* dl_count, uint64_t* dl, double scale2,
+                      double kc) {
 #pragma clang fp contract(off)
   __shared__ uint32_t cnt[kBlock / kWave];
+  __shared__ uint64_t dm[kUwTile / kWave];  // doubt mask of each group of the tile
   const int lane = __lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
@@ -159,103 +184,145 @@ void k_uw_flags_chain(const UnweightArgs a, uint64_t* unc, double scale2, double
   if (tile >= ntiles) return;  // whole block
   const double rmax = 1.0 / a.maxv;
   const bool rmax_ok = __builtin_fabs(rmax) >= 0x1p-1000 && __builtin_fabs(rmax) <= 0x1p+1000;
-  const double inv_m0sq = 1.0 / (a.d.m[0] * a.d.m[0]);
+  // (2K + 4) eps S^2 (1/a^2 + 1/c^2) = ks (1/a^2 + 1/c^2); kc = 2K + 4 for
+  // the squares engines
+  const double ks_unit = 0x1p-53 * scale2;
+  const double ks_fixed = kc * ks_unit;
+  const double ic2_0 = 1.0 / (a.d.m[0] * a.d.m[0]);
+  const double rmax2 = rmax * rmax;
+  const double C = a.d.maxw * a.d.maxw * rmax2 * __builtin_ldexp(1.0, -2 * (N - 1));
+  // the squared test needs q >= 0: max and maxw positive
+  const bool C_ok = rmax_ok && a.maxv > 0.0 && a.d.maxw > 0.0 && C >= 0x1p-900 && C <= 0x1p+900;
   const uint64_t off = (uint64_t)wave * kWave + (uint64_t)lane;
-  // key * (jump + k) of item 0; item j + 1 is kBlock events on
+  // key * (jump + k) of item 0 for the accept word and key * (gen_first + k +
+  // 3N) for the generation word; item j + 1 is kBlock events on
   uint64_t ykey = a.uw_key * (a.jump + tile * kUwTile + off);
   const uint64_t ystep = a.uw_key * (uint64_t)kBlock;
+  uint64_t ygen = a.d.key[0] * (a.gen_first + tile * kUwTile + off + 3 * (uint64_t)N);
+  const uint64_t gstep = a.d.key[0] * (uint64_t)kBlock;
   uint32_t c = 0;
-#pragma unroll
-  for (int j = 0; j < kItems; ++j, ykey += ystep) {
+  // unrolled where the body is small enough for it
+  constexpr int kUnroll = N <= 6 || is_squares<ENGINE>() ? kItems : 1;
+#pragma unroll kUnroll
+  for (int j = 0; j < kItems; ++j, ykey += ystep, ygen += gstep) {
     const uint64_t g = tile * (kUwTile / kWave) + (uint64_t)j * (kBlock / kWave) + wave;
     const uint64_t k0 = g * kWave;
     const uint64_t k = k0 + (uint64_t)lane;
-    double B, wt;
+    double B, q2;
     bool sure;
     if constexpr (N == 3 && is_squares<ENGINE>()) {
       // the one uniform is the lane's own word: counter gen_first + k + 3N
-      const OneWord w{squares_word_y<ENGINE>(
-          a.d.key[0], a.d.key[0] * (a.gen_first + k + 3 * (uint64_t)N))};
-      wt = chain_weight<N, ENGINE>(a.d, w, scale2, kc, inv_m0sq, B, sure);
+      const OneWord w{squares_word_y<ENGINE>(a.d.key[0], ygen)};
+      q2 = chain_q2<N, ENGINE>(a.d, w, C, ks_fixed, ks_unit, ic2_0, B, sure);
     } else {
       EventWords<N, ENGINE> words;
       words.init(a.d, a.gen_first + k0, lane);
-      wt = chain_weight<N, ENGINE>(a.d, words, scale2, kc, inv_m0sq, B, sure);
+      q2 = chain_q2<N, ENGINE>(a.d, words, C, ks_fixed, ks_unit, ic2_0, B, sure);
     }
     const bool live = k < a.n;
     const double u = (double)squares3_y(a.uw_key, ykey) * 0x1p-32;  // ctr = jump + k
-    const double q = wt * rmax;
-    const double aq = __builtin_fabs(q);
-    sure = sure && rmax_ok && aq >= 0x1p-1000 && aq <= 0x1p+1000 &&
-           __builtin_fabs(q - u) > aq * (B + 0x1p-48);
-    const uint64_t ball = __ballot(live && q > u);
+    const double u2 = u * u;
+    const double e = B + 0x1p-47;
+    sure = sure && C_ok && q2 >= 0x1p-1000 && q2 <= 0x1p+1000 &&
+           __builtin_fabs(q2 - u2) > q2 * (e * (2.0 + e));
+    const uint64_t ball = __ballot(live && q2 > u2);
     const uint64_t doubt = __ballot(live && !sure);
     if (lane == 0) {
       *(glob(a.flags) + g) = ball;
-      *(glob(unc) + g) = doubt;
+      dm[j * (kBlock / kWave) + wave] = doubt;
     }
     c += (uint32_t)__popcll(ball);
   }
   if (lane == 0) cnt[wave] = c;
   __syncthreads();
   if (threadIdx.x == 0) a.tile_count[tile] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
-}
-
-// The events left in doubt: one thread per group of 64 (almost all have none),
-// each doubtful event's momenta and PhaseSpaceWeight formed and the
-// reference's test applied (FlagDaugthers, Decays.inl:75-89); the flag word
-// corrected and the tile's count adjusted.
-template <int N, int ENGINE>
-__global__ __launch_bounds__(kBlock) void k_uw_patch_chain(const UnweightArgs a,
-                                                           const uint64_t* unc) {
-#pragma clang fp contract(off)
-  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
-  const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (g >= ngroups) return;
-  uint64_t m = *(glob(unc) + g);
-  if (m == 0) return;
-  const double rmax = 1.0 / a.maxv;
-  const bool rmax_ok = __builtin_fabs(rmax) >= 0x1p-1000 && __builtin_fabs(rmax) <= 0x1p+1000;
-  uint64_t fw = *(glob(a.flags) + g);
-  int delta = 0;
-  while (m) {
-    const int l = __builtin_ctzll(m);
-    m &= m - 1;
-    const uint64_t k = g * kWave + (uint64_t)l;
-    LaneEventWords<N, ENGINE> lw;
-    lw.init(a.d, a.gen_first + k);
-    double p[N][4];
-    genbod_f64<N, ENGINE>(a.d, lw, p);
-    if (a.d.do_boost) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) boost_folded(p[i], a.d);
-    }
-    const double u = (double)squares3_y(a.uw_key, a.uw_key * (a.jump + k)) * 0x1p-32;
-    const bool acc =
-        ratio_above(event_weight_f64<N>(p, a.d.m, a.d.maxw), a.maxv, rmax, rmax_ok, u);
-    const bool was = (fw >> l) & 1ULL;
-    if (acc != was) {
-      fw ^= 1ULL << l;
-      delta += acc ? 1 : -1;
+  if (wave == 0) {
+    // lane t < 16: group tile * 16 + t
+    constexpr int G = kUwTile / kWave;
+    const uint64_t m = lane < G ? dm[lane] : 0;
+    const uint64_t has = __ballot(m != 0);
+    if (has) {
+      uint64_t base = 0;
+      if (lane == 0) base = atomicAdd((unsigned long long*)dl_count, (unsigned long long)__popcll(has));
+      base = first_lane(base);
+      if (m) {
+        const uint64_t slot = base + (uint64_t)__popcll(has & ((1ULL << lane) - 1));
+        *(glob(dl) + 2 * slot) = tile * G + (uint64_t)lane;
+        *(glob(dl) + 2 * slot + 1) = m;
+      }
     }
   }
-  *(glob(a.flags) + g) = fw;
-  if (delta) atomicAdd(a.tile_count + g / (kUwTile / kWave), (uint32_t)delta);
 }
+
+// The events left in doubt: each wave takes entries of the list (a group
+// and the mask of its doubtful events; a group appears once) in turn, forms
+// each doubtful event's momenta and PhaseSpaceWeight in its own lane and
+// applies the reference's test (FlagDaugthers, Decays.inl:75-89); the flag
+// word is corrected and the tile's count adjusted.  The entry count is read
+// once, so every wave ends after at most count / (waves in the grid) + 1
+// entries.
+template <int N, int ENGINE>
+__global__ __launch_bounds__(kBlock) void k_uw_patch_chain(const UnweightArgs a,
+                                                           const uint64_t* dl_count,
+                                                           const uint64_t* dl) {
+#pragma clang fp contract(off)
+  const int lane = __lane_id();
+  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
+  const uint64_t w0 =
+      (uint64_t)blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t count = first_lane(*(glob(dl_count)));
+  const double rmax = 1.0 / a.maxv;
+  const bool rmax_ok = __builtin_fabs(rmax) >= 0x1p-1000 && __builtin_fabs(rmax) <= 0x1p+1000;
+  for (uint64_t e = w0; e < count; e += nwaves) {
+    const uint64_t g = first_lane(*(glob(dl) + 2 * e));
+    const uint64_t m = first_lane(*(glob(dl) + 2 * e + 1));
+    const bool mine = (m >> lane) & 1ULL;
+    bool acc = false;
+    if (mine) {
+      const uint64_t k = g * kWave + (uint64_t)lane;
+      LaneEventWords<N, ENGINE> lw;
+      lw.init(a.d, a.gen_first + k);
+      double p[N][4];
+      genbod_f64<N, ENGINE>(a.d, lw, p);
+      if (a.d.do_boost) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) boost_folded(p[i], a.d);
+      }
+      const double u = (double)squares3_y(a.uw_key, a.uw_key * (a.jump + k)) * 0x1p-32;
+      acc = ratio_above(event_weight_f64<N>(p, a.d.m, a.d.maxw), a.maxv, rmax, rmax_ok, u);
+    }
+    const uint64_t ball = __ballot(mine && acc);
+    if (lane == 0) {
+      const uint64_t fw = *(glob(a.flags) + g);
+      const uint64_t nw = (fw & ~m) | ball;
+      *(glob(a.flags) + g) = nw;
+      const int delta = __popcll(nw) - __popcll(fw);
+      if (delta) atomicAdd(a.tile_count + g / (kUwTile / kWave), (uint32_t)delta);
+    }
+  }
+}
+
+__global__ void k_zero_count(uint64_t* p) { *p = 0; }
 
 constexpr int kNumN = HPSP_MAX_N - 1;
 constexpr int kNumE = 6;
 
+// Blocks of the list kernel: enough waves to fill the chip at its occupancy
+// when every group is in the list, few enough that an almost empty list
+// costs little.
+constexpr uint64_t kPatchBlocks = 2048;
+
 template <int N, int E>
-static void chain_one(const UnweightArgs& a, uint64_t* unc, double scale2, double kc,
-                      hipStream_t s) {
+static void chain_one(const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl, double scale2,
+                      double kc, hipStream_t s) {
   const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
   const unsigned grid = (unsigned)((ntiles + kXcds - 1) / kXcds * kXcds);
-  hipLaunchKernelGGL((k_uw_flags_chain<N, E>), dim3(grid), dim3(kBlock), 0, s, a, unc, scale2,
-                     kc);
-  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
-  hipLaunchKernelGGL((k_uw_patch_chain<N, E>), dim3((unsigned)((ngroups + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, s, a, (const uint64_t*)unc);
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, dl_count);
+  hipLaunchKernelGGL((k_uw_flags_chain<N, E>), dim3(grid), dim3(kBlock), 0, s, a, dl_count, dl,
+                     scale2, kc);
+  const unsigned pgrid = (unsigned)(ntiles < kPatchBlocks ? ntiles : kPatchBlocks);
+  hipLaunchKernelGGL((k_uw_patch_chain<N, E>), dim3(pgrid), dim3(kBlock), 0, s, a,
+                     (const uint64_t*)dl_count, (const uint64_t*)dl);
 }
 
 template <int I>
@@ -263,15 +330,16 @@ static constexpr auto chain_at = &chain_one<2 + I / kNumE, I % kNumE>;
 
 template <int... I>
 static constexpr auto chain_table(std::integer_sequence<int, I...>) {
-  return std::array<void (*)(const UnweightArgs&, uint64_t*, double, double, hipStream_t),
+  return std::array<void (*)(const UnweightArgs&, uint64_t*, uint64_t*, double, double,
+                              hipStream_t),
                     sizeof...(I)>{chain_at<I>...};
 }
 
 static const auto kChain = chain_table(std::make_integer_sequence<int, kNumN * kNumE>{});
 
-int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* unc, double scale2,
-                       double kc, hipStream_t s) {
-  kChain[(N - 2) * kNumE + engine](a, unc, scale2, kc, s);
+int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl,
+                       double scale2, double kc, hipStream_t s) {
+  kChain[(N - 2) * kNumE + engine](a, dl_count, dl, scale2, kc, s);
   return (int)hipGetLastError();
 }
 

@@ -207,10 +207,12 @@ int launch_uw_scans(const UnweightArgs& a, hipStream_t s);
 // Flags of Decays::Unweight for events a.gen_first.. of a Generate with decay
 // a.d, decided from each event's generation weight where it is far enough
 // from the boundary (generate_unweight.hip); scale2 bounds E^2 + p^2 of any
-// subsystem, kc the momenta's error constant; unc: one word per group of 64
-// events (the events left in doubt, decided by a second kernel).
-int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* unc, double scale2,
-                       double kc, hipStream_t s);
+// subsystem, kc the momenta's error constant 2K + 4 of the squares engines
+// (the Random123 ones take theirs per event); dl_count and dl: a counter and
+// a list of two words per group of 64 events (the groups with events left in
+// doubt, decided by a second kernel).
+int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl,
+                       double scale2, double kc, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s);

@@ -87,14 +87,20 @@ def test_generate_unweight_headline_decay_many_blocks():
     assert torch.equal(out2, out3)
 
 
-@pytest.mark.parametrize("case", ["threshold3", "threshold10", "light", "philox_threshold"])
+@pytest.mark.parametrize("case", ["threshold3", "threshold10", "light", "philox_threshold",
+                                  "tiny_first", "tiny_12"])
 def test_generate_unweight_ill_conditioned(case):
+    """Near threshold and light daughters the bound is large and many events
+    take the exact test; with a first daughter of 1 keV the bound is far above
+    its limit and every event does."""
     M3 = 3.0969 + 0.493677 + 0.13957
     decays = {
         "threshold3": (M3 + 0.001, [3.0969, 0.493677, 0.13957], "squares3"),
         "threshold10": (10.0, [0.999] * 10, "squares3"),
         "light": (100.0, [0.000511, 0.000511, 90.0], "squares4"),
         "philox_threshold": (M3 + 0.01, [3.0969, 0.493677, 0.13957], "philox"),
+        "tiny_first": (5.279, [1e-6, 0.493677, 0.13957], "squares3"),
+        "tiny_12": (20.0, [1e-6] + [0.5] * 11, "threefry"),
     }
     M, m, engine = decays[case]
     dec = make_decay(M, m, engine=engine)
