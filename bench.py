@@ -123,7 +123,7 @@ def run_c2(torch, capi, n, steps, warmup):
     nacc2 = torch.zeros(1, dtype=torch.int64, device=dev)
     sh = capi.stream_handle()
     T = {k: Timer(torch) for k in ("weights", "gen", "uw", "flags", "move", "cols", "step",
-                                   "chain")}
+                                   "chain", "store")}
 
     def step(with_output=True):
         capi.check(lib.hpsp_generate_unweight(ctypes.byref(dec), SEED, 0, n, UW_SEED,
@@ -172,6 +172,9 @@ def run_c2(torch, capi, n, steps, warmup):
         timed("step", step)
         timed("chain", step, False)
         timed("weights", gen_weights)
+        # a plain store of the step's 12.9 GB (torch's fill of 12 columns):
+        # what HBM takes for the regenerating scatter's writes alone
+        timed("store", cols.fill_, 0.5)
         timed("gen", gen)
         timed("uw", unweight)
         timed("flags", unweight, False)
@@ -204,13 +207,14 @@ def run_c2(torch, capi, n, steps, warmup):
            "generate_ms": ms_of["gen"], "eager_step_ms": ms_of["gen"] + ms_of["uw"],
            "move_unweight_ms": ms_of["move"], "move_scatter_ms": move_ms,
            "move_step_ms": ms_of["gen"] + ms_of["move"],
-           "generate_momenta_only_ms": ms_of["cols"]}
+           "generate_momenta_only_ms": ms_of["cols"], "plain_store_ms": ms_of["store"]}
     gen_bytes = n * (32 * N + 8)
     res["generate_weights_GBs"] = n * 8 / (ms_of["weights"] * 1e6)
     regen_bytes = n * 32 * N
     move_bytes = n * 64 * N
     res["generate_GBs"] = gen_bytes / (ms_of["gen"] * 1e6)
     res["regen_scatter_GBs"] = regen_bytes / (regen_ms * 1e6)
+    res["plain_store_GBs"] = regen_bytes / (ms_of["store"] * 1e6)
     res["move_scatter_GBs"] = move_bytes / (move_ms * 1e6)
     res["generate_bytes_per_launch"] = gen_bytes
     res["regen_scatter_bytes_per_launch"] = regen_bytes
@@ -940,6 +944,9 @@ def main():
             "unit": "GB/s", "frac": c2["regen_scatter_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "k_uw_regen<3, 0>", "ms": c2["regen_scatter_ms"],
             "algorithmic_bytes_per_launch": c2["regen_scatter_bytes_per_launch"],
+            "plain_store": {"kernel": "torch fill of the 12 output columns (12.9 GB, no reads)",
+                            "ms": c2["plain_store_ms"], "achieved": c2["plain_store_GBs"],
+                            "regen_over_plain_store": c2["regen_scatter_GBs"] / c2["plain_store_GBs"]},
             "flags_pass": {"kernel": "k_uw_flags_chain<3, 0> + scans",
                            "bound": "valu (engine words, weight chain)",
                            "ms": c2["chain_flags_scan_ms"],
