@@ -182,6 +182,12 @@ def run_c2(torch, capi, n, steps, warmup):
         timed("cols", gen_cols)
     torch.cuda.synchronize()
     same = bool(torch.equal(out, out2)) and int(nacc.item()) == int(nacc2.item())
+    # groups of 64 events with events the chain left to the exact test: the
+    # counter of hpsp_generate_unweight's list, which follows hpsp_unweight's
+    # part of the workspace (capi.hip)
+    step()
+    off = lib.hpsp_unweight_workspace_size(n)
+    doubt_groups = int(ws[off:off + 8].cpu().view(torch.int64).item())
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
@@ -200,7 +206,7 @@ def run_c2(torch, capi, n, steps, warmup):
     regen2_ms = ms_of["uw"] - ms_of["flags"]
     move_ms = ms_of["move"] - ms_of["flags"]
     res = {"ms_per_step": ms, "events_per_s": n / (ms / 1000.0), "n_acc": int(nacc2.item()),
-           "same_as_two_pass": same,
+           "same_as_two_pass": same, "doubtful_groups": doubt_groups,
            "chain_flags_scan_ms": ms_of["chain"], "regen_scatter_ms": regen_ms,
            "two_pass_step_ms": ms_two, "generate_weights_ms": ms_of["weights"],
            "flags_scan_ms": ms_of["flags"], "two_pass_regen_scatter_ms": regen2_ms,

@@ -123,3 +123,29 @@ def test_generate_unweight_empty_and_errors():
                                         None, ptr(ws), ws_bytes, ptr(nacc), stream_handle()) != 0
     assert lib().hpsp_generate_unweight(ctypes.byref(dec), 1, 0, 1000, 2, ptr(out), 1000, None,
                                         None, ptr(ws), ws_bytes, None, stream_handle()) != 0
+
+
+def doubtful_groups(dec, n, seed=UW_SEED):
+    """The groups of 64 events the weight chain leaves to the exact test: the
+    counter of hpsp_generate_unweight's list, which follows hpsp_unweight's
+    part of the workspace (capi.hip)."""
+    ws_bytes = lib().hpsp_generate_unweight_workspace_size(n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
+    nacc = torch.zeros(1, dtype=torch.int64, device=DEV)
+    check(lib().hpsp_generate_unweight(ctypes.byref(dec), 159753, 0, n, seed, None, n, None, None,
+                                       ptr(ws), ws_bytes, ptr(nacc), stream_handle()),
+          "hpsp_generate_unweight")
+    torch.cuda.synchronize()
+    off = lib().hpsp_unweight_workspace_size(n)
+    return int(ws[off:off + 8].cpu().view(torch.int64).item())
+
+
+@pytest.mark.parametrize("engine,limit", [("squares3", 1e-3), ("squares4", 1e-3),
+                                          ("philox", 1e-3), ("threefry_long", 1e-3)])
+def test_chain_decides_almost_every_flag(engine, limit):
+    """On the headline decay the exact test runs for almost no group (every
+    engine: the Random123 ones take their bound's K per event)."""
+    dec = make_decay(5.279, [3.0969, 0.493677, 0.13957], engine=engine)
+    n = 1 << 22
+    g = doubtful_groups(dec, n)
+    assert g <= limit * (n // 64), (engine, g)
