@@ -1,11 +1,13 @@
 """Parity at the benchmarked sizes, against the oracle.
 
-* The headline step (3-body decay, 2^27 events: the weights pass of Generate,
-  then Decays::Unweight() by regeneration, as bench.py times it): the accepted
-  count and an order-sensitive digest of the whole stable partition (output
-  row -> container index) equal the oracle's streaming Generate + Unweight
-  over the same 2^27 events (Decays.inl:415-455), and the momenta of sampled
-  output rows equal the oracle's events at their indices within 1e-12 E.
+* The headline step (3-body decay, 2^27 events: Generate followed by
+  Decays::Unweight() in one call, hpsp_generate_unweight, as bench.py times
+  it: flags from each event's weight chain, the exact test where in doubt,
+  every event generated at its place): the accepted count and an
+  order-sensitive digest of the whole stable partition (output row ->
+  container index) equal the oracle's streaming Generate + Unweight over the
+  same 2^27 events (Decays.inl:415-455), and the momenta of sampled output
+  rows equal the oracle's events at their indices within 1e-12 E.
 * AverageOn of the three-Breit-Wigner Dalitz functor over 2^30 events in 64
   chunks of 2^24 (PhaseSpace.inl:255-273): mean within 1e-12 relative of the
   oracle's single pass over the same events, sigma within 1e-10.
@@ -18,8 +20,7 @@ import numpy as np
 import pytest
 import torch
 
-from gpu_util import (DEV, FP64, average_on, bw_functor, make_decay, momentum_error, ptr,
-                      unweight_regenerate)
+from gpu_util import DEV, average_on, bw_functor, make_decay, momentum_error, ptr
 from hydra_amd import capi
 
 M3, MASSES3 = 5.279, [3.0969, 0.493677, 0.13957]
@@ -34,11 +35,18 @@ def threads():
 def test_headline_step_partition_at_2p27(oracle):
     n = 1 << 27
     dec = make_decay(M3, MASSES3)
-    evw = torch.empty(n, dtype=torch.float64, device=DEV)
-    capi.check(capi.lib().hpsp_generate(ctypes.byref(dec), SEED, 0, n, FP64, None, n, 0, None, ptr(evw),
-                                        capi.stream_handle()), "hpsp_generate (weights)")
-    nacc, out, idx, w_out = unweight_regenerate(dec, SEED, 0, n, evw, UW_SEED, want_w=False)
-    del evw
+    lib = capi.lib()
+    ws_bytes = lib.hpsp_generate_unweight_workspace_size(n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
+    out = torch.empty((12, n), dtype=torch.float64, device=DEV)
+    idx = torch.empty(n, dtype=torch.int64, device=DEV)
+    nacc_d = torch.zeros(1, dtype=torch.int64, device=DEV)
+    capi.check(lib.hpsp_generate_unweight(ctypes.byref(dec), SEED, 0, n, UW_SEED, ptr(out), n,
+                                          ptr(idx), None, ptr(ws), ws_bytes, ptr(nacc_d),
+                                          capi.stream_handle()), "hpsp_generate_unweight")
+    torch.cuda.synchronize()
+    nacc = int(nacc_d.item())
+    del ws
     idx_h = idx.cpu().numpy().view(np.uint64)
     print(f"gpu: {nacc} accepted of {n}", flush=True)
     onacc, odig = oracle.unweight_digest(M3, MASSES3, SEED, n, UW_SEED, nthreads=threads())
