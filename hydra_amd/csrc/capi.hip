@@ -639,12 +639,16 @@ int hpsp_unweight_regenerate(const hpsp_decay* d, uint64_t gen_seed, uint64_t fi
   return hip_status(launch_unweight_regen(d->n, d->engine, a, s));
 }
 
-// hpsp_unweight's workspace, a counter and a list of two words per group of
-// 64 events (the groups with events the weight chain leaves in doubt)
+// hpsp_unweight's workspace, a counter, a list of two words per group of 64
+// events (the groups with events the weight chain leaves in doubt) and the
+// 3-body bound's table
 static size_t gu_list_offset(uint64_t n) { return uw_layout(n).total + align_up(sizeof(uint64_t)); }
-size_t hpsp_generate_unweight_workspace_size(uint64_t n) {
+static size_t gu_table_offset(uint64_t n) {
   const uint64_t ngroups = (n + kWave - 1) / kWave;
   return gu_list_offset(n) + align_up(2 * ngroups * sizeof(uint64_t));
+}
+size_t hpsp_generate_unweight_workspace_size(uint64_t n) {
+  return gu_table_offset(n) + align_up(2 * kBin3 * sizeof(double));
 }
 
 int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t first, uint64_t n,
@@ -661,6 +665,7 @@ int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t firs
   char* base = static_cast<char*>(ws);
   uint64_t* dl_count = reinterpret_cast<uint64_t*>(base + uw_layout(n).total);
   uint64_t* dl = reinterpret_cast<uint64_t*>(base + gu_list_offset(n));
+  double* tab3 = reinterpret_cast<double*>(base + gu_table_offset(n));
   hipStream_t s = as_stream(stream);
   UnweightArgs a;
   memset(&a, 0, sizeof(a));
@@ -696,7 +701,8 @@ int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t firs
   }
   // K of the squares engines (the Random123 ones set theirs per event)
   const double K = 0x1p17 * d->n;
-  rc = hip_status(launch_flags_chain(d->n, d->engine, a, dl_count, dl, scale2, 2.0 * K + 4.0, s));
+  rc = hip_status(
+      launch_flags_chain(d->n, d->engine, a, dl_count, dl, tab3, scale2, 2.0 * K + 4.0, s));
   if (rc) return rc;
   rc = hip_status(launch_uw_scans(a, s));
   if (rc) return rc;

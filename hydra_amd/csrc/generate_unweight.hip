@@ -161,6 +161,128 @@ __device__ __forceinline__ double chain_q2(const DecayK& d, const WordFn& word, 
   return C * P * rcp_fast(A);
 }
 
+// ---- the 3-body bound as a table ----------------------------------------
+// For N = 3 the bound depends on one per-event variable, the mass uniform r:
+// a0 = invMas[1] = r ecm + (m0 + m1); step 0 is pdk(a0; m1, m0), step 1
+// pdk(A; m2, a0) with A = invMas[2] the same for every event.  r's leading
+// 32 bits v pick a bin: the nearer end of [0, 1] (v >> 31) and the distance t
+// to it (v or ~v), in octaves of t cut in 8 (each t < 8 a bin of its own), so
+// bins are 1/8 of their distance to the end wide where B grows like 1/t, and
+// 1/32 of [0, 1] in the middle.  Over a bin r lies in [r_lo, r_hi] (also for
+// 64-bit words, whose 53-bit rounding stays in the bin's closed interval) and
+// a0 in [a_lo, a_hi], formed by the chain's own rounded operations, which are
+// monotonic in r.  Each term of B is bounded over the bin by its numerator at
+// a_hi and its denominator at a_lo (step 1: d2's (m2 - c)^2 at whichever end
+// is larger, a convex function of c), and 1/a^2, 1/c^2 at a_lo: B <= ks X + Y
+// with ks = (2K + 4) eps S^2 (per event for the Random123 engines),
+// X = kk0 (1/a_lo^2 + 1/m0^2) + kk1 (1/A^2 + 1/a_lo^2), Y = 4 eps (kk0 + kk1)
+// + 120 eps; a bin where a denominator is not positive gets infinity.
+__device__ __forceinline__ int bin3(uint32_t v) {
+  const uint32_t side = v >> 31;
+  const uint32_t t = side ? ~v : v;  // < 2^31
+  int idx;
+  if (t < 8) {
+    idx = (int)t;
+  } else {
+    const int e = 31 - __builtin_clz(t);  // 3 .. 30
+    idx = (e - 2) * 8 + (int)((t >> (e - 3)) & 7u);
+  }
+  return (int)side * kBin3Side + idx;
+}
+
+__global__ __launch_bounds__(kBlock) void k_chain_table3(const DecayK d, double* X, double* Y) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= kBin3) return;
+  const int side = i / kBin3Side, idx = i % kBin3Side;
+  uint64_t tlo, thi;
+  if (idx < 8) {
+    tlo = thi = (uint64_t)idx;
+  } else {
+    const int e = idx / 8 + 2, mb = idx % 8;
+    tlo = (uint64_t)(8 + mb) << (e - 3);
+    thi = ((uint64_t)(9 + mb) << (e - 3)) - 1;
+  }
+  double rlo, rhi;
+  if (side == 0) {
+    rlo = (double)tlo * 0x1p-32;
+    rhi = (double)(thi + 1) * 0x1p-32;
+  } else {
+    rlo = (double)(0xFFFFFFFFull - thi) * 0x1p-32;
+    rhi = (double)(0x100000000ull - tlo) * 0x1p-32;
+  }
+  // the chain's sums and a0 = r ecm + (m0 + m1), A = 1 ecm + (m0 + m1 + m2)
+  double sum = 0.0;
+  sum += d.m[0];
+  sum += d.m[1];
+  const double s01 = sum;
+  sum += d.m[2];
+  const double A = 1.0 * d.ecm + sum;
+  const double alo = rlo * d.ecm + s01, ahi = rhi * d.ecm + s01;
+  const double m0 = d.m[0], m1 = d.m[1], m2 = d.m[2];
+  constexpr double eps = 0x1p-53;
+  const double inf = __builtin_huge_val();
+  // step 0: a in [alo, ahi], b = m1, c = m0
+  const double spc0 = m1 + m0, smc0 = m1 - m0;
+  const double den1 = (alo - spc0) * (alo + spc0), den2 = (alo - smc0) * (alo + smc0);
+  const double ah2 = ahi * ahi, al2 = alo * alo;
+  const double kk0 = (den1 > 0.0 && den2 > 0.0)
+                         ? (ah2 + m0 * __builtin_fabs(smc0)) / den2 + (ah2 + m0 * spc0) / den1 + 1.0
+                         : inf;
+  // step 1: a = A, b = m2, c in [alo, ahi]
+  const double A2 = A * A;
+  const double e1 = (A - (m2 + ahi)) * (A + (m2 + ahi));
+  const double dl = m2 - alo, dh = m2 - ahi;
+  const double dm = __builtin_fmax(__builtin_fabs(dl), __builtin_fabs(dh));
+  const double e2 = (A - dm) * (A + dm);
+  const double kk1 = (e1 > 0.0 && e2 > 0.0)
+                         ? (A2 + ahi * dm) / e2 + (A2 + ahi * (m2 + ahi)) / e1 + 1.0
+                         : inf;
+  const double x = kk0 * (1.0 / al2 + 1.0 / (m0 * m0)) + kk1 * (1.0 / A2 + 1.0 / al2);
+  const double y = 4.0 * eps * (kk0 + kk1) + 120.0 * eps;
+  // the few roundings of these operations
+  X[i] = x * (1.0 + 0x1p-40);
+  Y[i] = y * (1.0 + 0x1p-40);
+}
+
+// The 3-body chain: q^2 as chain_q2 forms it, with the step of the common A
+// folded into C3 = C / A^2, and B from the table.
+template <int ENGINE, typename WordFn>
+__device__ __forceinline__ double chain3_q2(const DecayK& d, const WordFn& word, double C3,
+                                            double ks_fixed, double ks_unit, const double* X,
+                                            const double* Y, double& B) {
+#pragma clang fp contract(off)
+  constexpr double eps = 0x1p-53;
+  const uint64_t w0 = word(0);
+  const uint32_t v = is_squares<ENGINE>() ? (uint32_t)w0 : (uint32_t)(w0 >> 32);
+  double sum = 0.0;
+  sum += d.m[0];
+  sum += d.m[1];
+  const double s01 = sum;
+  sum += d.m[2];
+  const double A = 1.0 * d.ecm + sum;
+  const double a0 = to_unit<ENGINE>(w0) * d.ecm + s01;
+  const double m0 = d.m[0], m1 = d.m[1], m2 = d.m[2];
+  const double spc0 = m1 + m0, smc0 = m1 - m0;
+  const double x0 = ((a0 - spc0) * (a0 + spc0)) * ((a0 - smc0) * (a0 + smc0));
+  const double spc1 = m2 + a0, smc1 = m2 - a0;
+  const double x1 = ((A - spc1) * (A + spc1)) * ((A - smc1) * (A + smc1));
+  double ks = ks_fixed;
+  if constexpr (!is_squares<ENGINE>()) {
+    double s2 = 1.0;
+#pragma unroll
+    for (int i = 1; i < 3; ++i) {
+      const double cz = 2 * to_unit<ENGINE>(word(1 + 2 * (i - 1))) - 1;
+      s2 = __builtin_fmin(s2, (1.0 - cz) * (1.0 + cz));
+    }
+    const double inv_s = __builtin_amdgcn_rsq(__builtin_fmax(s2 - 4.0 * eps, 0x1p-120));
+    ks = ks_unit * (2.0 * 3 * (64.0 + 4.0 * inv_s) + 4.0);
+  }
+  const int b = bin3(v);
+  B = __builtin_fma(ks, X[b], Y[b]);
+  return C3 * (x0 * x1) * rcp_fast(a0 * a0);
+}
+
 }  // namespace
 
 // One 1024-event tile per block, items as k_uw_flags: thread item j is event
@@ -173,7 +295,7 @@ __device__ __forceinline__ double chain_q2(const DecayK& d, const WordFn& word, 
 template <int N, int ENGINE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
 void k_uw_flags_chain(const UnweightArgs a, uint64_t* dl_count, uint64_t* dl, double scale2,
-                      double kc) {
+                      double kc, const double* tab3) {
 #pragma clang fp contract(off)
   __shared__ uint32_t cnt[kBlock / kWave];
   __shared__ uint64_t dm[kUwTile / kWave];  // doubt mask of each group of the tile
@@ -193,6 +315,14 @@ void k_uw_flags_chain(const UnweightArgs a, uint64_t* dl_count, uint64_t* dl, do
   const double C = a.d.maxw * a.d.maxw * rmax2 * __builtin_ldexp(1.0, -2 * (N - 1));
   // the squared test needs q >= 0: max and maxw positive
   const bool C_ok = rmax_ok && a.maxv > 0.0 && a.d.maxw > 0.0 && C >= 0x1p-900 && C <= 0x1p+900;
+  // 3 bodies: the table (bins of X, then of Y) and C / A^2
+  [[maybe_unused]] const double* X3 = tab3;
+  [[maybe_unused]] const double* Y3 = tab3 + kBin3;
+  [[maybe_unused]] double C3 = 0.0;
+  if constexpr (N == 3) {
+    const double A = 1.0 * a.d.ecm + ((0.0 + a.d.m[0]) + a.d.m[1] + a.d.m[2]);
+    C3 = C / (A * A);
+  }
   const uint64_t off = (uint64_t)wave * kWave + (uint64_t)lane;
   // key * (jump + k) of item 0 for the accept word and key * (gen_first + k +
   // 3N) for the generation word; item j + 1 is kBlock events on
@@ -213,7 +343,13 @@ void k_uw_flags_chain(const UnweightArgs a, uint64_t* dl_count, uint64_t* dl, do
     if constexpr (N == 3 && is_squares<ENGINE>()) {
       // the one uniform is the lane's own word: counter gen_first + k + 3N
       const OneWord w{squares_word_y<ENGINE>(a.d.key[0], ygen)};
-      q2 = chain_q2<N, ENGINE>(a.d, w, C, ks_fixed, ks_unit, ic2_0, B, sure);
+      q2 = chain3_q2<ENGINE>(a.d, w, C3, ks_fixed, ks_unit, X3, Y3, B);
+      sure = B < 0x1p-10;  // false for NaN
+    } else if constexpr (N == 3) {
+      EventWords<N, ENGINE> words;
+      words.init(a.d, a.gen_first + k0, lane);
+      q2 = chain3_q2<ENGINE>(a.d, words, C3, ks_fixed, ks_unit, X3, Y3, B);
+      sure = B < 0x1p-10;
     } else {
       EventWords<N, ENGINE> words;
       words.init(a.d, a.gen_first + k0, lane);
@@ -313,13 +449,16 @@ constexpr int kNumE = 6;
 constexpr uint64_t kPatchBlocks = 2048;
 
 template <int N, int E>
-static void chain_one(const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl, double scale2,
-                      double kc, hipStream_t s) {
+static void chain_one(const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl, double* tab3,
+                      double scale2, double kc, hipStream_t s) {
   const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
   const unsigned grid = (unsigned)((ntiles + kXcds - 1) / kXcds * kXcds);
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, dl_count);
+  if constexpr (N == 3)
+    hipLaunchKernelGGL(k_chain_table3, dim3((kBin3 + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       a.d, tab3, tab3 + kBin3);
   hipLaunchKernelGGL((k_uw_flags_chain<N, E>), dim3(grid), dim3(kBlock), 0, s, a, dl_count, dl,
-                     scale2, kc);
+                     scale2, kc, (const double*)tab3);
   const unsigned pgrid = (unsigned)(ntiles < kPatchBlocks ? ntiles : kPatchBlocks);
   hipLaunchKernelGGL((k_uw_patch_chain<N, E>), dim3(pgrid), dim3(kBlock), 0, s, a,
                      (const uint64_t*)dl_count, (const uint64_t*)dl);
@@ -330,7 +469,7 @@ static constexpr auto chain_at = &chain_one<2 + I / kNumE, I % kNumE>;
 
 template <int... I>
 static constexpr auto chain_table(std::integer_sequence<int, I...>) {
-  return std::array<void (*)(const UnweightArgs&, uint64_t*, uint64_t*, double, double,
+  return std::array<void (*)(const UnweightArgs&, uint64_t*, uint64_t*, double*, double, double,
                               hipStream_t),
                     sizeof...(I)>{chain_at<I>...};
 }
@@ -338,8 +477,8 @@ static constexpr auto chain_table(std::integer_sequence<int, I...>) {
 static const auto kChain = chain_table(std::make_integer_sequence<int, kNumN * kNumE>{});
 
 int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl,
-                       double scale2, double kc, hipStream_t s) {
-  kChain[(N - 2) * kNumE + engine](a, dl_count, dl, scale2, kc, s);
+                       double* tab3, double scale2, double kc, hipStream_t s) {
+  kChain[(N - 2) * kNumE + engine](a, dl_count, dl, tab3, scale2, kc, s);
   return (int)hipGetLastError();
 }
 

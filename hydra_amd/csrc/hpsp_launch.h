@@ -210,9 +210,12 @@ int launch_uw_scans(const UnweightArgs& a, hipStream_t s);
 // subsystem, kc the momenta's error constant 2K + 4 of the squares engines
 // (the Random123 ones take theirs per event); dl_count and dl: a counter and
 // a list of two words per group of 64 events (the groups with events left in
-// doubt, decided by a second kernel).
+// doubt, decided by a second kernel); tab3: 2 kBin3 doubles, the 3-body
+// bound's table (written and read by the call when N = 3).
+constexpr int kBin3Side = 8 + 28 * 8;  // bins per end of [0, 1]
+constexpr int kBin3 = 2 * kBin3Side;
 int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl,
-                       double scale2, double kc, hipStream_t s);
+                       double* tab3, double scale2, double kc, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s);

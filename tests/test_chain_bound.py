@@ -161,3 +161,137 @@ def test_conditioning_is_needed(oracle):
 def test_engine_names_cover_the_kernels():
     assert set(ENGINES) == {"squares3", "squares4", "philox", "philox_long", "threefry",
                             "threefry_long"}
+
+
+# ---- the 3-body bound as a table (generate_unweight.hip, k_chain_table3) ----
+BIN3_SIDE = 8 + 28 * 8
+BIN3 = 2 * BIN3_SIDE
+
+
+def bin3(v):
+    """Bin of the mass uniform's leading 32 bits v (uint64 array)."""
+    v = np.asarray(v, dtype=np.uint64)
+    side = v >> np.uint64(31)
+    t = np.where(side == 1, np.uint64(0xFFFFFFFF) - v, v)
+    e = np.zeros_like(t, dtype=np.int64)
+    big = t >= 8
+    e[big] = np.floor(np.log2(t[big].astype(np.float64))).astype(np.int64)
+    # correct log2 rounding at powers of two
+    e[big] -= (np.uint64(1) << e[big].astype(np.uint64)) > t[big]
+    e[big] += (np.uint64(1) << (e[big] + 1).astype(np.uint64)) <= t[big]
+    idx = t.astype(np.int64)
+    sh = (e[big] - 3).astype(np.uint64)
+    idx[big] = (e[big] - 2) * 8 + ((t[big] >> sh) & np.uint64(7)).astype(np.int64)
+    return side.astype(np.int64) * BIN3_SIDE + idx
+
+
+def table3(oracle, M, masses):
+    """(X, Y) per bin: B <= ks X + Y over the bin."""
+    m0, m1, m2 = (float(x) for x in masses)
+    rc, ecm, _ = oracle.params(M, masses)
+    X = np.zeros(BIN3)
+    Y = np.zeros(BIN3)
+    s01 = (0.0 + m0) + m1
+    A = 1.0 * ecm + (s01 + m2)
+    for i in range(BIN3):
+        side, idx = divmod(i, BIN3_SIDE)
+        if idx < 8:
+            tlo = thi = idx
+        else:
+            e, mb = idx // 8 + 2, idx % 8
+            tlo = (8 + mb) << (e - 3)
+            thi = ((9 + mb) << (e - 3)) - 1
+        if side == 0:
+            rlo, rhi = tlo * 2.0 ** -32, (thi + 1) * 2.0 ** -32
+        else:
+            rlo, rhi = (0xFFFFFFFF - thi) * 2.0 ** -32, (0x100000000 - tlo) * 2.0 ** -32
+        alo, ahi = rlo * ecm + s01, rhi * ecm + s01
+        spc0, smc0 = m1 + m0, m1 - m0
+        den1, den2 = (alo - spc0) * (alo + spc0), (alo - smc0) * (alo + smc0)
+        kk0 = ((ahi * ahi + m0 * abs(smc0)) / den2 + (ahi * ahi + m0 * spc0) / den1 + 1.0
+               if den1 > 0 and den2 > 0 else np.inf)
+        e1 = (A - (m2 + ahi)) * (A + (m2 + ahi))
+        dm = max(abs(m2 - alo), abs(m2 - ahi))
+        e2 = (A - dm) * (A + dm)
+        kk1 = ((A * A + ahi * dm) / e2 + (A * A + ahi * (m2 + ahi)) / e1 + 1.0
+               if e1 > 0 and e2 > 0 else np.inf)
+        X[i] = kk0 * (1 / (alo * alo) + 1 / (m0 * m0)) + kk1 * (1 / (A * A) + 1 / (alo * alo))
+        Y[i] = 4 * EPS * (kk0 + kk1) + 120 * EPS
+    return X * (1 + 2.0 ** -40), Y * (1 + 2.0 ** -40)
+
+
+THREE_BODY = [("headline", 5.279, [3.0969, 0.493677, 0.13957]),
+              ("threshold3", 3.0969 + 0.493677 + 0.13957 + 0.001, [3.0969, 0.493677, 0.13957]),
+              ("light", 100.0, [0.000511, 0.000511, 90.0]),
+              ("heavy_last", 10.0, [0.1, 0.2, 8.0]),
+              ("equal", 3.0, [0.5, 0.5, 0.5])]
+
+
+def test_bin3_edges():
+    v = np.array([0, 1, 7, 8, 15, 16, 2 ** 30, 2 ** 31 - 1, 2 ** 31, 2 ** 32 - 8, 2 ** 32 - 1],
+                 dtype=np.uint64)
+    b = bin3(v)
+    assert list(b[:6]) == [0, 1, 7, 8, 15, 16]
+    assert b[6] == 28 * 8 and b[7] == 28 * 8 + 7
+    assert b[8] == BIN3_SIDE + 28 * 8 + 7 and b[-1] == BIN3_SIDE and b[-2] == BIN3_SIDE + 7
+    assert b.min() >= 0 and b.max() < BIN3
+
+
+@pytest.mark.parametrize("case", THREE_BODY, ids=[c[0] for c in THREE_BODY])
+def test_table3_bounds_every_event(oracle, case):
+    """Over each bin the table's X and Y bound the per-event terms of B (so
+    ks X + Y >= B for any ks >= 0): 2^20 random words plus both ends of
+    every bin, for 32- and 64-bit words."""
+    name, M, masses = case
+    m = np.asarray(masses)
+    rc, ecm, _ = oracle.params(M, m)
+    X, Y = table3(oracle, M, masses)
+    rng = np.random.default_rng(11)
+    # 64-bit words: random, and the extreme words of every 32-bit bin
+    w = rng.integers(0, 2 ** 64, 1 << 20, dtype=np.uint64)
+    edges = []
+    for i in range(BIN3):
+        side, idx = divmod(i, BIN3_SIDE)
+        if idx < 8:
+            tlo = thi = idx
+        else:
+            e, mb = idx // 8 + 2, idx % 8
+            tlo, thi = (8 + mb) << (e - 3), ((9 + mb) << (e - 3)) - 1
+        vlo, vhi = (tlo, thi) if side == 0 else (0xFFFFFFFF - thi, 0xFFFFFFFF - tlo)
+        edges += [vlo << 32, (vhi << 32) | 0xFFFFFFFF, vlo << 32 | 0x80000000]
+    w = np.concatenate([w, np.array(edges, dtype=np.uint64)])
+    for narrow in (True, False):
+        if narrow:
+            v = w >> np.uint64(32)
+            r = v.astype(np.float64) * 2.0 ** -32
+        else:
+            v = w >> np.uint64(32)
+            r = w.astype(np.float64) * 2.0 ** -64
+        b = bin3(v)
+        s01 = (0.0 + m[0]) + m[1]
+        A = 1.0 * ecm + (s01 + m[2])
+        a0 = r * ecm + s01
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xe = np.zeros_like(r)
+            ye = np.full_like(r, 120 * EPS)
+            ok = np.ones(r.shape, dtype=bool)
+            for a, bb, c, ic2 in ((a0, m[1], m[0], 1 / m[0] ** 2), (np.full_like(r, A), m[2], a0,
+                                                                      1 / a0 ** 2)):
+                spc, smc = bb + c, bb - c
+                d1, d2 = (a - spc) * (a + spc), (a - smc) * (a + smc)
+                ok &= np.minimum(d1, d2) > 0
+                kk = (a * a + c * np.abs(smc)) / d2 + (a * a + c * spc) / d1 + 1
+                xe += kk * (1 / a ** 2 + ic2)
+                ye += 4 * EPS * kk
+        sel = ok
+        assert np.all(X[b[sel]] >= xe[sel] * (1 - 1e-12)), name
+        assert np.all(Y[b[sel]] >= ye[sel] * (1 - 1e-12)), name
+        # events outside the domain (d1 or d2 not positive) land in bins of
+        # infinite bound
+        assert np.all(np.isinf(X[b[~ok]])), name
+    # the table leaves almost every event to the chain on the headline decay
+    if name == "headline":
+        ks = (2.0 * 2 ** 17 * 3 + 4) * EPS * (ecm + m.sum()) ** 2
+        Bt = ks * X + Y
+        frac = np.mean(Bt[bin3(w[:1 << 20] >> np.uint64(32))] >= 2.0 ** -10)
+        assert frac < 1e-3, frac

@@ -149,3 +149,28 @@ def test_chain_decides_almost_every_flag(engine, limit):
     n = 1 << 22
     g = doubtful_groups(dec, n)
     assert g <= limit * (n // 64), (engine, g)
+
+
+@pytest.mark.parametrize("M,masses", [(5.279, [3.0969, 0.493677, 0.13957]),
+                                      (10.0, [0.1, 0.2, 8.0])])
+def test_table3_on_device_equals_restatement(oracle, M, masses):
+    """The 3-body bound's table the call writes into its workspace (the last
+    2 x 464 doubles) equals the restatement that tests/test_chain_bound.py
+    checks against every event's bound."""
+    from test_chain_bound import BIN3, table3
+    dec = make_decay(M, masses)
+    n = 4096
+    ws_bytes = lib().hpsp_generate_unweight_workspace_size(n)
+    ws = torch.zeros(ws_bytes, dtype=torch.uint8, device=DEV)
+    nacc = torch.zeros(1, dtype=torch.int64, device=DEV)
+    check(lib().hpsp_generate_unweight(ctypes.byref(dec), 1, 0, n, UW_SEED, None, n, None, None,
+                                       ptr(ws), ws_bytes, ptr(nacc), stream_handle()),
+          "hpsp_generate_unweight")
+    torch.cuda.synchronize()
+    tab_bytes = 2 * BIN3 * 8
+    tab = ws[ws_bytes - tab_bytes:].cpu().numpy().view(np.float64)
+    X, Y = table3(oracle, M, masses)
+    for got, want in ((tab[:BIN3], X), (tab[BIN3:], Y)):
+        fin = np.isfinite(want)
+        assert np.array_equal(np.isfinite(got), fin)
+        np.testing.assert_allclose(got[fin], want[fin], rtol=1e-13, atol=0)
