@@ -699,10 +699,28 @@ int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t firs
     const double f = a.d.gamma * (1.0 + b);
     scale2 *= f * f;
   }
-  // K of the squares engines (the Random123 ones set theirs per event)
+  // the flags kernel's constants (its per-wave FP64 divisions, done once
+  // here); K of the squares engines (the Random123 ones set theirs per event)
   const double K = 0x1p17 * d->n;
-  rc = hip_status(
-      launch_flags_chain(d->n, d->engine, a, dl_count, dl, tab3, scale2, 2.0 * K + 4.0, s));
+  ChainK ck;
+  const double rmax = 1.0 / d->max_weight;
+  const bool rmax_ok = fabs(rmax) >= 0x1p-1000 && fabs(rmax) <= 0x1p+1000;
+  ck.ks_unit = 0x1p-53 * scale2;
+  ck.ks_fixed = (2.0 * K + 4.0) * ck.ks_unit;
+  ck.ic2_0 = 1.0 / (d->masses[0] * d->masses[0]);
+  ck.C = d->max_weight * d->max_weight * (rmax * rmax) * ldexp(1.0, -2 * (d->n - 1));
+  // the squared test needs q >= 0: max and maxw positive
+  ck.fast_ok = rmax_ok && d->max_weight > 0.0 && ck.C >= 0x1p-900 && ck.C <= 0x1p+900;
+  ck.A3 = 0.0;
+  ck.C3 = 0.0;
+  if (d->n == 3) {
+    // invMas[2] as the chain forms it: 1 * ecm + ((0 + m0) + m1 + m2)
+    double sum = 0.0;
+    for (int i = 0; i < 3; ++i) sum += d->masses[i];
+    ck.A3 = 1.0 * d->ecm + sum;
+    ck.C3 = ck.C / (ck.A3 * ck.A3);
+  }
+  rc = hip_status(launch_flags_chain(d->n, d->engine, a, dl_count, dl, tab3, ck, s));
   if (rc) return rc;
   rc = hip_status(launch_uw_scans(a, s));
   if (rc) return rc;

@@ -92,8 +92,9 @@ struct OneWord {
 // C = (maxw / max)^2 / 4^(N-1), and the bound B of the file comment; sure is
 // false where a term of B is not defined (d1 or d2 not positive) or B is not
 // below 2^-10 (there the terms the first-order bound leaves out are below
-// 2^-10 of it, far inside the margin K leaves).  With c the previous step's a (1/m0^2 at the first step), ka + kc
-// is (d1 (a^2 + c|b-c|) + d2 (a^2 + c(b+c))) / (d1 d2) + 1, and B needs
+// 2^-10 of it, far inside the margin K leaves).  With c the previous step's
+// a (1/m0^2 at the first step), ka + kc is
+// (d1 (a^2 + c|b-c|) + d2 (a^2 + c(b+c))) / (d1 d2) + 1, and B needs
 // only estimates of 1/(d1 d2) and 1/a^2: it is a bound with a wide margin,
 // formed with fused multiply-adds.
 template <int N, int ENGINE, typename WordFn>
@@ -176,7 +177,8 @@ __device__ __forceinline__ double chain_q2(const DecayK& d, const WordFn& word, 
 // is larger, a convex function of c), and 1/a^2, 1/c^2 at a_lo: B <= ks X + Y
 // with ks = (2K + 4) eps S^2 (per event for the Random123 engines),
 // X = kk0 (1/a_lo^2 + 1/m0^2) + kk1 (1/A^2 + 1/a_lo^2), Y = 4 eps (kk0 + kk1)
-// + 120 eps; a bin where a denominator is not positive gets infinity.
+// + 120 eps (stored as (X, Y) pairs); a bin where a denominator is not
+// positive gets infinity.
 __device__ __forceinline__ int bin3(uint32_t v) {
   const uint32_t side = v >> 31;
   const uint32_t t = side ? ~v : v;  // < 2^31
@@ -190,7 +192,7 @@ __device__ __forceinline__ int bin3(uint32_t v) {
   return (int)side * kBin3Side + idx;
 }
 
-__global__ __launch_bounds__(kBlock) void k_chain_table3(const DecayK d, double* X, double* Y) {
+__global__ __launch_bounds__(kBlock) void k_chain_table3(const DecayK d, double2* XY) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= kBin3) return;
@@ -241,33 +243,27 @@ __global__ __launch_bounds__(kBlock) void k_chain_table3(const DecayK d, double*
   const double x = kk0 * (1.0 / al2 + 1.0 / (m0 * m0)) + kk1 * (1.0 / A2 + 1.0 / al2);
   const double y = 4.0 * eps * (kk0 + kk1) + 120.0 * eps;
   // the few roundings of these operations
-  X[i] = x * (1.0 + 0x1p-40);
-  Y[i] = y * (1.0 + 0x1p-40);
+  XY[i] = make_double2(x * (1.0 + 0x1p-40), y * (1.0 + 0x1p-40));
 }
 
 // The 3-body chain: q^2 as chain_q2 forms it, with the step of the common A
 // folded into C3 = C / A^2, and B from the table.
 template <int ENGINE, typename WordFn>
-__device__ __forceinline__ double chain3_q2(const DecayK& d, const WordFn& word, double C3,
-                                            double ks_fixed, double ks_unit, const double* X,
-                                            const double* Y, double& B) {
+__device__ __forceinline__ double chain3_q2(const DecayK& d, const WordFn& word, const ChainK& ck,
+                                            const double2* XY, double& B) {
 #pragma clang fp contract(off)
   constexpr double eps = 0x1p-53;
   const uint64_t w0 = word(0);
   const uint32_t v = is_squares<ENGINE>() ? (uint32_t)w0 : (uint32_t)(w0 >> 32);
-  double sum = 0.0;
-  sum += d.m[0];
-  sum += d.m[1];
-  const double s01 = sum;
-  sum += d.m[2];
-  const double A = 1.0 * d.ecm + sum;
+  const double s01 = (0.0 + d.m[0]) + d.m[1];
+  const double A = ck.A3;
   const double a0 = to_unit<ENGINE>(w0) * d.ecm + s01;
   const double m0 = d.m[0], m1 = d.m[1], m2 = d.m[2];
   const double spc0 = m1 + m0, smc0 = m1 - m0;
   const double x0 = ((a0 - spc0) * (a0 + spc0)) * ((a0 - smc0) * (a0 + smc0));
   const double spc1 = m2 + a0, smc1 = m2 - a0;
   const double x1 = ((A - spc1) * (A + spc1)) * ((A - smc1) * (A + smc1));
-  double ks = ks_fixed;
+  double ks = ck.ks_fixed;
   if constexpr (!is_squares<ENGINE>()) {
     double s2 = 1.0;
 #pragma unroll
@@ -276,11 +272,11 @@ __device__ __forceinline__ double chain3_q2(const DecayK& d, const WordFn& word,
       s2 = __builtin_fmin(s2, (1.0 - cz) * (1.0 + cz));
     }
     const double inv_s = __builtin_amdgcn_rsq(__builtin_fmax(s2 - 4.0 * eps, 0x1p-120));
-    ks = ks_unit * (2.0 * 3 * (64.0 + 4.0 * inv_s) + 4.0);
+    ks = ck.ks_unit * (2.0 * 3 * (64.0 + 4.0 * inv_s) + 4.0);
   }
-  const int b = bin3(v);
-  B = __builtin_fma(ks, X[b], Y[b]);
-  return C3 * (x0 * x1) * rcp_fast(a0 * a0);
+  const double2 xy = XY[bin3(v)];
+  B = __builtin_fma(ks, xy.x, xy.y);
+  return ck.C3 * (x0 * x1) * rcp_fast(a0 * a0);
 }
 
 }  // namespace
@@ -294,8 +290,8 @@ __device__ __forceinline__ double chain3_q2(const DecayK& d, const WordFn& word,
 // them.
 template <int N, int ENGINE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
-void k_uw_flags_chain(const UnweightArgs a, uint64_t* dl_count, uint64_t* dl, double scale2,
-                      double kc, const double* tab3) {
+void k_uw_flags_chain(const UnweightArgs a, uint64_t* dl_count, uint64_t* dl, const ChainK ck,
+                      const double* tab3) {
 #pragma clang fp contract(off)
   __shared__ uint32_t cnt[kBlock / kWave];
   __shared__ uint64_t dm[kUwTile / kWave];  // doubt mask of each group of the tile
@@ -304,25 +300,8 @@ void k_uw_flags_chain(const UnweightArgs a, uint64_t* dl_count, uint64_t* dl, do
   const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
   const uint64_t tile = xcd_tile(ntiles);
   if (tile >= ntiles) return;  // whole block
-  const double rmax = 1.0 / a.maxv;
-  const bool rmax_ok = __builtin_fabs(rmax) >= 0x1p-1000 && __builtin_fabs(rmax) <= 0x1p+1000;
-  // (2K + 4) eps S^2 (1/a^2 + 1/c^2) = ks (1/a^2 + 1/c^2); kc = 2K + 4 for
-  // the squares engines
-  const double ks_unit = 0x1p-53 * scale2;
-  const double ks_fixed = kc * ks_unit;
-  const double ic2_0 = 1.0 / (a.d.m[0] * a.d.m[0]);
-  const double rmax2 = rmax * rmax;
-  const double C = a.d.maxw * a.d.maxw * rmax2 * __builtin_ldexp(1.0, -2 * (N - 1));
-  // the squared test needs q >= 0: max and maxw positive
-  const bool C_ok = rmax_ok && a.maxv > 0.0 && a.d.maxw > 0.0 && C >= 0x1p-900 && C <= 0x1p+900;
-  // 3 bodies: the table (bins of X, then of Y) and C / A^2
-  [[maybe_unused]] const double* X3 = tab3;
-  [[maybe_unused]] const double* Y3 = tab3 + kBin3;
-  [[maybe_unused]] double C3 = 0.0;
-  if constexpr (N == 3) {
-    const double A = 1.0 * a.d.ecm + ((0.0 + a.d.m[0]) + a.d.m[1] + a.d.m[2]);
-    C3 = C / (A * A);
-  }
+  // 3 bodies: the table, bins of (X, Y) pairs
+  [[maybe_unused]] const double2* XY3 = reinterpret_cast<const double2*>(tab3);
   const uint64_t off = (uint64_t)wave * kWave + (uint64_t)lane;
   // key * (jump + k) of item 0 for the accept word and key * (gen_first + k +
   // 3N) for the generation word; item j + 1 is kBlock events on
@@ -343,23 +322,23 @@ void k_uw_flags_chain(const UnweightArgs a, uint64_t* dl_count, uint64_t* dl, do
     if constexpr (N == 3 && is_squares<ENGINE>()) {
       // the one uniform is the lane's own word: counter gen_first + k + 3N
       const OneWord w{squares_word_y<ENGINE>(a.d.key[0], ygen)};
-      q2 = chain3_q2<ENGINE>(a.d, w, C3, ks_fixed, ks_unit, X3, Y3, B);
+      q2 = chain3_q2<ENGINE>(a.d, w, ck, XY3, B);
       sure = B < 0x1p-10;  // false for NaN
     } else if constexpr (N == 3) {
       EventWords<N, ENGINE> words;
       words.init(a.d, a.gen_first + k0, lane);
-      q2 = chain3_q2<ENGINE>(a.d, words, C3, ks_fixed, ks_unit, X3, Y3, B);
+      q2 = chain3_q2<ENGINE>(a.d, words, ck, XY3, B);
       sure = B < 0x1p-10;
     } else {
       EventWords<N, ENGINE> words;
       words.init(a.d, a.gen_first + k0, lane);
-      q2 = chain_q2<N, ENGINE>(a.d, words, C, ks_fixed, ks_unit, ic2_0, B, sure);
+      q2 = chain_q2<N, ENGINE>(a.d, words, ck.C, ck.ks_fixed, ck.ks_unit, ck.ic2_0, B, sure);
     }
     const bool live = k < a.n;
     const double u = (double)squares3_y(a.uw_key, ykey) * 0x1p-32;  // ctr = jump + k
     const double u2 = u * u;
     const double e = B + 0x1p-47;
-    sure = sure && C_ok && q2 >= 0x1p-1000 && q2 <= 0x1p+1000 &&
+    sure = sure && ck.fast_ok && q2 >= 0x1p-1000 && q2 <= 0x1p+1000 &&
            __builtin_fabs(q2 - u2) > q2 * (e * (2.0 + e));
     const uint64_t ball = __ballot(live && q2 > u2);
     const uint64_t doubt = __ballot(live && !sure);
@@ -450,15 +429,15 @@ constexpr uint64_t kPatchBlocks = 2048;
 
 template <int N, int E>
 static void chain_one(const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl, double* tab3,
-                      double scale2, double kc, hipStream_t s) {
+                      const ChainK& ck, hipStream_t s) {
   const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
   const unsigned grid = (unsigned)((ntiles + kXcds - 1) / kXcds * kXcds);
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, dl_count);
   if constexpr (N == 3)
     hipLaunchKernelGGL(k_chain_table3, dim3((kBin3 + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                       a.d, tab3, tab3 + kBin3);
+                       a.d, reinterpret_cast<double2*>(tab3));
   hipLaunchKernelGGL((k_uw_flags_chain<N, E>), dim3(grid), dim3(kBlock), 0, s, a, dl_count, dl,
-                     scale2, kc, (const double*)tab3);
+                     ck, (const double*)tab3);
   const unsigned pgrid = (unsigned)(ntiles < kPatchBlocks ? ntiles : kPatchBlocks);
   hipLaunchKernelGGL((k_uw_patch_chain<N, E>), dim3(pgrid), dim3(kBlock), 0, s, a,
                      (const uint64_t*)dl_count, (const uint64_t*)dl);
@@ -469,7 +448,7 @@ static constexpr auto chain_at = &chain_one<2 + I / kNumE, I % kNumE>;
 
 template <int... I>
 static constexpr auto chain_table(std::integer_sequence<int, I...>) {
-  return std::array<void (*)(const UnweightArgs&, uint64_t*, uint64_t*, double*, double, double,
+  return std::array<void (*)(const UnweightArgs&, uint64_t*, uint64_t*, double*, const ChainK&,
                               hipStream_t),
                     sizeof...(I)>{chain_at<I>...};
 }
@@ -477,8 +456,8 @@ static constexpr auto chain_table(std::integer_sequence<int, I...>) {
 static const auto kChain = chain_table(std::make_integer_sequence<int, kNumN * kNumE>{});
 
 int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl,
-                       double* tab3, double scale2, double kc, hipStream_t s) {
-  kChain[(N - 2) * kNumE + engine](a, dl_count, dl, tab3, scale2, kc, s);
+                       double* tab3, const ChainK& ck, hipStream_t s) {
+  kChain[(N - 2) * kNumE + engine](a, dl_count, dl, tab3, ck, s);
   return (int)hipGetLastError();
 }
 

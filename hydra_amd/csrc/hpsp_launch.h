@@ -206,16 +206,24 @@ int launch_event_weight(int N, const WeightArgs& a, hipStream_t s);
 int launch_uw_scans(const UnweightArgs& a, hipStream_t s);
 // Flags of Decays::Unweight for events a.gen_first.. of a Generate with decay
 // a.d, decided from each event's generation weight where it is far enough
-// from the boundary (generate_unweight.hip); scale2 bounds E^2 + p^2 of any
-// subsystem, kc the momenta's error constant 2K + 4 of the squares engines
-// (the Random123 ones take theirs per event); dl_count and dl: a counter and
-// a list of two words per group of 64 events (the groups with events left in
+// from the boundary (generate_unweight.hip); dl_count and dl: a counter and a
+// list of two words per group of 64 events (the groups with events left in
 // doubt, decided by a second kernel); tab3: 2 kBin3 doubles, the 3-body
-// bound's table (written and read by the call when N = 3).
+// bound's table (written and read by the call when N = 3); ck: the call's
+// constants, formed on the host.
 constexpr int kBin3Side = 8 + 28 * 8;  // bins per end of [0, 1]
 constexpr int kBin3 = 2 * kBin3Side;
+struct ChainK {
+  double C;         // (maxw / max)^2 / 4^(N-1)
+  double C3;        // C / A^2, A = invMas[2] (N = 3)
+  double A3;        // that A
+  double ks_unit;   // eps S^2, S^2 bounding E^2 + p^2 of any subsystem
+  double ks_fixed;  // (2K + 4) eps S^2 with the squares engines' K = 2^17 N
+  double ic2_0;     // 1 / m0^2
+  int fast_ok;      // max and maxw positive, 1/max and C in range
+};
 int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl,
-                       double* tab3, double scale2, double kc, hipStream_t s);
+                       double* tab3, const ChainK& ck, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s);

@@ -155,7 +155,7 @@ def test_chain_decides_almost_every_flag(engine, limit):
                                       (10.0, [0.1, 0.2, 8.0])])
 def test_table3_on_device_equals_restatement(oracle, M, masses):
     """The 3-body bound's table the call writes into its workspace (the last
-    2 x 464 doubles) equals the restatement that tests/test_chain_bound.py
+    464 (X, Y) pairs) equals the restatement that tests/test_chain_bound.py
     checks against every event's bound."""
     from test_chain_bound import BIN3, table3
     dec = make_decay(M, masses)
@@ -168,9 +168,9 @@ def test_table3_on_device_equals_restatement(oracle, M, masses):
           "hpsp_generate_unweight")
     torch.cuda.synchronize()
     tab_bytes = 2 * BIN3 * 8
-    tab = ws[ws_bytes - tab_bytes:].cpu().numpy().view(np.float64)
+    tab = ws[ws_bytes - tab_bytes:].cpu().numpy().view(np.float64).reshape(BIN3, 2)
     X, Y = table3(oracle, M, masses)
-    for got, want in ((tab[:BIN3], X), (tab[BIN3:], Y)):
+    for got, want in ((tab[:, 0], X), (tab[:, 1], Y)):
         fin = np.isfinite(want)
         assert np.array_equal(np.isfinite(got), fin)
         np.testing.assert_allclose(got[fin], want[fin], rtol=1e-13, atol=0)
