@@ -953,7 +953,8 @@ def main():
             "plain_store": {"kernel": "torch fill of the 12 output columns (12.9 GB, no reads)",
                             "ms": c2["plain_store_ms"], "achieved": c2["plain_store_GBs"],
                             "regen_over_plain_store": c2["regen_scatter_GBs"] / c2["plain_store_GBs"]},
-            "flags_pass": {"kernel": "k_uw_flags_chain<3, 0> + scans",
+            "flags_pass": {"kernel": "k_chain_table3 + k_uw_flags_chain<3, 0> + k_uw_patch_chain"
+                                     " + scans",
                            "bound": "valu (engine words, weight chain)",
                            "ms": c2["chain_flags_scan_ms"],
                            "events_per_s": args.n / (c2["chain_flags_scan_ms"] * 1e-3)},
