@@ -320,7 +320,9 @@ def run_c2_cpp(n, steps, warmup):
 
 def run_c2_mixed(torch, capi, n=1 << 27):
     """The headline 3-body decay, FP64 against mixed (FP32 columns): Generate
-    with the PhaseSpaceWeight column, and the momenta alone."""
+    with the PhaseSpaceWeight column, the momenta alone, and the headline step
+    (Generate + Unweight() in one call; the mixed form takes its flags from
+    the mixed container's weight column, the FP64 generation weight)."""
     N = 3
     res = {"events": n}
     for tag, with_w in (("generate", True), ("momenta_only", False)):
@@ -345,6 +347,34 @@ def run_c2_mixed(torch, capi, n=1 << 27):
             del cols, evw
             torch.cuda.empty_cache()
         res[f"{tag}_mixed_speedup"] = res[f"{tag}_fp64_ms"] / res[f"{tag}_mixed_ms"]
+    # the headline step (Generate + Unweight() in one call) in both forms: FP64
+    # columns (hpsp_generate_unweight) and FP32 ones (hpsp_generate_unweight_mixed)
+    lib = capi.lib()
+    dec = capi.make_decay(M3, MASSES3)
+    sh = capi.stream_handle()
+    nacc = torch.zeros(1, dtype=torch.int64, device="cuda:0")
+    for name, f32 in (("fp64", False), ("mixed", True)):
+        cols = torch.empty((4 * N, n), dtype=torch.float32 if f32 else torch.float64,
+                           device="cuda:0")
+        wsb = (lib.hpsp_unweight_workspace_size(n) if f32
+               else lib.hpsp_generate_unweight_workspace_size(n))
+        ws = torch.empty(wsb, dtype=torch.uint8, device="cuda:0")
+        fn = lib.hpsp_generate_unweight_mixed if f32 else lib.hpsp_generate_unweight
+        t = Timer(torch)
+        for i in range(6):
+            if i:
+                t.start()
+            capi.check(fn(ctypes.byref(dec), SEED, 0, n, UW_SEED, capi.ptr(cols), n, None, None,
+                          capi.ptr(ws), wsb, capi.ptr(nacc), sh), "generate_unweight")
+            if i:
+                t.stop()
+        res[f"step_{name}_ms"] = t.avg_ms
+        res[f"step_{name}_events_per_s"] = n / (t.avg_ms * 1e-3)
+        res[f"step_{name}_n_acc"] = int(nacc.item())
+        res[f"step_{name}_GBs"] = n * (16 if f32 else 32) * N / (t.avg_ms * 1e6)
+        del cols, ws
+        torch.cuda.empty_cache()
+    res["step_mixed_speedup"] = res["step_fp64_ms"] / res["step_mixed_ms"]
     res["errors"] = mixed_errors(torch, capi, M3, MASSES3, 1 << 22)
     torch.cuda.empty_cache()
     return res

@@ -157,6 +157,8 @@ _SIGS = {
     "hpsp_generate_unweight_workspace_size": (_SZ, [_U64]),
     "hpsp_generate_unweight": (_I, [ctypes.POINTER(Decay), _U64, _U64, _U64, _U64, _P, _U64, _P,
                                     _P, _P, _SZ, _P, _P]),
+    "hpsp_generate_unweight_mixed": (_I, [ctypes.POINTER(Decay), _U64, _U64, _U64, _U64, _P, _U64,
+                                          _P, _P, _P, _SZ, _P, _P]),
     "hpsp_unweight_sample_workspace_size": (_SZ, [_U64]),
     "hpsp_unweight_sample": (_I, [_I, _P, _U64, _U64, _P, _D, _U64, _U64, _I, _P, _U64, _P, _P,
                                   _P, _SZ, _P, _P]),

@@ -728,6 +728,49 @@ int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t firs
   return hip_status(launch_unweight_regen(d->n, d->engine, a, s));
 }
 
+int hpsp_generate_unweight_mixed(const hpsp_decay* d, uint64_t gen_seed, uint64_t first,
+                                 uint64_t n, uint64_t seed, float* cols_out, uint64_t ld_out,
+                                 uint64_t* idx_out, double* w_out, void* ws, size_t ws_bytes,
+                                 uint64_t* n_acc_dev, void* stream) {
+  int rc = check_decay(d);
+  if (rc) return rc;
+  if (!n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (n > 0 && cols_out && ld_out < n) return HPSP_ERR_INVALID_ARGUMENT;
+  if (first > UINT64_MAX - n) return HPSP_ERR_INVALID_ARGUMENT;
+  const UwLayout L = uw_layout(n);
+  if (!ws || ws_bytes < L.total) return HPSP_ERR_WORKSPACE;
+  char* base = static_cast<char*>(ws);
+  hipStream_t s = as_stream(stream);
+  UnweightArgs a;
+  memset(&a, 0, sizeof(a));
+  a.d = make_decayk(d, gen_seed);
+  a.f.kind = HPSP_F_ONE;
+  a.maxv = d->max_weight;
+  uint64_t s0 = seed;
+  a.uw_key = splitmix64_host(&s0);
+  a.n = n;
+  a.ld_in = n;
+  // FP32 columns, carried in the double pointer the kernels' arguments hold
+  a.cols_out = reinterpret_cast<double*>(cols_out);
+  a.ld_out = ld_out;
+  a.idx_out = idx_out;
+  a.w_out = w_out;
+  a.flags = reinterpret_cast<uint64_t*>(base + L.flags);
+  a.tile_count = reinterpret_cast<uint32_t*>(base + L.tile_count);
+  a.tile_off = reinterpret_cast<uint64_t*>(base + L.tile_off);
+  a.run_sum = reinterpret_cast<uint64_t*>(base + L.run_sum);
+  a.n_acc = n_acc_dev;
+  a.regen = 1;
+  a.gen_first = first;
+  if (n == 0) return hip_status(launch_unweight(d->n, a, s));  // the count: 0
+  rc = hip_status(launch_flags_mixed(d->n, d->engine, a, s));
+  if (rc) return rc;
+  rc = hip_status(launch_uw_scans(a, s));
+  if (rc) return rc;
+  if (!cols_out && !idx_out && !w_out) return HPSP_OK;
+  return hip_status(launch_unweight_regen_mixed(d->n, d->engine, a, s));
+}
+
 size_t hpsp_unweight_sample_workspace_size(uint64_t n) {
   return uw_layout(n).total + align_up((n ? n : 1) * sizeof(uint64_t));
 }

@@ -12,6 +12,7 @@
 #include <utility>
 
 #include "hpsp_launch.h"
+#include "uw_dst.h"
 
 namespace hpsp {
 
@@ -82,15 +83,19 @@ __device__ __forceinline__ double weight_chain(const DecayK& d, const WordFn& wo
   return wt;
 }
 
+// The FP32 kinematics with their fused multiply-adds written out and
+// contraction off, so that every kernel that inlines it (Generate, AverageOn,
+// the regenerating scatter) forms the same bits whatever the code around it.
 template <int N, int ENGINE, typename WordFn>
 __device__ __forceinline__ double genbod_mixed(const DecayK& d, const WordFn& word,
                                                float (&p)[N][4]) {
+#pragma clang fp contract(off)
   float pd[N - 1];
   double invMas[N];
   const double wt = weight_chain<N, ENGINE>(d, word, invMas, pd);
 
   const float m0 = (float)d.m[0];
-  p[0][0] = __builtin_amdgcn_sqrtf(pd[0] * pd[0] + m0 * m0);
+  p[0][0] = __builtin_amdgcn_sqrtf(__builtin_fmaf(pd[0], pd[0], m0 * m0));
   p[0][1] = 0.0f;
   p[0][2] = pd[0];
   p[0][3] = 0.0f;
@@ -98,7 +103,7 @@ __device__ __forceinline__ double genbod_mixed(const DecayK& d, const WordFn& wo
 #pragma unroll
   for (int i = 1; i < N; ++i) {
     const float mi = (float)d.m[i];
-    p[i][0] = __builtin_amdgcn_sqrtf(pd[i - 1] * pd[i - 1] + mi * mi);
+    p[i][0] = __builtin_amdgcn_sqrtf(__builtin_fmaf(pd[i - 1], pd[i - 1], mi * mi));
     p[i][1] = 0.0f;
     p[i][2] = -pd[i - 1];
     p[i][3] = 0.0f;
@@ -122,12 +127,12 @@ __device__ __forceinline__ double genbod_mixed(const DecayK& d, const WordFn& wo
       } else {
         float x = p[j][1];
         const float y = p[j][2];
-        p[j][1] = cZ * x - sZ * y;
-        p[j][2] = sZ * x + cZ * y;
+        p[j][1] = __builtin_fmaf(cZ, x, -(sZ * y));
+        p[j][2] = __builtin_fmaf(sZ, x, cZ * y);
         x = p[j][1];
         const float z = p[j][3];
-        p[j][1] = cY * x - sY * z;
-        p[j][3] = sY * x + cY * z;
+        p[j][1] = __builtin_fmaf(cY, x, -(sY * z));
+        p[j][3] = __builtin_fmaf(sY, x, cY * z);
       }
     }
     if (i == N - 1) break;
@@ -136,7 +141,7 @@ __device__ __forceinline__ double genbod_mixed(const DecayK& d, const WordFn& wo
     // (gamma-1)/beta^2 written as gamma^2/(gamma+1) (no cancellation).
     const float M = (float)invMas[i];
     const float pi2 = pd[i] * pd[i];
-    const float beta = pd[i] * __builtin_amdgcn_rsqf(pi2 + M * M);
+    const float beta = pd[i] * __builtin_amdgcn_rsqf(__builtin_fmaf(M, M, pi2));
     const float b2 = beta * beta;
     const float gamma = __builtin_amdgcn_rsqf(1.0f - b2);
     const float gb2yy = gamma * gamma * __builtin_amdgcn_rcpf(gamma + 1.0f) * b2;
@@ -144,25 +149,26 @@ __device__ __forceinline__ double genbod_mixed(const DecayK& d, const WordFn& wo
 #pragma unroll
     for (int j = 0; j <= i; ++j) {
       const float e2 = p[j][0], py2 = p[j][2];
-      p[j][0] = gamma * e2 + gby * py2;
-      p[j][2] = gby * e2 + gb2yy * py2 + py2;
+      p[j][0] = __builtin_fmaf(gamma, e2, gby * py2);
+      p[j][2] = __builtin_fmaf(gby, e2, __builtin_fmaf(gb2yy, py2, py2));
     }
   }
   return wt;
 }
 
 __device__ __forceinline__ void boost_general_f32(float (&v)[4], float bx, float by, float bz) {
-  const float b2 = bx * bx + by * by + bz * bz;
+#pragma clang fp contract(off)
+  const float b2 = __builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx));
   if (b2 > 0.0f && b2 < 1.0f) {
     const float gamma = __builtin_amdgcn_rsqf(1.0f - b2);
     const float gb2 = gamma * gamma * __builtin_amdgcn_rcpf(gamma + 1.0f);
     const float e2 = v[0], px2 = v[1], py2 = v[2], pz2 = v[3];
-    const float bp = bx * px2 + by * py2 + bz * pz2;
+    const float bp = __builtin_fmaf(bz, pz2, __builtin_fmaf(by, py2, bx * px2));
     v[0] = gamma * (e2 + bp);
-    const float k = gb2 * bp + gamma * e2;
-    v[1] = px2 + k * bx;
-    v[2] = py2 + k * by;
-    v[3] = pz2 + k * bz;
+    const float k = __builtin_fmaf(gb2, bp, gamma * e2);
+    v[1] = __builtin_fmaf(k, bx, px2);
+    v[2] = __builtin_fmaf(k, by, py2);
+    v[3] = __builtin_fmaf(k, bz, pz2);
   }
 }
 
@@ -362,8 +368,167 @@ __global__ __launch_bounds__(kBlock) void k_average_mixed(const AvgArgs a) {
   }
 }
 
+// ---- Generate followed by Unweight in one call, mixed precision ----------
+// The container of a mixed Generate holds FP32 momenta and, as its weight
+// column, the FP64 generation weight (k_generate_mixed).  Decays::Unweight's
+// test on that column, RN(w / max) > u (FlagDaugthers, Decays.inl:75-89),
+// therefore needs only each event's FP64 weight chain, which
+// k_uw_flags_mixed forms with the same code as the mixed Generate (the same
+// bits: no bound and no exact second test are needed); k_uw_regen_mixed then
+// generates every event again, with the same code and so the same bits, at
+// its place in the stable partition.  Both kernels live in this translation
+// unit so that the FP32 kinematics compile as the mixed Generate's do.
+
+// Flags of one 1024-event tile per block, items as k_uw_flags: thread item j
+// is event tile * 1024 + j * 256 + wave * 64 + lane.
+template <int N, int ENGINE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
+void k_uw_flags_mixed(const UnweightArgs a) {
+  __shared__ uint32_t cnt[kBlock / kWave];
+  const int lane = __lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
+  const uint64_t tile = xcd_tile(ntiles);
+  if (tile >= ntiles) return;  // whole block
+  const double rmax = 1.0 / a.maxv;
+  const bool rmax_ok = __builtin_fabs(rmax) >= 0x1p-1000 && __builtin_fabs(rmax) <= 0x1p+1000;
+  const uint64_t off = (uint64_t)wave * kWave + (uint64_t)lane;
+  uint64_t ykey = a.uw_key * (a.jump + tile * kUwTile + off);
+  const uint64_t ystep = a.uw_key * (uint64_t)kBlock;
+  uint32_t c = 0;
+  // unrolled where the body is small enough for it
+  constexpr int kUnroll = N <= 6 || is_squares<ENGINE>() ? kItems : 1;
+#pragma unroll kUnroll
+  for (int j = 0; j < kItems; ++j, ykey += ystep) {
+    const uint64_t g = tile * (kUwTile / kWave) + (uint64_t)j * (kBlock / kWave) + wave;
+    const uint64_t k0 = g * kWave;
+    const uint64_t k = k0 + (uint64_t)lane;
+    double invMas[N];
+    float pd[N - 1];
+    EventWordsM<N, ENGINE> words;
+    words.init(a.d, a.gen_first + k0, lane);
+    const double wt = weight_chain<N, ENGINE>(a.d, words, invMas, pd);
+    const double u = (double)squares3_y(a.uw_key, ykey) * 0x1p-32;  // ctr = jump + k
+    const bool acc = k < a.n && ratio_above(wt, a.maxv, rmax, rmax_ok, u);
+    const uint64_t ball = __ballot(acc);
+    if (lane == 0) *(glob(a.flags) + g) = ball;
+    c += (uint32_t)__popcll(ball);
+  }
+  if (lane == 0) cnt[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) a.tile_count[tile] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+}
+
+// Squares words of one event from the block's words in LDS (as k_uw_regen):
+// thread t's own word (counter c0 + t) and words j >= 1 at w[j], w = sw + t.
+struct BlockWordsM {
+  const uint32_t* w;
+  uint32_t own;
+  __device__ __forceinline__ uint64_t operator()(int j) const { return j == 0 ? own : w[j]; }
+};
+
+// The regenerating scatter of k_uw_regen with the mixed GENBOD: one group of
+// 64 events per wave, the block's 256 events staged in LDS in partition
+// order (all 4N FP32 columns at once: 12 KB at N = 3) and each column written
+// as the block's two runs; cols_out holds FP32 columns (UnweightArgs carries
+// it as a double pointer).  w_out receives each output row's weight column
+// value, the FP64 generation weight.
+template <int N, int ENGINE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
+void k_uw_regen_mixed(const UnweightArgs a) {
+  constexpr int C = 4 * N;
+  constexpr int CS = C <= 24 ? C : C / 2;  // columns staged at once
+  __shared__ float sh[CS][kBlock];
+  constexpr int W = 3 * N - 4;
+  __shared__ uint32_t sw[is_squares<ENGINE>() ? kBlock + W - 1 : 1];
+  constexpr int kWavesPerBlock = kBlock / kWave;
+  const uint64_t ngroups = (a.n + kWave - 1) / kWave;
+  const uint64_t nblocks = (ngroups + kWavesPerBlock - 1) / kWavesPerBlock;
+  const uint64_t per_xcd = (nblocks + kXcds - 1) / kXcds;
+  const uint64_t blk = (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (blk >= nblocks) return;  // whole block
+  const int lane = __lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t g0 = blk * kWavesPerBlock;
+  const BlockPart bp = block_part(a, g0, wave, ngroups);
+  const uint64_t goff = bp.goff;
+  const uint32_t acc_wave = bp.acc_wave, a_blk = bp.a_blk;
+  const uint64_t nacc = *a.n_acc;
+  const uint64_t kb = g0 * kWave;
+  const uint64_t g = g0 + (uint64_t)wave;
+  const uint64_t k0 = g * kWave;
+  const uint64_t k = k0 + (uint64_t)lane;
+  const bool live_wave = g < ngroups;  // wave-uniform
+  uint32_t own = 0;
+  if constexpr (is_squares<ENGINE>()) {
+    const uint64_t c0 = a.gen_first + kb + 3 * (uint64_t)N;
+    const uint32_t t = threadIdx.x;
+    own = squares_word_y<ENGINE>(a.d.key[0], a.d.key[0] * (c0 + t));
+    sw[t] = own;
+    if (t < (uint32_t)(W - 1))
+      sw[kBlock + t] = squares_word_y<ENGINE>(a.d.key[0], a.d.key[0] * (c0 + kBlock + t));
+    __syncthreads();
+  }
+  float p[N][4];
+  double wt = 0.0;
+  uint32_t r = 0;  // the event's rank in the block's partition order
+  if (live_wave) {
+    const uint64_t fw = bp.fw;
+    if constexpr (is_squares<ENGINE>()) {
+      wt = genbod_mixed<N, ENGINE>(a.d, BlockWordsM{sw + threadIdx.x, own}, p);
+    } else {
+      EventWordsM<N, ENGINE> words;
+      words.init(a.d, a.gen_first + k0, lane);
+      wt = genbod_mixed<N, ENGINE>(a.d, words, p);
+    }
+    final_boost_f32<N>(p, a.d);
+    const uint64_t below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    const uint32_t acc_below = acc_wave + (uint32_t)__popcll(fw & below);
+    const bool kept = (fw >> lane) & 1ULL;
+    const uint32_t t = (uint32_t)(wave * kWave + lane);
+    r = kept ? acc_below : a_blk + (t - acc_below);
+    if (k < a.n) {
+      const uint64_t dst = (r < a_blk) ? goff + r : nacc + (kb - goff) + (r - a_blk);
+      if (a.w_out) *(glob(a.w_out) + dst) = wt;
+      if (a.idx_out) *(glob(a.idx_out) + dst) = k;
+    }
+  }
+  const uint64_t live = (a.n - kb) < (uint64_t)kBlock ? (a.n - kb) : (uint64_t)kBlock;
+  const uint32_t t = threadIdx.x;
+  const uint64_t dst = (t < a_blk) ? goff + t : nacc + (kb - goff) + (t - a_blk);
+  float* cols = reinterpret_cast<float*>(a.cols_out);
+#pragma unroll
+  for (int h = 0; h < C; h += CS) {
+    if (h) __syncthreads();  // the previous columns have been read
+    if (live_wave && k < a.n) {
+#pragma unroll
+      for (int c = h; c < h + CS; ++c) sh[c - h][r] = p[c / 4][c % 4];
+    }
+    __syncthreads();
+    if (cols && t < live) {
+      // one 64-bit address per thread, stepped by the column stride
+      gptr<float> q = glob(cols + dst) + (uint64_t)h * a.ld_out;
+#pragma unroll
+      for (int c = h; c < h + CS; ++c, q += a.ld_out) *q = sh[c - h][t];
+    }
+  }
+}
+
 constexpr int kNumN = HPSP_MAX_N - 1;
 constexpr int kNumE = 6;
+
+template <int N, int E>
+static void gu_mixed_one(const UnweightArgs& a, hipStream_t s, int part) {
+  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
+  if (part == 0) {
+    const unsigned grid = (unsigned)((ntiles + kXcds - 1) / kXcds * kXcds);
+    hipLaunchKernelGGL((k_uw_flags_mixed<N, E>), dim3(grid), dim3(kBlock), 0, s, a);
+  } else {
+    const uint64_t nblocks = (a.n + kBlock - 1) / kBlock;
+    const uint64_t grid = (nblocks + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL((k_uw_regen_mixed<N, E>), dim3((unsigned)grid), dim3(kBlock), 0, s, a);
+  }
+}
 
 template <int N, int E>
 static void gen_one(const GenArgs& in, hipStream_t s) {
@@ -405,8 +570,27 @@ static constexpr auto avg_table(std::integer_sequence<int, I...>) {
   return std::array<void (*)(const AvgArgs&, hipStream_t), sizeof...(I)>{avg_at<I>...};
 }
 
+template <int I>
+static constexpr auto gum_at = &gu_mixed_one<2 + I / kNumE, I % kNumE>;
+template <int... I>
+static constexpr auto gum_table(std::integer_sequence<int, I...>) {
+  return std::array<void (*)(const UnweightArgs&, hipStream_t, int), sizeof...(I)>{gum_at<I>...};
+}
+
 static const auto kGen = gen_table(std::make_integer_sequence<int, kNumN * kNumE>{});
 static const auto kAvg = avg_table(std::make_integer_sequence<int, kNumN * kNumE>{});
+static const auto kGuMixed = gum_table(std::make_integer_sequence<int, kNumN * kNumE>{});
+
+int launch_flags_mixed(int N, int engine, const UnweightArgs& a, hipStream_t s) {
+  kGuMixed[(N - 2) * kNumE + engine](a, s, 0);
+  return (int)hipGetLastError();
+}
+
+int launch_unweight_regen_mixed(int N, int engine, const UnweightArgs& a, hipStream_t s) {
+  if (a.n == 0) return (int)hipSuccess;
+  kGuMixed[(N - 2) * kNumE + engine](a, s, 1);
+  return (int)hipGetLastError();
+}
 
 int launch_generate_mixed(int N, int engine, const GenArgs& a, hipStream_t s) {
   kGen[(N - 2) * kNumE + engine](a, s);

@@ -224,6 +224,11 @@ struct ChainK {
 };
 int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl,
                        double* tab3, const ChainK& ck, hipStream_t s);
+// The mixed form (genbod_mixed.hip): flags from each event's FP64 weight
+// chain, the mixed Generate's weight column; then every event generated again
+// at its place, FP32 columns in a.cols_out (carried as a double pointer).
+int launch_flags_mixed(int N, int engine, const UnweightArgs& a, hipStream_t s);
+int launch_unweight_regen_mixed(int N, int engine, const UnweightArgs& a, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);
 int launch_unweight(int N, const UnweightArgs& a, hipStream_t s);

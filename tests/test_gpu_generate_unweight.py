@@ -14,8 +14,9 @@ import numpy as np
 import pytest
 import torch
 
-from gpu_util import DEV, check, generate, lib, make_decay, ptr, stream_handle, \
+from gpu_util import DEV, check, generate, lib, make_decay, ptr, stream_handle, unweight, \
     unweight_regenerate
+from hydra_amd import capi
 from oracle import ENGINES
 
 pytestmark = pytest.mark.gpu
@@ -174,3 +175,75 @@ def test_table3_on_device_equals_restatement(oracle, M, masses):
         fin = np.isfinite(want)
         assert np.array_equal(np.isfinite(got), fin)
         np.testing.assert_allclose(got[fin], want[fin], rtol=1e-13, atol=0)
+
+
+# ---- the mixed-precision form ------------------------------------------------
+
+def generate_unweight_mixed(dec, gen_seed, first, n, seed, ld=None):
+    ld = ld or max(n, 1)
+    ws_bytes = lib().hpsp_unweight_workspace_size(n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
+    out = torch.empty((4 * dec.n, ld), dtype=torch.float32, device=DEV)
+    idx = torch.empty(max(n, 1), dtype=torch.int64, device=DEV)
+    w = torch.empty(max(n, 1), dtype=torch.float64, device=DEV)
+    nacc = torch.full((1,), 12345, dtype=torch.int64, device=DEV)
+    check(lib().hpsp_generate_unweight_mixed(ctypes.byref(dec), gen_seed, first, n, seed,
+                                             ptr(out), ld, ptr(idx), ptr(w), ptr(ws), ws_bytes,
+                                             ptr(nacc), stream_handle()),
+          "hpsp_generate_unweight_mixed")
+    torch.cuda.synchronize()
+    return int(nacc.item()), out, idx, w
+
+
+def same_mixed(dec, gen_seed, first, n, uw_seed):
+    """The one call against the mixed Generate (FP32 columns and its weight
+    column) followed by the moving Unweight of those columns with that
+    column as the weights (Decays.inl:415-455)."""
+    cols32, _, evw = generate(dec, gen_seed, n, first=first, prec=capi.MIXED, out_f32=True,
+                              want_wt=False)
+    k1, out1, idx1, w1 = unweight(dec, cols32[:, :n].double().contiguous(), uw_seed, w_in=evw,
+                                  n=n)
+    print(f"mixed N={dec.n} engine={dec.engine} n={n} first={first}", flush=True)
+    k2, out2, idx2, w2 = generate_unweight_mixed(dec, gen_seed, first, n, uw_seed)
+    assert k1 == k2
+    assert np.array_equal(bits(out1[:, :n]), bits(out2[:, :n].double()))
+    assert np.array_equal(idx1.cpu().numpy()[:n], idx2.cpu().numpy()[:n])
+    assert np.array_equal(bits(w1[:n]), bits(w2[:n]))
+    return k1
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 7, 12])
+@pytest.mark.parametrize("engine", list(ENGINES))
+def test_generate_unweight_mixed_equals_generate_then_unweight(N, engine):
+    rng = np.random.default_rng(900 + 13 * N + ENGINES[engine])
+    m = list(rng.uniform(0.1, 1.0, N))
+    M = sum(m) * rng.uniform(1.5, 3.0) + 2.0
+    dec = make_decay(M, m, engine=engine)
+    k = same_mixed(dec, int(rng.integers(0, 2**63)), int(rng.integers(0, 2**40)), 20000 + N,
+                   int(rng.integers(0, 2**63)))
+    assert 0 <= k <= 20000 + N
+
+
+def test_generate_unweight_mixed_sizes_and_headline():
+    dec = make_decay(5.279, [3.0969, 0.493677, 0.13957])
+    for n in (1, 63, 257, 1023, 4096 + 17):
+        same_mixed(dec, 159753, 7, n, UW_SEED)
+    same_mixed(dec, 159753, 0, 1 << 22, UW_SEED)
+    near = make_decay(10.0, [0.999] * 10)
+    same_mixed(near, 2024, 3, 1 << 18, UW_SEED)
+
+
+def test_generate_unweight_mixed_empty_and_errors():
+    dec = make_decay(5.279, [3.0969, 0.493677, 0.13957])
+    k, _, _, _ = generate_unweight_mixed(dec, 1, 0, 0, UW_SEED)
+    assert k == 0
+    nacc = torch.zeros(1, dtype=torch.int64, device=DEV)
+    out = torch.empty((12, 1000), dtype=torch.float32, device=DEV)
+    ws_bytes = lib().hpsp_unweight_workspace_size(1000)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
+    assert lib().hpsp_generate_unweight_mixed(ctypes.byref(dec), 1, 0, 1000, 2, ptr(out), 1000,
+                                              None, None, ptr(ws), 16, ptr(nacc),
+                                              stream_handle()) != 0
+    assert lib().hpsp_generate_unweight_mixed(ctypes.byref(dec), 1, 0, 1000, 2, ptr(out), 999,
+                                              None, None, ptr(ws), ws_bytes, ptr(nacc),
+                                              stream_handle()) != 0
