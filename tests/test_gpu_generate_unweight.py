@@ -247,3 +247,32 @@ def test_generate_unweight_mixed_empty_and_errors():
     assert lib().hpsp_generate_unweight_mixed(ctypes.byref(dec), 1, 0, 1000, 2, ptr(out), 999,
                                               None, None, ptr(ws), ws_bytes, ptr(nacc),
                                               stream_handle()) != 0
+
+
+def test_generate_unweight_strides_and_partial_outputs():
+    """A column stride above n, and the index map alone (no columns), in both
+    forms: the same rows and indices as the full call."""
+    dec = make_decay(5.279, [3.0969, 0.493677, 0.13957], engine="squares4")
+    n = 5000
+    k1, out1, idx1, w1 = generate_unweight(dec, 77, 11, n, UW_SEED)
+    k2, out2, idx2, w2 = generate_unweight(dec, 77, 11, n, UW_SEED, ld=n + 37)
+    assert k1 == k2
+    assert np.array_equal(bits(out1[:, :n]), bits(out2[:, :n]))
+    assert np.array_equal(idx1.cpu().numpy(), idx2.cpu().numpy())
+    # the index map alone
+    ws_bytes = lib().hpsp_generate_unweight_workspace_size(n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
+    idx3 = torch.empty(n, dtype=torch.int64, device=DEV)
+    nacc = torch.zeros(1, dtype=torch.int64, device=DEV)
+    check(lib().hpsp_generate_unweight(ctypes.byref(dec), 77, 11, n, UW_SEED, None, n, ptr(idx3),
+                                       None, ptr(ws), ws_bytes, ptr(nacc), stream_handle()),
+          "hpsp_generate_unweight")
+    torch.cuda.synchronize()
+    assert int(nacc.item()) == k1
+    assert np.array_equal(idx1.cpu().numpy(), idx3.cpu().numpy())
+    # the mixed form with a stride above n
+    m1 = generate_unweight_mixed(dec, 77, 11, n, UW_SEED)
+    m2 = generate_unweight_mixed(dec, 77, 11, n, UW_SEED, ld=n + 37)
+    assert m1[0] == m2[0]
+    assert np.array_equal(m1[1][:, :n].cpu().numpy(), m2[1][:, :n].cpu().numpy())
+    assert np.array_equal(m1[2].cpu().numpy(), m2[2].cpu().numpy())
