@@ -348,32 +348,41 @@ def run_c2_mixed(torch, capi, n=1 << 27):
             torch.cuda.empty_cache()
         res[f"{tag}_mixed_speedup"] = res[f"{tag}_fp64_ms"] / res[f"{tag}_mixed_ms"]
     # the headline step (Generate + Unweight() in one call) in both forms: FP64
-    # columns (hpsp_generate_unweight) and FP32 ones (hpsp_generate_unweight_mixed)
+    # columns (hpsp_generate_unweight) and FP32 ones (hpsp_generate_unweight_mixed),
+    # timed in alternating rounds so that both see the same clocks
     lib = capi.lib()
     dec = capi.make_decay(M3, MASSES3)
     sh = capi.stream_handle()
     nacc = torch.zeros(1, dtype=torch.int64, device="cuda:0")
+    wsb = lib.hpsp_generate_unweight_workspace_size(n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda:0")
+    forms = {}
     for name, f32 in (("fp64", False), ("mixed", True)):
         cols = torch.empty((4 * N, n), dtype=torch.float32 if f32 else torch.float64,
                            device="cuda:0")
-        wsb = (lib.hpsp_unweight_workspace_size(n) if f32
-               else lib.hpsp_generate_unweight_workspace_size(n))
-        ws = torch.empty(wsb, dtype=torch.uint8, device="cuda:0")
         fn = lib.hpsp_generate_unweight_mixed if f32 else lib.hpsp_generate_unweight
-        t = Timer(torch)
-        for i in range(6):
-            if i:
+        forms[name] = (cols, fn, f32, Timer(torch))
+
+    def call(name):
+        cols, fn, _, _ = forms[name]
+        capi.check(fn(ctypes.byref(dec), SEED, 0, n, UW_SEED, capi.ptr(cols), n, None, None,
+                      capi.ptr(ws), wsb, capi.ptr(nacc), sh), "generate_unweight")
+
+    for name in forms:
+        call(name)
+    for _ in range(3):
+        for name, (_, _, _, t) in forms.items():
+            for _ in range(4):
                 t.start()
-            capi.check(fn(ctypes.byref(dec), SEED, 0, n, UW_SEED, capi.ptr(cols), n, None, None,
-                          capi.ptr(ws), wsb, capi.ptr(nacc), sh), "generate_unweight")
-            if i:
+                call(name)
                 t.stop()
+            res[f"step_{name}_n_acc"] = int(nacc.item())
+    for name, (_, _, f32, t) in forms.items():
         res[f"step_{name}_ms"] = t.avg_ms
         res[f"step_{name}_events_per_s"] = n / (t.avg_ms * 1e-3)
-        res[f"step_{name}_n_acc"] = int(nacc.item())
         res[f"step_{name}_GBs"] = n * (16 if f32 else 32) * N / (t.avg_ms * 1e6)
-        del cols, ws
-        torch.cuda.empty_cache()
+    del forms, ws
+    torch.cuda.empty_cache()
     res["step_mixed_speedup"] = res["step_fp64_ms"] / res["step_mixed_ms"]
     res["errors"] = mixed_errors(torch, capi, M3, MASSES3, 1 << 22)
     torch.cuda.empty_cache()

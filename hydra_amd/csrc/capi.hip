@@ -651,6 +651,32 @@ size_t hpsp_generate_unweight_workspace_size(uint64_t n) {
   return gu_table_offset(n) + align_up(2 * kBin3 * sizeof(double));
 }
 
+// The chain flags kernels' constants, formed once here instead of by every
+// wave (FP64 divisions); scale2 bounds E^2 + p^2 of any subsystem; K of the
+// squares engines (the Random123 ones set theirs per event).
+static ChainK make_chaink(const hpsp_decay* d, double scale2) {
+  const double K = 0x1p17 * d->n;
+  ChainK ck;
+  ck.rmax = 1.0 / d->max_weight;
+  ck.rmax_ok = fabs(ck.rmax) >= 0x1p-1000 && fabs(ck.rmax) <= 0x1p+1000;
+  ck.ks_unit = 0x1p-53 * scale2;
+  ck.ks_fixed = (2.0 * K + 4.0) * ck.ks_unit;
+  ck.ic2_0 = 1.0 / (d->masses[0] * d->masses[0]);
+  ck.C = d->max_weight * d->max_weight * (ck.rmax * ck.rmax) * ldexp(1.0, -2 * (d->n - 1));
+  // the squared test needs q >= 0: max and maxw positive
+  ck.fast_ok = ck.rmax_ok && d->max_weight > 0.0 && ck.C >= 0x1p-900 && ck.C <= 0x1p+900;
+  ck.A3 = 0.0;
+  ck.C3 = 0.0;
+  if (d->n == 3) {
+    // invMas[2] as the chain forms it: 1 * ecm + ((0 + m0) + m1 + m2)
+    double sum = 0.0;
+    for (int i = 0; i < 3; ++i) sum += d->masses[i];
+    ck.A3 = 1.0 * d->ecm + sum;
+    ck.C3 = ck.C / (ck.A3 * ck.A3);
+  }
+  return ck;
+}
+
 int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t first, uint64_t n,
                            uint64_t seed, double* cols_out, uint64_t ld_out, uint64_t* idx_out,
                            double* w_out, void* ws, size_t ws_bytes, uint64_t* n_acc_dev,
@@ -699,27 +725,7 @@ int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t firs
     const double f = a.d.gamma * (1.0 + b);
     scale2 *= f * f;
   }
-  // the flags kernel's constants (its per-wave FP64 divisions, done once
-  // here); K of the squares engines (the Random123 ones set theirs per event)
-  const double K = 0x1p17 * d->n;
-  ChainK ck;
-  const double rmax = 1.0 / d->max_weight;
-  const bool rmax_ok = fabs(rmax) >= 0x1p-1000 && fabs(rmax) <= 0x1p+1000;
-  ck.ks_unit = 0x1p-53 * scale2;
-  ck.ks_fixed = (2.0 * K + 4.0) * ck.ks_unit;
-  ck.ic2_0 = 1.0 / (d->masses[0] * d->masses[0]);
-  ck.C = d->max_weight * d->max_weight * (rmax * rmax) * ldexp(1.0, -2 * (d->n - 1));
-  // the squared test needs q >= 0: max and maxw positive
-  ck.fast_ok = rmax_ok && d->max_weight > 0.0 && ck.C >= 0x1p-900 && ck.C <= 0x1p+900;
-  ck.A3 = 0.0;
-  ck.C3 = 0.0;
-  if (d->n == 3) {
-    // invMas[2] as the chain forms it: 1 * ecm + ((0 + m0) + m1 + m2)
-    double sum = 0.0;
-    for (int i = 0; i < 3; ++i) sum += d->masses[i];
-    ck.A3 = 1.0 * d->ecm + sum;
-    ck.C3 = ck.C / (ck.A3 * ck.A3);
-  }
+  const ChainK ck = make_chaink(d, scale2);
   rc = hip_status(launch_flags_chain(d->n, d->engine, a, dl_count, dl, tab3, ck, s));
   if (rc) return rc;
   rc = hip_status(launch_uw_scans(a, s));
@@ -738,8 +744,9 @@ int hpsp_generate_unweight_mixed(const hpsp_decay* d, uint64_t gen_seed, uint64_
   if (n > 0 && cols_out && ld_out < n) return HPSP_ERR_INVALID_ARGUMENT;
   if (first > UINT64_MAX - n) return HPSP_ERR_INVALID_ARGUMENT;
   const UwLayout L = uw_layout(n);
-  if (!ws || ws_bytes < L.total) return HPSP_ERR_WORKSPACE;
+  if (!ws || ws_bytes < hpsp_generate_unweight_workspace_size(n)) return HPSP_ERR_WORKSPACE;
   char* base = static_cast<char*>(ws);
+  double* tab3 = reinterpret_cast<double*>(base + gu_table_offset(n));
   hipStream_t s = as_stream(stream);
   UnweightArgs a;
   memset(&a, 0, sizeof(a));
@@ -763,7 +770,8 @@ int hpsp_generate_unweight_mixed(const hpsp_decay* d, uint64_t gen_seed, uint64_
   a.regen = 1;
   a.gen_first = first;
   if (n == 0) return hip_status(launch_unweight(d->n, a, s));  // the count: 0
-  rc = hip_status(launch_flags_mixed(d->n, d->engine, a, s));
+  // no momenta are involved in the mixed flags' bound: no scale
+  rc = hip_status(launch_flags_mixed(d->n, d->engine, a, tab3, make_chaink(d, 0.0), s));
   if (rc) return rc;
   rc = hip_status(launch_uw_scans(a, s));
   if (rc) return rc;

@@ -11,6 +11,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "genbod_mixed.h"
 #include "hpsp_launch.h"
 #include "uw_dst.h"
 
@@ -39,53 +40,6 @@ __device__ __forceinline__ void polar_f32(uint32_t w, float& cZ, float& sZ) {
   sZ = __builtin_amdgcn_sqrtf(a * b) * 0x1p-31f;
 }
 
-// pdk in FP64 with the reference's subtraction order (so the a - b - c
-// cancellation near threshold is the FP64 one) and a one-ulp square root and
-// reciprocal instead of correctly rounded ones.
-__device__ __forceinline__ double pdk_fast(double a, double b, double c) {
-#pragma clang fp contract(off)
-  const double x = (a - b - c) * (a + b + c) * (a - b + c) * (a + b - c);
-  return sqrt_fast(x) * rcp_fast(2 * a);
-}
-
-// FP64 weight chain of DecayMother::process (DecayMother.h:170-212):
-// sorted uniforms, invMas, pdk and the weight product in the reference's
-// order; the momenta pd are handed on in FP32.
-template <int N, int ENGINE, typename WordFn>
-__device__ __forceinline__ double weight_chain(const DecayK& d, const WordFn& word,
-                                               double (&invMas)[N], float (&pd)[N - 1]) {
-#pragma clang fp contract(off)
-  double rno[N];
-  rno[0] = 0.0;
-  rno[N - 1] = 1.0;
-  if constexpr (N > 2) {
-    using WT = std::conditional_t<is_squares<ENGINE>(), uint32_t, uint64_t>;
-    WT v[N - 2];
-#pragma unroll
-    for (int k = 0; k < N - 2; ++k) v[k] = (WT)word(k);
-    sort_net(v);
-#pragma unroll
-    for (int k = 0; k < N - 2; ++k) rno[k + 1] = to_unit<ENGINE>(v[k]);
-  }
-  double sum = 0.0;
-#pragma unroll
-  for (int n = 0; n < N; ++n) {
-    sum += d.m[n];
-    invMas[n] = rno[n] * d.ecm + sum;
-  }
-  double wt = d.maxw;
-#pragma unroll
-  for (int n = 0; n < N - 1; ++n) {
-    const double q = pdk_fast(invMas[n + 1], invMas[n], d.m[n + 1]);
-    wt *= q;
-    pd[n] = (float)q;
-  }
-  return wt;
-}
-
-// The FP32 kinematics with their fused multiply-adds written out and
-// contraction off, so that every kernel that inlines it (Generate, AverageOn,
-// the regenerating scatter) forms the same bits whatever the code around it.
 template <int N, int ENGINE, typename WordFn>
 __device__ __forceinline__ double genbod_mixed(const DecayK& d, const WordFn& word,
                                                float (&p)[N][4]) {
@@ -372,52 +326,10 @@ __global__ __launch_bounds__(kBlock) void k_average_mixed(const AvgArgs a) {
 // The container of a mixed Generate holds FP32 momenta and, as its weight
 // column, the FP64 generation weight (k_generate_mixed).  Decays::Unweight's
 // test on that column, RN(w / max) > u (FlagDaugthers, Decays.inl:75-89),
-// therefore needs only each event's FP64 weight chain, which
-// k_uw_flags_mixed forms with the same code as the mixed Generate (the same
-// bits: no bound and no exact second test are needed); k_uw_regen_mixed then
-// generates every event again, with the same code and so the same bits, at
-// its place in the stable partition.  Both kernels live in this translation
-// unit so that the FP32 kinematics compile as the mixed Generate's do.
-
-// Flags of one 1024-event tile per block, items as k_uw_flags: thread item j
-// is event tile * 1024 + j * 256 + wave * 64 + lane.
-template <int N, int ENGINE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
-void k_uw_flags_mixed(const UnweightArgs a) {
-  __shared__ uint32_t cnt[kBlock / kWave];
-  const int lane = __lane_id();
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
-  const uint64_t tile = xcd_tile(ntiles);
-  if (tile >= ntiles) return;  // whole block
-  const double rmax = 1.0 / a.maxv;
-  const bool rmax_ok = __builtin_fabs(rmax) >= 0x1p-1000 && __builtin_fabs(rmax) <= 0x1p+1000;
-  const uint64_t off = (uint64_t)wave * kWave + (uint64_t)lane;
-  uint64_t ykey = a.uw_key * (a.jump + tile * kUwTile + off);
-  const uint64_t ystep = a.uw_key * (uint64_t)kBlock;
-  uint32_t c = 0;
-  // unrolled where the body is small enough for it
-  constexpr int kUnroll = N <= 6 || is_squares<ENGINE>() ? kItems : 1;
-#pragma unroll kUnroll
-  for (int j = 0; j < kItems; ++j, ykey += ystep) {
-    const uint64_t g = tile * (kUwTile / kWave) + (uint64_t)j * (kBlock / kWave) + wave;
-    const uint64_t k0 = g * kWave;
-    const uint64_t k = k0 + (uint64_t)lane;
-    double invMas[N];
-    float pd[N - 1];
-    EventWordsM<N, ENGINE> words;
-    words.init(a.d, a.gen_first + k0, lane);
-    const double wt = weight_chain<N, ENGINE>(a.d, words, invMas, pd);
-    const double u = (double)squares3_y(a.uw_key, ykey) * 0x1p-32;  // ctr = jump + k
-    const bool acc = k < a.n && ratio_above(wt, a.maxv, rmax, rmax_ok, u);
-    const uint64_t ball = __ballot(acc);
-    if (lane == 0) *(glob(a.flags) + g) = ball;
-    c += (uint32_t)__popcll(ball);
-  }
-  if (lane == 0) cnt[wave] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) a.tile_count[tile] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
-}
+// therefore needs only each event's FP64 weight chain (k_uw_flags_mixed,
+// generate_unweight.hip); k_uw_regen_mixed then generates every event again,
+// with the same code and so the same bits, at its place in the stable
+// partition.
 
 // Squares words of one event from the block's words in LDS (as k_uw_regen):
 // thread t's own word (counter c0 + t) and words j >= 1 at w[j], w = sw + t.
@@ -518,16 +430,10 @@ constexpr int kNumN = HPSP_MAX_N - 1;
 constexpr int kNumE = 6;
 
 template <int N, int E>
-static void gu_mixed_one(const UnweightArgs& a, hipStream_t s, int part) {
-  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
-  if (part == 0) {
-    const unsigned grid = (unsigned)((ntiles + kXcds - 1) / kXcds * kXcds);
-    hipLaunchKernelGGL((k_uw_flags_mixed<N, E>), dim3(grid), dim3(kBlock), 0, s, a);
-  } else {
-    const uint64_t nblocks = (a.n + kBlock - 1) / kBlock;
-    const uint64_t grid = (nblocks + kXcds - 1) / kXcds * kXcds;
-    hipLaunchKernelGGL((k_uw_regen_mixed<N, E>), dim3((unsigned)grid), dim3(kBlock), 0, s, a);
-  }
+static void regen_mixed_one(const UnweightArgs& a, hipStream_t s) {
+  const uint64_t nblocks = (a.n + kBlock - 1) / kBlock;
+  const uint64_t grid = (nblocks + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL((k_uw_regen_mixed<N, E>), dim3((unsigned)grid), dim3(kBlock), 0, s, a);
 }
 
 template <int N, int E>
@@ -571,24 +477,19 @@ static constexpr auto avg_table(std::integer_sequence<int, I...>) {
 }
 
 template <int I>
-static constexpr auto gum_at = &gu_mixed_one<2 + I / kNumE, I % kNumE>;
+static constexpr auto rgm_at = &regen_mixed_one<2 + I / kNumE, I % kNumE>;
 template <int... I>
-static constexpr auto gum_table(std::integer_sequence<int, I...>) {
-  return std::array<void (*)(const UnweightArgs&, hipStream_t, int), sizeof...(I)>{gum_at<I>...};
+static constexpr auto rgm_table(std::integer_sequence<int, I...>) {
+  return std::array<void (*)(const UnweightArgs&, hipStream_t), sizeof...(I)>{rgm_at<I>...};
 }
 
 static const auto kGen = gen_table(std::make_integer_sequence<int, kNumN * kNumE>{});
 static const auto kAvg = avg_table(std::make_integer_sequence<int, kNumN * kNumE>{});
-static const auto kGuMixed = gum_table(std::make_integer_sequence<int, kNumN * kNumE>{});
-
-int launch_flags_mixed(int N, int engine, const UnweightArgs& a, hipStream_t s) {
-  kGuMixed[(N - 2) * kNumE + engine](a, s, 0);
-  return (int)hipGetLastError();
-}
+static const auto kRegenMixed = rgm_table(std::make_integer_sequence<int, kNumN * kNumE>{});
 
 int launch_unweight_regen_mixed(int N, int engine, const UnweightArgs& a, hipStream_t s) {
   if (a.n == 0) return (int)hipSuccess;
-  kGuMixed[(N - 2) * kNumE + engine](a, s, 1);
+  kRegenMixed[(N - 2) * kNumE + engine](a, s);
   return (int)hipGetLastError();
 }
 

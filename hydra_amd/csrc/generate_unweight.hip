@@ -45,6 +45,7 @@
 #include <utility>
 
 #include "genbod_f64.h"
+#include "genbod_mixed.h"
 #include "uw_dst.h"
 
 namespace hpsp {
@@ -74,6 +75,14 @@ struct LaneEventWords {
   }
 };
 
+// The N - 2 mass words of one event, held in registers (so that a lane can
+// run the weight chain on its own, outside the wave's word sharing).
+template <int N>
+struct MassWords {
+  uint64_t w[N > 2 ? N - 2 : 1];
+  __device__ __forceinline__ uint64_t operator()(int j) const { return w[j]; }
+};
+
 // A wave-uniform 64-bit value from the first active lane (scalar registers).
 __device__ __forceinline__ uint64_t first_lane(uint64_t v) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
@@ -97,16 +106,17 @@ struct OneWord {
 // (d1 (a^2 + c|b-c|) + d2 (a^2 + c(b+c))) / (d1 d2) + 1, and B needs
 // only estimates of 1/(d1 d2) and 1/a^2: it is a bound with a wide margin,
 // formed with fused multiply-adds.
-template <int N, int ENGINE, typename WordFn>
+template <int N, int ENGINE, bool MOM = true, typename WordFn>
 __device__ __forceinline__ double chain_q2(const DecayK& d, const WordFn& word, double C,
                                            double ks_fixed, double ks_unit, double ic2_0,
                                            double& B, bool& sure) {
 #pragma clang fp contract(off)
   constexpr double eps = 0x1p-53;
   // ks = (2K + 4) eps S^2: fixed K for the squares engines, per event for
-  // the Random123 ones (polar angle words N - 2 + 2 (i - 1), as GENBOD)
-  double ks = ks_fixed;
-  if constexpr (!is_squares<ENGINE>()) {
+  // the Random123 ones (polar angle words N - 2 + 2 (i - 1), as GENBOD); 0
+  // without the momenta's term (MOM false: the rounding terms alone)
+  double ks = MOM ? ks_fixed : 0.0;
+  if constexpr (MOM && !is_squares<ENGINE>()) {
     double s2 = 1.0;
 #pragma unroll
     for (int i = 1; i < N; ++i) {
@@ -248,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_table3(const DecayK d, double2
 
 // The 3-body chain: q^2 as chain_q2 forms it, with the step of the common A
 // folded into C3 = C / A^2, and B from the table.
-template <int ENGINE, typename WordFn>
+template <int ENGINE, bool MOM = true, typename WordFn>
 __device__ __forceinline__ double chain3_q2(const DecayK& d, const WordFn& word, const ChainK& ck,
                                             const double2* XY, double& B) {
 #pragma clang fp contract(off)
@@ -263,8 +273,8 @@ __device__ __forceinline__ double chain3_q2(const DecayK& d, const WordFn& word,
   const double x0 = ((a0 - spc0) * (a0 + spc0)) * ((a0 - smc0) * (a0 + smc0));
   const double spc1 = m2 + a0, smc1 = m2 - a0;
   const double x1 = ((A - spc1) * (A + spc1)) * ((A - smc1) * (A + smc1));
-  double ks = ck.ks_fixed;
-  if constexpr (!is_squares<ENGINE>()) {
+  double ks = MOM ? ck.ks_fixed : 0.0;
+  if constexpr (MOM && !is_squares<ENGINE>()) {
     double s2 = 1.0;
 #pragma unroll
     for (int i = 1; i < 3; ++i) {
@@ -419,8 +429,115 @@ __global__ __launch_bounds__(kBlock) void k_uw_patch_chain(const UnweightArgs a,
 
 __global__ void k_zero_count(uint64_t* p) { *p = 0; }
 
+// The flags of the mixed-precision form (hpsp_generate_unweight_mixed): the
+// test RN(w / max) > u on the mixed container's weight column, w = the mixed
+// Generate's FP64 weight chain (one-ulp square roots and reciprocals,
+// genbod_mixed.h).  The squared chain of k_uw_flags_chain, with the bound's
+// rounding terms alone (no momenta are involved), gives q^2 within
+// B_m = 4 B(K = 0) of both w and the exact chain for the same invariant
+// masses: each pdk factor (a - b - c etc.) of either chain carries a rounding
+// of a few eps of a, amplified at most 3 (ka + kc) times, and the square
+// roots, reciprocals and products a few eps more, which B(K = 0) =
+// sum_i 4 eps (ka + kc) + 40 N eps covers four times over.  Where the test on
+// q^2 is not settled the lane forms w with the mixed Generate's code and
+// applies the reference's test.
+template <int N, int ENGINE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(waves_for(N), 8)))
+void k_uw_flags_mixed(const UnweightArgs a, const ChainK ck, const double* tab3) {
+#pragma clang fp contract(off)
+  __shared__ uint32_t cnt[kBlock / kWave];
+  const int lane = __lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
+  const uint64_t tile = xcd_tile(ntiles);
+  if (tile >= ntiles) return;  // whole block
+  [[maybe_unused]] const double2* XY3 = reinterpret_cast<const double2*>(tab3);
+  const uint64_t off = (uint64_t)wave * kWave + (uint64_t)lane;
+  uint64_t ykey = a.uw_key * (a.jump + tile * kUwTile + off);
+  const uint64_t ystep = a.uw_key * (uint64_t)kBlock;
+  uint64_t ygen = a.d.key[0] * (a.gen_first + tile * kUwTile + off + 3 * (uint64_t)N);
+  const uint64_t gstep = a.d.key[0] * (uint64_t)kBlock;
+  uint32_t c = 0;
+  constexpr int kUnroll = N <= 6 || is_squares<ENGINE>() ? kItems : 1;
+#pragma unroll kUnroll
+  for (int j = 0; j < kItems; ++j, ykey += ystep, ygen += gstep) {
+    const uint64_t g = tile * (kUwTile / kWave) + (uint64_t)j * (kBlock / kWave) + wave;
+    const uint64_t k0 = g * kWave;
+    const uint64_t k = k0 + (uint64_t)lane;
+    // the event's mass words, from the wave's shared words where they are
+    // shared, before any lane branches off
+    MassWords<N> mw;
+    if constexpr (N == 3 && is_squares<ENGINE>()) {
+      mw.w[0] = squares_word_y<ENGINE>(a.d.key[0], ygen);  // counter gen_first + k + 3N
+    } else if constexpr (N > 2) {
+      EventWords<N, ENGINE> words;
+      words.init(a.d, a.gen_first + k0, lane);
+#pragma unroll
+      for (int i = 0; i < N - 2; ++i) mw.w[i] = words(i);
+    }
+    double B, q2;
+    bool sure;
+    if constexpr (N == 3) {
+      q2 = chain3_q2<ENGINE, false>(a.d, mw, ck, XY3, B);
+      B *= 4.0;
+      sure = B < 0x1p-10;  // false for NaN
+    } else {
+      q2 = chain_q2<N, ENGINE, false>(a.d, mw, ck.C, 0.0, 0.0, ck.ic2_0, B, sure);
+      B *= 4.0;
+      sure = sure && B < 0x1p-10;
+    }
+    const bool live = k < a.n;
+    const double u = (double)squares3_y(a.uw_key, ykey) * 0x1p-32;  // ctr = jump + k
+    const double u2 = u * u;
+    const double e = B + 0x1p-47;
+    sure = sure && ck.fast_ok && q2 >= 0x1p-1000 && q2 <= 0x1p+1000 &&
+           __builtin_fabs(q2 - u2) > q2 * (e * (2.0 + e));
+    bool acc = q2 > u2;
+    if (live && !sure) {
+      // the container's weight, as the mixed Generate forms it
+      double invMas[N];
+      float pd[N - 1];
+      const double w = weight_chain<N, ENGINE>(a.d, mw, invMas, pd);
+      acc = ratio_above(w, a.maxv, ck.rmax, ck.rmax_ok, u);
+    }
+    const uint64_t ball = __ballot(live && acc);
+    if (lane == 0) *(glob(a.flags) + g) = ball;
+    c += (uint32_t)__popcll(ball);
+  }
+  if (lane == 0) cnt[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) a.tile_count[tile] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+}
+
 constexpr int kNumN = HPSP_MAX_N - 1;
 constexpr int kNumE = 6;
+
+template <int N, int E>
+static void flags_mixed_one(const UnweightArgs& a, double* tab3, const ChainK& ck,
+                            hipStream_t s) {
+  const uint64_t ntiles = (a.n + kUwTile - 1) / kUwTile;
+  const unsigned grid = (unsigned)((ntiles + kXcds - 1) / kXcds * kXcds);
+  if constexpr (N == 3)
+    hipLaunchKernelGGL(k_chain_table3, dim3((kBin3 + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       a.d, reinterpret_cast<double2*>(tab3));
+  hipLaunchKernelGGL((k_uw_flags_mixed<N, E>), dim3(grid), dim3(kBlock), 0, s, a, ck,
+                     (const double*)tab3);
+}
+
+template <int I>
+static constexpr auto fm_at = &flags_mixed_one<2 + I / kNumE, I % kNumE>;
+template <int... I>
+static constexpr auto fm_table(std::integer_sequence<int, I...>) {
+  return std::array<void (*)(const UnweightArgs&, double*, const ChainK&, hipStream_t),
+                    sizeof...(I)>{fm_at<I>...};
+}
+static const auto kFlagsMixed = fm_table(std::make_integer_sequence<int, kNumN * kNumE>{});
+
+int launch_flags_mixed(int N, int engine, const UnweightArgs& a, double* tab3, const ChainK& ck,
+                       hipStream_t s) {
+  kFlagsMixed[(N - 2) * kNumE + engine](a, tab3, ck, s);
+  return (int)hipGetLastError();
+}
 
 // Blocks of the list kernel: enough waves to fill the chip at its occupancy
 // when every group is in the list, few enough that an almost empty list

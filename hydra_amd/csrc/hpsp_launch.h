@@ -221,13 +221,17 @@ struct ChainK {
   double ks_fixed;  // (2K + 4) eps S^2 with the squares engines' K = 2^17 N
   double ic2_0;     // 1 / m0^2
   int fast_ok;      // max and maxw positive, 1/max and C in range
+  double rmax;      // RN(1 / max) and whether it is normal (ratio_above)
+  int rmax_ok;
 };
 int launch_flags_chain(int N, int engine, const UnweightArgs& a, uint64_t* dl_count, uint64_t* dl,
                        double* tab3, const ChainK& ck, hipStream_t s);
-// The mixed form (genbod_mixed.hip): flags from each event's FP64 weight
-// chain, the mixed Generate's weight column; then every event generated again
-// at its place, FP32 columns in a.cols_out (carried as a double pointer).
-int launch_flags_mixed(int N, int engine, const UnweightArgs& a, hipStream_t s);
+// The mixed form: flags from each event's FP64 weight chain, the mixed
+// Generate's weight column (generate_unweight.hip; tab3 and ck as for
+// launch_flags_chain); then every event generated again at its place, FP32
+// columns in a.cols_out, carried as a double pointer (genbod_mixed.hip).
+int launch_flags_mixed(int N, int engine, const UnweightArgs& a, double* tab3, const ChainK& ck,
+                       hipStream_t s);
 int launch_unweight_regen_mixed(int N, int engine, const UnweightArgs& a, hipStream_t s);
 int launch_max_weight(int N, const WeightArgs& a, double* block_max, uint64_t nblocks,
                       double* out, hipStream_t s);

@@ -295,3 +295,57 @@ def test_table3_bounds_every_event(oracle, case):
         Bt = ks * X + Y
         frac = np.mean(Bt[bin3(w[:1 << 20] >> np.uint64(32))] >= 2.0 ** -10)
         assert frac < 1e-3, frac
+
+
+@pytest.mark.parametrize("case", THREE_BODY, ids=[c[0] for c in THREE_BODY])
+def test_mixed_chain_within_rounding_bound(oracle, case):
+    """The mixed form's flags (k_uw_flags_mixed) trust the squared chain where
+    |q^2 - u^2| exceeds its margin with B_m = 4 B(K = 0): the mixed weight
+    chain (pdk as the reference orders it, (a - b - c)(a + b + c)(a - b + c)
+    (a + b - c), with one-ulp square roots and reciprocals) and the squared
+    chain (d1 d2 from b + c and b - c) must both lie within B_m of the exact
+    chain for the same invariant masses.  Checked in FP64 with correctly
+    rounded square roots and divisions (the device's one-ulp ones add an ulp
+    or two, far inside B_m's 480 eps floor) against the chain in 64-bit
+    extended precision, on 2^18 random mass uniforms and the ends of [0, 1]."""
+    name, M, masses = case
+    m = np.asarray(masses, dtype=np.float64)
+    rc, ecm, mw = oracle.params(M, m)
+    rng = np.random.default_rng(5)
+    r = np.concatenate([rng.random(1 << 18), [2.0 ** -30, 1e-9, 1 - 2.0 ** -30, 0.5]])
+    s01 = (0.0 + m[0]) + m[1]
+    A = 1.0 * ecm + (s01 + m[2])
+    a0 = r * ecm + s01
+    with np.errstate(divide="ignore", invalid="ignore"):
+        # the mixed chain: pdk(invMas[n+1], invMas[n], m[n+1]) in the reference's order
+        def pdk_ref(a, b, c):
+            return np.sqrt((a - b - c) * (a + b + c) * (a - b + c) * (a + b - c)) / (2 * a)
+        wt_m = mw * pdk_ref(a0, m[0], m[1]) * pdk_ref(A, a0, m[2])
+        # the squared chain (chain3_q2): q^2 max^2 = maxw^2 x0 x1 / (4 a0^2 A^2)
+        spc0, smc0 = m[1] + m[0], m[1] - m[0]
+        x0 = ((a0 - spc0) * (a0 + spc0)) * ((a0 - smc0) * (a0 + smc0))
+        spc1, smc1 = m[2] + a0, m[2] - a0
+        x1 = ((A - spc1) * (A + spc1)) * ((A - smc1) * (A + smc1))
+        wt_q = np.sqrt(mw * mw / 16.0 / (A * A) * (x0 * x1) / (a0 * a0))
+        # the exact chain in extended precision
+        L = np.longdouble
+        aL, AL = a0.astype(L), L(A)
+        def pdk_L(a, b, c):
+            b, c = L(b) if np.isscalar(b) else b, L(c)
+            return np.sqrt((a * a - (b + c) ** 2) * (a * a - (b - c) ** 2)) / (2 * a)
+        wt_x = L(mw) * pdk_L(aL, m[1], m[0]) * pdk_L(AL, m[2], aL)
+        # B(K = 0) per event, as chain_q2 forms it without the momenta's term
+        B = np.full(r.shape, 120 * EPS)
+        ok = np.ones(r.shape, dtype=bool)
+        for a, b, c in ((a0, m[1], np.full_like(r, m[0])), (np.full_like(r, A), m[2], a0)):
+            spc, smc = b + c, b - c
+            d1, d2 = (a - spc) * (a + spc), (a - smc) * (a + smc)
+            ok &= np.minimum(d1, d2) > 0
+            kk = (a * a + c * np.abs(smc)) / d2 + (a * a + c * spc) / d1 + 1
+            B += 4 * EPS * kk
+    Bm = 4 * B
+    sel = ok & (Bm < 2.0 ** -10) & (wt_x > 0)
+    assert sel.mean() > 0.5, name
+    em = np.abs((wt_m[sel].astype(L) - wt_x[sel]) / wt_x[sel]).astype(np.float64)
+    eq = np.abs((wt_q[sel].astype(L) - wt_x[sel]) / wt_x[sel]).astype(np.float64)
+    assert np.all(em + eq <= Bm[sel]), (name, float(np.max((em + eq) / Bm[sel])))

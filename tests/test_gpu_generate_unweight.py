@@ -181,7 +181,7 @@ def test_table3_on_device_equals_restatement(oracle, M, masses):
 
 def generate_unweight_mixed(dec, gen_seed, first, n, seed, ld=None):
     ld = ld or max(n, 1)
-    ws_bytes = lib().hpsp_unweight_workspace_size(n)
+    ws_bytes = lib().hpsp_generate_unweight_workspace_size(n)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
     out = torch.empty((4 * dec.n, ld), dtype=torch.float32, device=DEV)
     idx = torch.empty(max(n, 1), dtype=torch.int64, device=DEV)
@@ -231,6 +231,13 @@ def test_generate_unweight_mixed_sizes_and_headline():
     same_mixed(dec, 159753, 0, 1 << 22, UW_SEED)
     near = make_decay(10.0, [0.999] * 10)
     same_mixed(near, 2024, 3, 1 << 18, UW_SEED)
+    # where the squared chain's bound is large and lanes take the mixed chain
+    M3 = 3.0969 + 0.493677 + 0.13957
+    for M, m, eng in ((M3 + 0.001, [3.0969, 0.493677, 0.13957], "squares3"),
+                      (100.0, [0.000511, 0.000511, 90.0], "philox"),
+                      (5.279, [1e-6, 0.493677, 0.13957], "squares4"),
+                      (20.0, [1e-6] + [0.5] * 11, "threefry")):
+        same_mixed(make_decay(M, m, engine=eng), 2024, 3, 1 << 18, UW_SEED)
 
 
 def test_generate_unweight_mixed_empty_and_errors():
@@ -239,7 +246,7 @@ def test_generate_unweight_mixed_empty_and_errors():
     assert k == 0
     nacc = torch.zeros(1, dtype=torch.int64, device=DEV)
     out = torch.empty((12, 1000), dtype=torch.float32, device=DEV)
-    ws_bytes = lib().hpsp_unweight_workspace_size(1000)
+    ws_bytes = lib().hpsp_generate_unweight_workspace_size(1000)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=DEV)
     assert lib().hpsp_generate_unweight_mixed(ctypes.byref(dec), 1, 0, 1000, 2, ptr(out), 1000,
                                               None, None, ptr(ws), 16, ptr(nacc),
