@@ -303,9 +303,10 @@ template <int N, int E>
 static void gen_one(const GenArgs& in, hipStream_t s) {
   GenArgs a = in;
   const uint64_t ngroups = (a.n + kWave - 1) / kWave;
-  // momentum columns of at most 3 daughters without PhaseSpaceWeight: bound
-  // by the stores (more daughters bring arithmetic enough to gain from 16)
-  uint64_t cap = (a.cols && !a.evw && N <= 3) ? kRunMaxStores : kRunMax;
+  // momentum columns of at most 3 daughters (with or without the weight
+  // column): bound by the stores (more daughters bring arithmetic enough to
+  // gain from 16)
+  uint64_t cap = (a.cols && N <= 3) ? kRunMaxStores : kRunMax;
   if (a.run_cap && a.run_cap < cap) cap = a.run_cap;
   a.run = run_groups(ngroups, cap);
   const unsigned g = grid_for_runs((ngroups + a.run - 1) / a.run);

@@ -255,11 +255,12 @@ int launch_rng_words(int engine, const uint64_t* key, uint64_t discard, uint64_t
 // k_evaluate_f64 over one mother): a longer run shares more engine words
 // between consecutive groups, but the runs must still give every SIMD several
 // waves; the longest power of two up to max_run that leaves kRunMinRuns runs.
-// A launch bound by its column stores (momenta of at most 3 daughters without
-// PhaseSpaceWeight) takes max_run = kRunMaxStores: longer runs spread the
-// chip's concurrent writes over a wider window of the columns (3-body momenta
-// alone at 2^27 events: 2.25 ms with runs of 4, 2.42 ms with 16), where
-// launches with more arithmetic per byte gain from 16 (4 to 8 daughters: 2-4%).
+// A launch bound by its column stores (momenta of at most 3 daughters, with or
+// without the weight column) takes max_run = kRunMaxStores: longer runs spread
+// the chip's concurrent writes over a wider window of the columns (3-body
+// momenta alone at 2^27 events: 2.25 ms with runs of 4, 2.42 ms with 16;
+// momenta and weights 2.55 ms against 2.63-2.69), where launches with more
+// arithmetic per byte gain from 16 (4 to 8 daughters: 2-4%).
 inline uint32_t run_groups(uint64_t ngroups, uint64_t max_run = kRunMax) {
   uint64_t r = 1;
   while (r < max_run && ngroups >= 2 * r * kRunMinRuns) r *= 2;
