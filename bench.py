@@ -333,27 +333,37 @@ def run_c2_mixed(torch, capi, n=1 << 27):
     the mixed container's weight column, the FP64 generation weight)."""
     N = 3
     res = {"events": n}
+    lib = capi.lib()
+    dec = capi.make_decay(M3, MASSES3)
+    sh = capi.stream_handle()
+    # both precisions timed in alternating rounds, so that both see the same clocks
     for tag, with_w in (("generate", True), ("momenta_only", False)):
+        gen = {}
         for prec, f32, name in ((capi.FP64, False, "fp64"), (capi.MIXED, True, "mixed")):
-            lib = capi.lib()
-            dec = capi.make_decay(M3, MASSES3)
             cols = torch.empty((4 * N, n), dtype=torch.float32 if f32 else torch.float64,
                                device="cuda:0")
             evw = torch.empty(n, dtype=torch.float64, device="cuda:0") if with_w else None
-            sh = capi.stream_handle()
-            t = Timer(torch)
-            for i in range(6):
-                if i:
-                    t.start()
-                capi.check(lib.hpsp_generate(ctypes.byref(dec), SEED, 0, n, prec, capi.ptr(cols),
-                                             n, int(f32), None, capi.ptr(evw), sh), "generate")
-                if i:
-                    t.stop()
+            gen[name] = (prec, f32, cols, evw, Timer(torch))
+
+        def gcall(name):
+            prec, f32, cols, evw, _ = gen[name]
+            capi.check(lib.hpsp_generate(ctypes.byref(dec), SEED, 0, n, prec, capi.ptr(cols),
+                                         n, int(f32), None, capi.ptr(evw), sh), "generate")
+
+        for name in gen:
+            gcall(name)
+        for _ in range(3):
+            for name, g in gen.items():
+                for _ in range(4):
+                    g[4].start()
+                    gcall(name)
+                    g[4].stop()
+        for name, (_, f32, _, _, t) in gen.items():
             res[f"{tag}_{name}_ms"] = t.avg_ms
             b = n * ((16 if f32 else 32) * N + (8 if with_w else 0))
             res[f"{tag}_{name}_GBs"] = b / (t.avg_ms * 1e6)
-            del cols, evw
-            torch.cuda.empty_cache()
+        del gen
+        torch.cuda.empty_cache()
         res[f"{tag}_mixed_speedup"] = res[f"{tag}_fp64_ms"] / res[f"{tag}_mixed_ms"]
     # the headline step (Generate + Unweight() in one call) in both forms: FP64
     # columns (hpsp_generate_unweight) and FP32 ones (hpsp_generate_unweight_mixed),

@@ -101,6 +101,7 @@ class ServerInfo(ctypes.Structure):
 
 
 NOALLOC_MAX_EVENTS = 1 << 28  # HPSP_NOALLOC_MAX_EVENTS
+MAX_EVENTS = 1 << 48  # HPSP_MAX_EVENTS: events per call of any entry point
 
 REQ_GENERATE, REQ_AVERAGE = 0, 1
 ERR_SHUTDOWN = 7

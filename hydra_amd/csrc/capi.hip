@@ -18,6 +18,9 @@ namespace {
 constexpr size_t kAlign = 256;
 size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
+// n beyond the per-call limit: no size or grid derived from it may wrap
+bool too_many(uint64_t n) { return n > HPSP_MAX_EVENTS; }
+
 struct UwLayout {
   size_t flags, tile_count, tile_off, run_sum, maxv, block_max, total;
 };
@@ -144,7 +147,7 @@ const char* hpsp_error_string(int status) {
 
 int hpsp_rng_words(int engine, uint64_t seed, uint64_t discard, uint64_t n, uint64_t* out,
                    void* stream) {
-  if (!valid_engine(engine)) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!valid_engine(engine) || too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   if (n == 0) return HPSP_OK;
   if (!out) return HPSP_ERR_INVALID_ARGUMENT;
   uint64_t key[4];
@@ -158,6 +161,7 @@ static int generate_common(const hpsp_decay* d, uint64_t seed, uint64_t first, u
   int rc = check_decay(d);
   if (rc) return rc;
   if (prec != HPSP_FP64 && prec != HPSP_MIXED) return HPSP_ERR_INVALID_ARGUMENT;
+  if (too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   if (n == 0) return HPSP_OK;
   if (!cols && !wt && !evw) return HPSP_ERR_INVALID_ARGUMENT;
   if (cols && ld < n) return HPSP_ERR_INVALID_ARGUMENT;
@@ -199,7 +203,7 @@ int evaluate_common(const hpsp_decay* d, const hpsp_functor* fs, int nf, uint64_
                     double* out, uint64_t ld, void* stream) {
   int rc = check_decay(d);
   if (rc) return rc;
-  if (nf < 1 || nf > HPSP_MAX_EVAL_FUNCTORS || !fs) return HPSP_ERR_INVALID_ARGUMENT;
+  if (nf < 1 || nf > HPSP_MAX_EVAL_FUNCTORS || !fs || too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   if (n == 0) return HPSP_OK;
   if (!out || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
   if (mothers && mld < n) return HPSP_ERR_INVALID_ARGUMENT;
@@ -289,7 +293,7 @@ int hpsp_histogram_fill(const hpsp_histogram* h, const double* x, uint64_t ldx, 
   memset(&a, 0, sizeof(a));
   int rc = make_histk(h, &a.h);
   if (rc) return rc;
-  if (!contents || (n > 0 && (!x || ldx < n))) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!contents || too_many(n) || (n > 0 && (!x || ldx < n))) return HPSP_ERR_INVALID_ARGUMENT;
   a.x = x;
   a.ldx = ldx;
   a.w = w;
@@ -313,7 +317,7 @@ int hpsp_histogram_fill_events(const hpsp_histogram* h, const hpsp_decay* d,
   if (!contents || !axes) return HPSP_ERR_INVALID_ARGUMENT;
   if (weight_mode < HPSP_HIST_WEIGHT_ONE || weight_mode > HPSP_HIST_WEIGHT_PHSP)
     return HPSP_ERR_INVALID_ARGUMENT;
-  if (n > 0 && (!cols || ld < n)) return HPSP_ERR_INVALID_ARGUMENT;
+  if (too_many(n) || (n > 0 && (!cols || ld < n))) return HPSP_ERR_INVALID_ARGUMENT;
   if (n > 0 && weight_mode == HPSP_HIST_WEIGHT_COLUMN && !w) return HPSP_ERR_INVALID_ARGUMENT;
   for (int i = 0; i < h->ndim; ++i) {
     rc = make_functork(&axes[i], d->n, &a.axes[i]);
@@ -335,7 +339,7 @@ int hpsp_histogram_fill_events(const hpsp_histogram* h, const hpsp_decay* d,
 namespace {
 
 int make_particlesk(const hpsp_particles* p, uint64_t n, ParticlesK* k) {
-  if (!p || p->n < 1 || p->n > HPSP_MAX_PARTICLES) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!p || p->n < 1 || p->n > HPSP_MAX_PARTICLES || too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   memset(k, 0, sizeof(*k));
   k->n = p->n;
   for (int i = 0; i < p->n; ++i) {
@@ -408,7 +412,7 @@ int hpsp_range_init(hpsp_decay* d, const double mother4[4], int n, const double*
 namespace {
 
 int range_args(const hpsp_decay* d, uint64_t seed, uint64_t first, uint64_t n, RangeArgs* a) {
-  if (!d || d->engine != HPSP_MINSTD_RAND || d->n < 2 || d->n > HPSP_MAX_N)
+  if (!d || d->engine != HPSP_MINSTD_RAND || d->n < 2 || d->n > HPSP_MAX_N || too_many(n))
     return HPSP_ERR_INVALID_ARGUMENT;
   const uint64_t skip = 3 * (uint64_t)d->n;
   if (first > UINT64_MAX - skip || n > UINT64_MAX - skip - first) return HPSP_ERR_INVALID_ARGUMENT;
@@ -465,6 +469,7 @@ int hpsp_event_weight(const hpsp_decay* d, const hpsp_functor* f, const double* 
                       uint64_t ld, uint64_t n, double* w_out, void* stream) {
   int rc = check_decay_masses(d);
   if (rc) return rc;
+  if (too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   if (n == 0) return HPSP_OK;
   if (!cols || !w_out || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
   WeightArgs a;
@@ -480,14 +485,16 @@ int hpsp_event_weight(const hpsp_decay* d, const hpsp_functor* f, const double* 
   return hip_status(launch_event_weight(d->n, a, as_stream(stream)));
 }
 
-size_t hpsp_unweight_workspace_size(uint64_t n) { return uw_layout(n).total; }
+size_t hpsp_unweight_workspace_size(uint64_t n) {
+  return too_many(n) ? SIZE_MAX : uw_layout(n).total;
+}
 
 int hpsp_max_weight(const hpsp_decay* d, const hpsp_functor* f, const double* cols,
                     uint64_t ld, uint64_t n, void* ws, size_t ws_bytes, double* max_dev,
                     void* stream) {
   int rc = check_decay_masses(d);
   if (rc) return rc;
-  if (n == 0 || !cols || !max_dev || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
+  if (n == 0 || too_many(n) || !cols || !max_dev || ld < n) return HPSP_ERR_INVALID_ARGUMENT;
   const UwLayout L = uw_layout(n);
   if (!ws || ws_bytes < L.total) return HPSP_ERR_WORKSPACE;
   WeightArgs a;
@@ -510,7 +517,7 @@ int hpsp_unweight(const hpsp_decay* d, const hpsp_functor* f, double max_value, 
                   void* ws, size_t ws_bytes, uint64_t* n_acc_dev, void* stream) {
   int rc = check_decay_masses(d);
   if (rc) return rc;
-  if (!n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!n_acc_dev || too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   if (n > 0) {
     if (!cols_in || ld_in < n) return HPSP_ERR_INVALID_ARGUMENT;
     if (cols_out && ld_out < n) return HPSP_ERR_INVALID_ARGUMENT;
@@ -561,7 +568,8 @@ int hpsp_unweight_values(int n_particles, const double* values, double max_value
                          const double* cols_in, uint64_t ld_in, uint64_t n, double* cols_out,
                          uint64_t ld_out, uint64_t* idx_out, double* values_out, void* ws,
                          size_t ws_bytes, uint64_t* n_acc_dev, void* stream) {
-  if (n_particles < 2 || n_particles > HPSP_MAX_N || !n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (n_particles < 2 || n_particles > HPSP_MAX_N || !n_acc_dev || too_many(n))
+    return HPSP_ERR_INVALID_ARGUMENT;
   if (n > 0) {
     if (!values) return HPSP_ERR_INVALID_ARGUMENT;
     if (cols_out && (!cols_in || ld_in < n || ld_out < n)) return HPSP_ERR_INVALID_ARGUMENT;
@@ -605,7 +613,7 @@ int hpsp_unweight_regenerate(const hpsp_decay* d, uint64_t gen_seed, uint64_t fi
                              size_t ws_bytes, uint64_t* n_acc_dev, void* stream) {
   int rc = check_decay(d);
   if (rc) return rc;
-  if (!n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!n_acc_dev || too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   if (n > 0 && (!w || (cols_out && ld_out < n))) return HPSP_ERR_INVALID_ARGUMENT;
   if (first > UINT64_MAX - n) return HPSP_ERR_INVALID_ARGUMENT;
   const UwLayout L = uw_layout(n);
@@ -648,6 +656,7 @@ static size_t gu_table_offset(uint64_t n) {
   return gu_list_offset(n) + align_up(2 * ngroups * sizeof(uint64_t));
 }
 size_t hpsp_generate_unweight_workspace_size(uint64_t n) {
+  if (too_many(n)) return SIZE_MAX;
   return gu_table_offset(n) + align_up(2 * kBin3 * sizeof(double));
 }
 
@@ -683,7 +692,7 @@ int hpsp_generate_unweight(const hpsp_decay* d, uint64_t gen_seed, uint64_t firs
                            void* stream) {
   int rc = check_decay(d);
   if (rc) return rc;
-  if (!n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!n_acc_dev || too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   if (n > 0 && cols_out && ld_out < n) return HPSP_ERR_INVALID_ARGUMENT;
   if (first > UINT64_MAX - n) return HPSP_ERR_INVALID_ARGUMENT;
   const UwLayout L = uw_layout(n);
@@ -740,7 +749,7 @@ int hpsp_generate_unweight_mixed(const hpsp_decay* d, uint64_t gen_seed, uint64_
                                  uint64_t* n_acc_dev, void* stream) {
   int rc = check_decay(d);
   if (rc) return rc;
-  if (!n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!n_acc_dev || too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   if (n > 0 && cols_out && ld_out < n) return HPSP_ERR_INVALID_ARGUMENT;
   if (first > UINT64_MAX - n) return HPSP_ERR_INVALID_ARGUMENT;
   const UwLayout L = uw_layout(n);
@@ -780,6 +789,7 @@ int hpsp_generate_unweight_mixed(const hpsp_decay* d, uint64_t gen_seed, uint64_
 }
 
 size_t hpsp_unweight_sample_workspace_size(uint64_t n) {
+  if (too_many(n)) return SIZE_MAX;
   return uw_layout(n).total + align_up((n ? n : 1) * sizeof(uint64_t));
 }
 
@@ -788,7 +798,7 @@ int hpsp_unweight_sample(int ncols, const double* cols_in, uint64_t ld_in, uint6
                          int order, double* cols_out, uint64_t ld_out, uint64_t* idx_out,
                          double* w_out, void* ws, size_t ws_bytes, uint64_t* n_acc_dev,
                          void* stream) {
-  if (ncols < 0 || !n_acc_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (ncols < 0 || !n_acc_dev || too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   if (order != HPSP_ORDER_STABLE && order != HPSP_ORDER_SEQUENTIAL)
     return HPSP_ERR_INVALID_ARGUMENT;
   if (n > 0) {
@@ -827,6 +837,7 @@ int hpsp_unweight_sample(int ncols, const double* cols_in, uint64_t ld_in, uint6
 }
 
 size_t hpsp_average_on_workspace_size(uint64_t n) {
+  if (too_many(n)) return SIZE_MAX;
   const uint64_t m = (n + kAvgChunk - 1) / kAvgChunk;
   return align_up((m ? m : 1) * sizeof(Stats));
 }
@@ -838,7 +849,7 @@ static int average_common(const hpsp_decay* d, const hpsp_functor* f, uint64_t s
   int rc = check_decay(d);
   if (rc) return rc;
   if (prec != HPSP_FP64 && prec != HPSP_MIXED) return HPSP_ERR_INVALID_ARGUMENT;
-  if (!out_dev) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!out_dev || too_many(n)) return HPSP_ERR_INVALID_ARGUMENT;
   if (mothers && mld < n) return HPSP_ERR_INVALID_ARGUMENT;
   if (!ws || ws_bytes < hpsp_average_on_workspace_size(n)) return HPSP_ERR_WORKSPACE;
   AvgArgs a;
@@ -894,7 +905,8 @@ int hpsp_average_on_mothers(const hpsp_decay* d, const hpsp_functor* f, uint64_t
 }
 
 int hpsp_checksum(const void* data, uint64_t bytes, uint64_t* sum_dev, void* stream) {
-  if (!sum_dev || (bytes % 8) != 0 || (bytes > 0 && !data)) return HPSP_ERR_INVALID_ARGUMENT;
+  if (!sum_dev || (bytes % 8) != 0 || too_many(bytes / 8) || (bytes > 0 && !data))
+    return HPSP_ERR_INVALID_ARGUMENT;
   return hip_status(launch_checksum(static_cast<const uint64_t*>(data), bytes / 8, sum_dev,
                                     as_stream(stream)));
 }

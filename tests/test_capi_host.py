@@ -276,3 +276,47 @@ def test_noalloc_limit_matches_header():
     text = open(capi.HEADER_PATH).read()
     m = re.search(r"#define HPSP_NOALLOC_MAX_EVENTS \(1ULL << (\d+)\)", text)
     assert m and capi.NOALLOC_MAX_EVENTS == 1 << int(m.group(1))
+
+
+def test_event_limit_matches_header():
+    import re
+    text = open(capi.HEADER_PATH).read()
+    m = re.search(r"#define HPSP_MAX_EVENTS \(1ULL << (\d+)\)", text)
+    assert m and capi.MAX_EVENTS == 1 << int(m.group(1))
+
+
+def test_event_counts_beyond_the_limit_are_refused():
+    # sizes derived from such a count could wrap: the workspace sizes say so
+    # (SIZE_MAX) and every counting entry point refuses it on the host
+    L = capi.lib()
+    big = capi.MAX_EVENTS + 1
+    size_max = (1 << 64) - 1
+    for f in (L.hpsp_unweight_workspace_size, L.hpsp_generate_unweight_workspace_size,
+              L.hpsp_unweight_sample_workspace_size, L.hpsp_average_on_workspace_size):
+        assert f(big) == size_max
+        assert f(capi.MAX_EVENTS) < size_max
+    assert L.hpsp_unweight_workspace_size((1 << 64) - 1) == size_max
+    d = capi.make_decay(5.279, [3.0969, 0.493677, 0.13957])
+    p = ctypes.c_void_p(64)
+    ws = ctypes.c_void_p(1 << 20)
+    one = capi.functor_array([capi.make_functor(capi.F_ONE)])
+    bad = capi.ERR_INVALID_ARGUMENT
+    for n in (big, (1 << 64) - 1):
+        assert L.hpsp_rng_words(0, 1, 0, n, p, None) == bad
+        assert L.hpsp_generate(ctypes.byref(d), 1, 0, n, 0, p, n, 0, None, None, None) == bad
+        assert L.hpsp_generate_mothers(ctypes.byref(d), 1, 0, n, p, n, 0, p, n, 0, None, None,
+                                       None) == bad
+        assert L.hpsp_evaluate(ctypes.byref(d), one, 1, 1, 0, n, p, n, None) == bad
+        assert L.hpsp_event_weight(ctypes.byref(d), None, p, n, n, p, None) == bad
+        assert L.hpsp_max_weight(ctypes.byref(d), None, p, n, n, ws, size_max, p, None) == bad
+        assert L.hpsp_unweight(ctypes.byref(d), None, -1.0, 1, p, n, n, p, None, 0, None, None,
+                               ws, size_max, p, None) == bad
+        assert L.hpsp_unweight_regenerate(ctypes.byref(d), 1, 0, n, p, 1, p, n, None, None, ws,
+                                          size_max, p, None) == bad
+        assert L.hpsp_generate_unweight(ctypes.byref(d), 1, 0, n, 1, p, n, None, None, ws,
+                                        size_max, p, None) == bad
+        assert L.hpsp_generate_unweight_mixed(ctypes.byref(d), 1, 0, n, 1, p, n, None, None, ws,
+                                              size_max, p, None) == bad
+        assert L.hpsp_average_on(ctypes.byref(d), None, 1, 0, n, 0, ws, size_max, p,
+                                 None) == bad
+    assert L.hpsp_checksum(p, big * 8, p, None) == bad
