@@ -218,23 +218,21 @@ __global__ __launch_bounds__(kBlock) void k_generate_mixed(const GenArgs a) {
           for (int i = 0; i < N; ++i)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-              __builtin_nontemporal_store(p[i][q],
-                                          uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane);
+              *(glob_u(c + (uint64_t)(4 * i + q) * a.ld) + lane) = p[i][q];
         } else {
           double* c = static_cast<double*>(a.cols) + k0;
 #pragma unroll
           for (int i = 0; i < N; ++i)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-              __builtin_nontemporal_store((double)p[i][q],
-                                          uniform_ptr(c + (uint64_t)(4 * i + q) * a.ld) + lane);
+              *(glob_u(c + (uint64_t)(4 * i + q) * a.ld) + lane) = (double)p[i][q];
         }
-        if (a.wt) __builtin_nontemporal_store(wt, uniform_ptr(a.wt + k0) + lane);
+        if (a.wt) *(glob_u(a.wt + k0) + lane) = wt;
         // PhaseSpaceWeight of the mixed form: the FP64 generation weight,
         // which it equals in exact arithmetic.  Recomputed from the FP32
         // momenta (masses of running sums, then pdk) it would lose up to all
         // of its digits where a - b - c cancels near threshold.
-        if (a.evw) __builtin_nontemporal_store(wt, uniform_ptr(a.evw + k0) + lane);
+        if (a.evw) *(glob_u(a.evw + k0) + lane) = wt;
       }
     }
   }

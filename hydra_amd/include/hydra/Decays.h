@@ -2,8 +2,9 @@
 // (reference: hydra/Decays.h:67-651, hydra/detail/Decays.inl:114-511).
 //
 // Device storage is structure-of-arrays by component: 4N FP64 columns of
-// capacity() events each (column 4*i + c holds component c of particle i),
-// so every kernel access is coalesced.  Rows are assembled into tuples of
+// capacity() events each (column 4*i + c holds component c of particle i;
+// capacity() is a multiple of 64, detail::column_stride), so every kernel
+// access is coalesced and every column starts on a cache line.  Rows are assembled into tuples of
 // Vector4R on access.  The container also keeps the PhaseSpaceWeight of each
 // event when it is known exactly (written by PhaseSpace::Generate, permuted by
 // Unweight), so Unweight need not re-read the momenta to flag events.
@@ -279,7 +280,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     fWValid = false;
   }
   void shrink_to_fit() {
-    if (fCap > fSize) regrow(fSize);
+    if (fCap > detail::column_stride(fSize)) regrow(fSize);
   }
 
   value_type operator[](size_t k) const {
@@ -507,14 +508,16 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     if (fCheck && fGenValid) guard_record();
   }
 
+  // Columns for at least cap events, column_stride(cap) apart.
   void regrow(size_t cap) {
     materialize();
-    detail::device_buffer<double> cols(4 * N * cap);
-    detail::device_buffer<double> w(cap);
+    const size_t ld = detail::column_stride(cap);
+    detail::device_buffer<double> cols(4 * N * ld);
+    detail::device_buffer<double> w(ld);
     const size_t keep = fSize < cap ? fSize : cap;
     if (keep && fCols.data())
       for (size_t c = 0; c < 4 * N; ++c)
-        detail::hpsp_check(hpsp_memcpy_device(cols.data() + c * cap, fCols.data() + c * fCap,
+        detail::hpsp_check(hpsp_memcpy_device(cols.data() + c * ld, fCols.data() + c * fCap,
                                               keep * sizeof(double), nullptr),
                            "Decays::regrow");
     if (keep && fWValid)
@@ -523,7 +526,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     detail::hpsp_sync("Decays::regrow");
     fCols.swap(cols);
     fW.swap(w);
-    fCap = cap;
+    fCap = ld;
   }
 
   // hydra::unweight(device::sys, events, functor, ...) (Random.inl:134-230):
@@ -543,8 +546,9 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     // the container's own PhaseSpaceWeight column moves with the events
     detail::hpsp_check(hpsp_unweight_sample(4 * (int)N, fCols.data(), fCap, n, w.data(), max_pdf,
                                             (uint64_t)rng_seed, (uint64_t)rng_jump,
-                                            HPSP_ORDER_SEQUENTIAL, out.data(), n ? n : 1,
-                                            nullptr, nullptr, ws, wsb, spare_nacc(), nullptr),
+                                            HPSP_ORDER_SEQUENTIAL, out.data(),
+                                            detail::column_stride(n), nullptr, nullptr, ws, wsb,
+                                            spare_nacc(), nullptr),
                        "hydra::unweight");
     detail::device_buffer<double>* wout = nullptr;
     if (wc) {
@@ -569,8 +573,9 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     auto& out = spare_cols(n);
     const size_t wsb = hpsp_unweight_workspace_size(n);
     detail::hpsp_check(hpsp_unweight_regenerate(&fGen, fGenSeed, 0, n, fW.data(), (uint64_t)seed,
-                                                out.data(), n ? n : 1, nullptr, nullptr,
-                                                spare_ws(wsb), wsb, spare_nacc(), nullptr),
+                                                out.data(), detail::column_stride(n), nullptr,
+                                                nullptr, spare_ws(wsb), wsb, spare_nacc(),
+                                                nullptr),
                        "Decays::Unweight");
     fWValid = false;
     return adopt(out, nullptr);
@@ -585,8 +590,8 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     auto& out = spare_cols(n);
     const size_t wsb = hpsp_generate_unweight_workspace_size(n);
     detail::hpsp_check(hpsp_generate_unweight(&fGen, fGenSeed, 0, n, (uint64_t)seed, out.data(),
-                                              n ? n : 1, nullptr, nullptr, spare_ws(wsb), wsb,
-                                              spare_nacc(), nullptr),
+                                              detail::column_stride(n), nullptr, nullptr,
+                                              spare_ws(wsb), wsb, spare_nacc(), nullptr),
                        "Decays::Unweight");
     fWValid = false;
     return adopt(out, nullptr);
@@ -600,7 +605,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
                        "Decays::Unweight");
     fCols.swap(out);
     if (wout) fW.swap(*wout);
-    fCap = fSize ? fSize : 1;
+    fCap = detail::column_stride(fSize);
     fGenValid = false;
     fLazy = false;
     return Range<iterator>(begin(), begin() + (std::ptrdiff_t)acc);
@@ -625,7 +630,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     if (b.size() < need || b.size() / 2 > need) b.allocate(need);
   }
   detail::device_buffer<double>& spare_cols(size_t n) {
-    fit(fSpare.cols, 4 * N * (n ? n : 1));
+    fit(fSpare.cols, 4 * N * detail::column_stride(n));
     return fSpare.cols;
   }
   detail::device_buffer<double>& spare_w(size_t n) {
@@ -658,7 +663,8 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     auto& out = spare_cols(n);
     const size_t wsb = hpsp_unweight_workspace_size(n);
     detail::hpsp_check(hpsp_unweight_values((int)N, vals.data(), max_value, (uint64_t)seed,
-                                            fCols.data(), fCap, n, out.data(), n ? n : 1,
+                                            fCols.data(), fCap, n, out.data(),
+                                            detail::column_stride(n),
                                             nullptr, nullptr, spare_ws(wsb), wsb, spare_nacc(),
                                             nullptr),
                        "Decays::Unweight");
@@ -675,7 +681,7 @@ class Decays<hydra::tuple<Particles...>, hydra::device::sys_t> {
     const double* win = weights_cache();
     detail::device_buffer<double>* wout = win ? &spare_w(n) : nullptr;
     detail::hpsp_check(hpsp_unweight(&d, f, max_value, (uint64_t)seed, fCols.data(), fCap, n,
-                                     win, out.data(), n ? n : 1, nullptr,
+                                     win, out.data(), detail::column_stride(n), nullptr,
                                      wout ? wout->data() : nullptr, spare_ws(wsb), wsb,
                                      spare_nacc(), nullptr),
                        "Decays::Unweight");

@@ -26,6 +26,15 @@ inline void hpsp_check(int rc, const char* where) {
 // (parallel_for.h:164-166 synchronises after each launch).
 inline void hpsp_sync(const char* where) { hpsp_check(hpsp_stream_synchronize(nullptr), where); }
 
+// Column stride of the device containers: the event count rounded up to a
+// multiple of 64, so that every column starts 512 bytes after the previous
+// one and the kernels' groups of 64 events fill whole cache lines.  With a
+// stride of n itself every column but the first starts mid-line unless n is a
+// multiple of 16, and each group's stores then leave partial lines, which
+// HBM completes by read-modify-write: a 3-body Generate of 2^27 events took
+// 3.0-4.7 ms at a stride of n + 1 against 2.35 ms at n.
+inline size_t column_stride(size_t n) { return n ? (n + 63) / 64 * 64 : 64; }
+
 template <typename T>
 class device_buffer {
  public:

@@ -3,8 +3,9 @@
 // examples/phase_space/phsp_evaluating_functor.inl:204-210).  Each T is a
 // double or a declarg of double.  The device version keeps one FP64 column of
 // ld() values per tuple element in device memory (column j at data() +
-// j * ld()); the host version keeps the same layout in host memory and is
-// constructed from a device one.  Rows are read as tuples by value.
+// j * ld(), ld() = detail::column_stride(size())); the host version keeps
+// its columns size() apart in host memory and is constructed from a device
+// one.  Rows are read as tuples by value.
 #pragma once
 
 #include <cstddef>
@@ -64,10 +65,10 @@ class multivector<hydra::tuple<T...>, hydra::device::sys_t> {
   static constexpr size_t ncols = sizeof...(T);
 
   multivector() = default;
-  explicit multivector(size_t n) : n_(n), buf_(ncols * (n ? n : 1)) {}
+  explicit multivector(size_t n) : n_(n), buf_(ncols * detail::column_stride(n)) {}
 
   size_t size() const { return n_; }
-  size_t ld() const { return n_ ? n_ : 1; }
+  size_t ld() const { return detail::column_stride(n_); }
   double* data() { return buf_.data(); }
   const double* data() const { return buf_.data(); }
   double* column(size_t j) { return buf_.data() + j * ld(); }
@@ -99,11 +100,12 @@ class multivector<hydra::tuple<T...>, hydra::host::sys_t> {
   explicit multivector(size_t n = 0) : n_(n), data_(ncols * (n ? n : 1)) {}
   // copy from the device (one transfer of all columns)
   explicit multivector(multivector<hydra::tuple<T...>, hydra::device::sys_t> const& d)
-      : n_(d.size()), data_(ncols * d.ld()) {
+      : n_(d.size()), data_(ncols * (d.size() ? d.size() : 1)) {
     if (n_)
-      detail::hpsp_check(
-          hpsp_memcpy_to_host(data_.data(), d.data(), data_.size() * sizeof(double)),
-          "multivector");
+      detail::hpsp_check(hpsp_memcpy2d_to_host(data_.data(), n_ * sizeof(double), d.data(),
+                                               d.ld() * sizeof(double), n_ * sizeof(double),
+                                               ncols),
+                         "multivector");
   }
 
   size_t size() const { return n_; }

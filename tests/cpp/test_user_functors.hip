@@ -58,9 +58,18 @@ std::vector<double> host_column(const double* dev, size_t n) {
   return h;
 }
 
+// The columns of a device multivector, size() values each (its columns are
+// ld() apart on the device)
 template <typename MV>
 std::vector<double> host_all(const MV& mv) {
-  return host_column(mv.data(), MV::ncols * mv.ld());
+  const size_t n = mv.size();
+  std::vector<double> h(MV::ncols * n);
+  if (n)
+    hydra::detail::hpsp_check(hpsp_memcpy2d_to_host(h.data(), n * sizeof(double), mv.data(),
+                                                    mv.ld() * sizeof(double), n * sizeof(double),
+                                                    MV::ncols),
+                              "copy");
+  return h;
 }
 
 // |sum_k c_k / (M_k^2 - s_k - i M_k G_k)|^2 with the operations of the
