@@ -90,7 +90,7 @@ class Completion(ctypes.Structure):
     _fields_ = [("ticket", ctypes.c_uint64), ("user_tag", ctypes.c_uint64),
                 ("status", ctypes.c_int32), ("kind", ctypes.c_int32), ("stats", Stats),
                 ("cols", ctypes.c_void_p), ("weights", ctypes.c_void_p),
-                ("n", ctypes.c_uint64), ("submit_s", ctypes.c_double),
+                ("n", ctypes.c_uint64), ("ld", ctypes.c_uint64), ("submit_s", ctypes.c_double),
                 ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
 
 

@@ -75,6 +75,9 @@ void verify(const Case& c, const hpsp_completion& got) {
   const hpsp_request& r = c.r;
   CHECK(got.status == HPSP_OK, "tag %llu status %d", (unsigned long long)r.user_tag, got.status);
   CHECK(got.n == r.n && got.kind == r.kind, "tag %llu header", (unsigned long long)r.user_tag);
+  CHECK(r.kind != HPSP_REQ_GENERATE || (got.ld >= r.n && got.ld % 64 == 0 && got.ld > 0),
+        "tag %llu column stride %llu", (unsigned long long)r.user_tag,
+        (unsigned long long)got.ld);
   CHECK(got.submit_s <= got.admit_s && got.admit_s <= got.done_s, "tag %llu times",
         (unsigned long long)r.user_tag);
   if (r.kind == HPSP_REQ_AVERAGE) {
@@ -93,11 +96,11 @@ void verify(const Case& c, const hpsp_completion& got) {
     for (int col = 0; col < 4 * c.N; ++col) {
       const uint64_t v = fake_event_word(c.key, evt, col, r.prec);
       if (r.out_f32) {
-        const uint32_t g = static_cast<const uint32_t*>(got.cols)[col * r.n + i];
+        const uint32_t g = static_cast<const uint32_t*>(got.cols)[col * got.ld + i];
         CHECK(g == (uint32_t)v, "tag %llu event %llu col %d", (unsigned long long)r.user_tag,
               (unsigned long long)i, col);
       } else {
-        const uint64_t g = static_cast<const uint64_t*>(got.cols)[col * r.n + i];
+        const uint64_t g = static_cast<const uint64_t*>(got.cols)[col * got.ld + i];
         CHECK(g == v, "tag %llu event %llu col %d", (unsigned long long)r.user_tag,
               (unsigned long long)i, col);
       }

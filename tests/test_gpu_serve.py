@@ -68,7 +68,10 @@ def served(c, d):
     if d["kind"] == capi.REQ_AVERAGE:
         return np.array([c.stats.w, c.stats.mean, c.stats.m2])
     esz = 4 if d["out_f32"] else 8
-    cols = host_copy(c.cols, d["n"] * 4 * d["N"] * esz)
+    # the served columns are c.ld apart: keep the first n values of each
+    assert c.ld >= d["n"] and c.ld % 64 == 0
+    cols = host_copy(c.cols, c.ld * 4 * d["N"] * esz)
+    cols = cols.reshape(4 * d["N"], c.ld * esz)[:, :d["n"] * esz].reshape(-1)
     w = None
     if d["want_weights"] and d["n"]:
         w = host_copy(c.weights, d["n"] * 8).view(np.float64)
