@@ -215,6 +215,49 @@ int main(int argc, char** argv) {
           same6 = 0;
       std::printf("\"unweight_after_write\": [%d, %d],\n", cache_before, same6);
     }
+    // a size that is not a multiple of 64: the container's columns are
+    // ld() = 1024 apart, and Generate and Unweight() give the bits of the C
+    // ABI calls on columns exactly n apart
+    {
+      const size_t n9 = 1000;
+      hpsp_decay d9;
+      hydra::detail::hpsp_check(hpsp_decay_init(&d9, 3, P_mass, masses, HPSP_SQUARES3), "init");
+      hydra::detail::device_buffer<double> direct(12 * n9), direct_uw(12 * n9);
+      hydra::detail::device_buffer<uint64_t> nacc9(1);
+      const size_t wsb = hpsp_generate_unweight_workspace_size(n9);
+      hydra::detail::device_buffer<unsigned char> ws9(wsb);
+      hydra::detail::hpsp_check(hpsp_generate(&d9, 159753, 0, n9, HPSP_FP64, direct.data(), n9,
+                                              0, nullptr, nullptr, nullptr),
+                                "hpsp_generate");
+      hydra::detail::hpsp_check(
+          hpsp_generate_unweight(&d9, 159753, 0, n9, 0x180ec6d33cfd0abaULL, direct_uw.data(), n9,
+                                 nullptr, nullptr, ws9.data(), wsb, nacc9.data(), nullptr),
+          "hpsp_generate_unweight");
+      std::vector<double> want(12 * n9), want_uw(12 * n9);
+      uint64_t want_n = 0;
+      hydra::detail::hpsp_check(hpsp_memcpy_to_host(want.data(), direct.data(), 12 * n9 * 8), "copy");
+      hydra::detail::hpsp_check(hpsp_memcpy_to_host(want_uw.data(), direct_uw.data(), 12 * n9 * 8),
+                                "copy");
+      hydra::detail::hpsp_check(hpsp_memcpy_to_host(&want_n, nacc9.data(), 8), "copy");
+      hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> E9(P_mass, masses, n9);
+      const int stride_ok = (E9.ld() % 64 == 0 && E9.ld() >= n9 && E9.capacity() == E9.ld()) ? 1 : 0;
+      phsp.Generate(Parent, E9);
+      hydra::Decays<hydra::tuple<A, B, C>, hydra::host::sys_t> h9(E9);
+      int same_gen = 1;
+      for (size_t c = 0; c < 12; ++c)
+        if (std::memcmp(h9.data() + c * h9.ld(), want.data() + c * n9, n9 * sizeof(double)) != 0)
+          same_gen = 0;
+      hydra::Decays<hydra::tuple<A, B, C>, hydra::device::sys_t> U9(P_mass, masses, n9);
+      phsp.Generate(Parent, U9);
+      auto acc9 = U9.Unweight();
+      hydra::Decays<hydra::tuple<A, B, C>, hydra::host::sys_t> hu9(U9);
+      int same_uw = (acc9.size() == want_n && U9.ld() % 64 == 0) ? 1 : 0;
+      for (size_t c = 0; c < 12 && same_uw; ++c)
+        if (std::memcmp(hu9.data() + c * hu9.ld(), want_uw.data() + c * n9, n9 * sizeof(double)) !=
+            0)
+          same_uw = 0;
+      std::printf("\"padded_stride\": [%d, %d, %d],\n", stride_ok, same_gen, same_uw);
+    }
     // the opt-in deferred-path check: a write through a column pointer taken
     // before Generate is caught at Unweight(); an untouched container passes
     {

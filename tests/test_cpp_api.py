@@ -71,6 +71,9 @@ def test_gpu_generate_unweight_average(golden, oracle):
     # momenta written through a column pointer: the kept weights are dropped
     # and Unweight() flags with the weights of the new momenta
     assert r["unweight_after_write"] == [0, 1]
+    # 1000 events: columns 1024 apart in the container, the bits of the C
+    # ABI on columns 1000 apart (Generate, and Generate + Unweight())
+    assert r["padded_stride"] == [1, 1, 1]
     # opt-in check of the deferred path: a stale-pointer write is caught
     assert r["deferred_checks"] == [1, 1, 1]
     # Generate + Unweight() cycles on one container reuse its buffers and give
